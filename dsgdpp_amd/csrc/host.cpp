@@ -1,0 +1,1104 @@
+// Host side of dsgdpp_amd: the C ABI (include/dsgdpp.h), functor-string
+// parsing, d x 2d blocking into the device layout, the per-epoch stratum
+// schedule and task streams, the decay/loss/trace run loop, and Matrix Market
+// I/O.  Device work is in kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/dsgdpp.h"
+#include "internal.h"
+#include "philox.h"
+
+using namespace dsgdpp;
+
+namespace {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPC(call)                                                                      \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw Error(DSGDPP_E_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                             " at " #call);                             \
+    } while (0)
+
+int hostThreads() {
+    unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(h, 16u));
+}
+
+template <typename F>
+void parallelFor(int64_t n, F f) {
+    int th = (int)std::min<int64_t>(hostThreads(), std::max<int64_t>(n, 1));
+    if (th <= 1) { f(0, n); return; }
+    std::vector<std::thread> ts;
+    for (int k = 0; k < th; ++k) ts.emplace_back(f, n * k / th, n * (k + 1) / th);
+    for (auto& t : ts) t.join();
+}
+
+// ---------------------------------------------------------------------------
+// functor strings: "Name(a,b,..)" or "Name" (tools/parse.h:39-61)
+// ---------------------------------------------------------------------------
+bool splitFunctor(const std::string& s, std::string& name, std::vector<double>& args) {
+    args.clear();
+    const size_t open = s.find('(');
+    if (s.empty() || s.back() != ')' || open == std::string::npos) {
+        name = s;
+        return true;
+    }
+    name = s.substr(0, open);
+    const std::string inner = s.substr(open + 1, s.size() - open - 2);
+    std::stringstream ss(inner);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) {
+        char* end = nullptr;
+        const double v = strtod(tok.c_str(), &end);
+        if (end == tok.c_str() || *end != '\0') return false;
+        args.push_back(v);
+    }
+    if (inner.empty() || inner.back() == ',') return false;   // lexical_cast of an empty token fails
+    return true;
+}
+
+// computeDefaultBlockOffsets (mf/matrix/distributed_matrix_impl.cc:24-40):
+// the first size%blocks blocks hold one extra row; out[blocks] = size.
+std::vector<int64_t> defaultOffsets(int64_t size, int64_t blocks) {
+    std::vector<int64_t> off(blocks + 1, 0);
+    const int64_t base = size / blocks, extra = size % blocks;
+    for (int64_t b = 1; b <= blocks; ++b) off[b] = off[b - 1] + base + (b - 1 < extra ? 1 : 0);
+    return off;
+}
+
+// ---------------------------------------------------------------------------
+// master stream (stands in for rg::Random32, see philox.h)
+// ---------------------------------------------------------------------------
+struct Stream {
+    uint32_t k0 = 0, k1 = 0;
+    uint64_t next = 0;
+    uint64_t u64() { return stream_draw(k0, k1, next++); }
+    uint64_t below(uint64_t b) { return bounded(u64(), b); }
+    // rg::shuffle stand-in: Fisher-Yates from the back (n-1 draws)
+    void shuffle(uint32_t* a, uint32_t n) {
+        for (uint32_t i = n; i > 1; --i) std::swap(a[i - 1], a[below(i)]);
+    }
+};
+
+// d x d DSGD schedule, row = sub-epoch, column = task (mf/sgd/dsgd_impl.h:246-325)
+std::vector<uint32_t> dsgdSchedule(uint32_t w, uint32_t t, int order, Stream& rng) {
+    const uint32_t d = w * t;
+    std::vector<uint32_t> s((size_t)d * d);
+    for (uint32_t e = 0; e < d; ++e)
+        for (uint32_t id = 0; id < d; ++id) s[(size_t)e * d + id] = (e + id) % d;
+    auto identity = [](uint32_t k) {
+        std::vector<uint32_t> v(k);
+        std::iota(v.begin(), v.end(), 0u);
+        return v;
+    };
+    if (order == DSGDPP_STRATUM_SEQ) return s;
+    if (order == DSGDPP_STRATUM_RSEQ) {
+        std::vector<uint32_t> relabel = identity(d);
+        rng.shuffle(relabel.data(), d);
+        for (auto& v : s) v = relabel[v];
+    } else if (order == DSGDPP_STRATUM_WR) {
+        for (uint32_t e = 0; e < d; ++e) rng.shuffle(&s[(size_t)e * d], d);
+    } else if (order == DSGDPP_STRATUM_WOR) {
+        // mf::shuffle (mf/matrix/op/shuffle.h:37-49): rows by P_r, then columns by P_c
+        std::vector<uint32_t> rowp = identity(d), colp = identity(d);
+        rng.shuffle(rowp.data(), d);
+        rng.shuffle(colp.data(), d);
+        std::vector<uint32_t> out(s.size());
+        for (uint32_t e = 0; e < d; ++e)
+            for (uint32_t id = 0; id < d; ++id) out[(size_t)e * d + colp[id]] = s[(size_t)rowp[e] * d + id];
+        s.swap(out);
+    } else if (order == DSGDPP_STRATUM_COWOR) {
+        std::vector<uint32_t> relabel = identity(d);
+        rng.shuffle(relabel.data(), d);
+        const std::vector<uint32_t> groups = dsgdSchedule(w, 1, DSGDPP_STRATUM_WOR, rng);
+        for (uint32_t gi = 0; gi < w; ++gi)
+            for (uint32_t gj = 0; gj < w; ++gj) {
+                const uint32_t goff = groups[(size_t)gi * w + gj] * t;
+                const std::vector<uint32_t> local = dsgdSchedule(t, 1, DSGDPP_STRATUM_WOR, rng);
+                for (uint32_t ti = 0; ti < t; ++ti)
+                    for (uint32_t tj = 0; tj < t; ++tj)
+                        s[(size_t)(gi * t + ti) * d + gj * t + tj] = relabel[goff + local[(size_t)ti * t + tj]];
+            }
+    } else {
+        throw Error(DSGDPP_E_ARG, "Invalid stratum order");
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct HostBlocks {
+    std::vector<int64_t> start;   // d*2d+1
+    int64_t nnz = 0;
+    bool loaded = false;
+};
+
+}  // namespace
+
+struct dsgdpp_ctx {
+    std::string err;
+    std::string updateStr, regularizeStr, lossStr, truncateStr;
+    UpdParams upd;
+    double lossLambda = 0;
+    int64_t r = 0;
+    uint32_t w = 1, t = 1, d = 1;
+    int sgdOrder = DSGDPP_SGD_WOR, stratumOrder = DSGDPP_STRATUM_COWOR;
+    uint64_t seed = 0;
+    int precision = DSGDPP_FP64_STRICT, device = 0, kernel = 0;
+    Stream master;
+
+    int64_t m = -1, n = -1;
+    std::vector<int64_t> off1, off2;
+    HostBlocks blocks[2];
+    std::vector<int64_t> nnz1, nnz2;
+    bool haveFactors = false;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    DevProblem P;
+    void* dPts[2] = {nullptr, nullptr};
+    int64_t* dBlk[2] = {nullptr, nullptr};
+    double* dPartials = nullptr;
+    double* hPinned = nullptr;
+    uint32_t* dSched = nullptr;
+    PermJob* dJobs = nullptr;
+    int32_t* dChains = nullptr;
+    size_t jobsCap = 0, chainsCap = 0;
+
+    std::vector<uint32_t> schedule;
+    std::vector<uint64_t> seeds;
+
+    size_t factorBytes() const {
+        return precision <= DSGDPP_FP64 ? 8 : precision == DSGDPP_FP32 ? 4 : 2;
+    }
+    bool fp64() const { return precision <= DSGDPP_FP64; }
+};
+
+namespace {
+
+thread_local std::string g_createError;
+
+template <typename F>
+int guarded(dsgdpp_ctx* c, F f) {
+    if (!c) return DSGDPP_E_ARG;
+    try {
+        HIPC(hipSetDevice(c->device));
+        f();
+        return DSGDPP_OK;
+    } catch (const Error& e) {
+        c->err = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        c->err = e.what();
+        return DSGDPP_E_ARG;
+    } catch (const std::bad_alloc&) {
+        c->err = "out of host memory";
+        return DSGDPP_E_DEVICE;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return DSGDPP_E_DEVICE;
+    }
+}
+
+void dfree(void*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <typename T>
+void dalloc(T*& p, size_t bytes) {
+    void* q = nullptr;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    HIPC(hipMalloc(&q, std::max<size_t>(bytes, 16)));
+    p = (T*)q;
+}
+
+void configure(dsgdpp_ctx* c, const dsgdpp_cfg* cfg) {
+    if (!cfg) throw Error(DSGDPP_E_ARG, "null configuration");
+    if (cfg->abi_version != DSGDPP_ABI_VERSION) throw Error(DSGDPP_E_ARG, "ABI version mismatch");
+    c->updateStr = cfg->update ? cfg->update : "";
+    c->regularizeStr = cfg->regularize ? cfg->regularize : "";
+    c->lossStr = cfg->loss ? cfg->loss : "";
+    c->truncateStr = cfg->truncate ? cfg->truncate : "";
+    std::string un, rn, ln;
+    std::vector<double> ua, ra, la;
+    if (!splitFunctor(c->updateStr, un, ua)) throw Error(DSGDPP_E_ARG, "Invalid argument in " + c->updateStr);
+    if (!splitFunctor(c->regularizeStr, rn, ra)) throw Error(DSGDPP_E_ARG, "Invalid argument in " + c->regularizeStr);
+    if (!splitFunctor(c->lossStr, ln, la)) throw Error(DSGDPP_E_ARG, "Invalid argument in " + c->lossStr);
+    // accepted combinations: tools/mfdsgdpp.cc:162-358
+    const char* combos = "Invalid combination of update, regularize, and loss arguments";
+    if (rn != "None") throw Error(DSGDPP_E_ARG, combos);
+    if (!ra.empty()) throw Error(DSGDPP_E_ARG, "Invalid number of arguments in " + c->regularizeStr);
+    if (un == "Nzsl" && ln == "Nzsl") {
+        if (!ua.empty()) throw Error(DSGDPP_E_ARG, "Invalid number of arguments in " + c->updateStr);
+        if (!la.empty()) throw Error(DSGDPP_E_ARG, "Invalid number of arguments in " + c->lossStr);
+        c->upd.kind = UPD_NZSL;
+    } else if ((un == "Nzsl_L2" && ln == "Nzsl_L2") || (un == "Nzsl_Nzl2" && ln == "Nzsl_Nzl2")) {
+        if (ua.size() != 1) throw Error(DSGDPP_E_ARG, "Invalid number of arguments in " + c->updateStr);
+        if (la.size() != 1) throw Error(DSGDPP_E_ARG, "Invalid number of arguments in " + c->lossStr);
+        c->upd.kind = un == "Nzsl_L2" ? UPD_NZSL_L2 : UPD_NZSL_NZL2;
+        c->upd.lambda = ua[0];
+        c->lossLambda = la[0];
+    } else {
+        throw Error(DSGDPP_E_ARG, combos);
+    }
+    c->upd.truncate = false;
+    if (!c->truncateStr.empty()) {   // parseTruncate (parse.h:115-139)
+        std::string tn;
+        std::vector<double> ta;
+        if (!splitFunctor(c->truncateStr, tn, ta) || !tn.empty() || ta.size() != 2)
+            throw Error(DSGDPP_E_ARG, "Error in argument truncate with options " + c->truncateStr);
+        c->upd.truncate = true;
+        c->upd.tmin = ta[0];
+        c->upd.tmax = ta[1];
+    }
+    if (cfg->rank <= 0) throw Error(DSGDPP_E_ARG, "rank must be positive");
+    if (cfg->rank > 256) throw Error(DSGDPP_E_UNSUPPORTED, "rank above 256 is not supported by the GPU path");
+    if (cfg->world_size <= 0 || cfg->tasks_per_rank <= 0) throw Error(DSGDPP_E_ARG, "invalid blocking");
+    if (cfg->sgd_order < 0 || cfg->sgd_order > 2) throw Error(DSGDPP_E_ARG, "Invalid sgd order");
+    if (cfg->stratum_order < 0 || cfg->stratum_order > 4) throw Error(DSGDPP_E_ARG, "Invalid stratum order");
+    if (cfg->precision < 0 || cfg->precision > 3) throw Error(DSGDPP_E_ARG, "Invalid precision");
+    c->r = cfg->rank;
+    c->w = (uint32_t)cfg->world_size;
+    c->t = (uint32_t)cfg->tasks_per_rank;
+    c->d = c->w * c->t;
+    c->sgdOrder = cfg->sgd_order;
+    c->stratumOrder = cfg->stratum_order;
+    c->seed = cfg->seed;
+    c->precision = cfg->precision;
+    c->device = cfg->device;
+    c->kernel = cfg->kernel;
+    master_key(c->seed, &c->master.k0, &c->master.k1);
+    c->master.next = 0;
+}
+
+// ---------------------------------------------------------------------------
+// blocking: ReadMatrixBlocksMm (read_impl.h:365-486) puts each entry into the
+// block of its (row, column) with block-local indices and sorts every block
+// row-major (read_impl.h:163; ublas sort sums duplicates).  Here blocks are
+// sorted stably, duplicates summed in input order, and the result is written
+// as one array of 16-byte point records, block after block (d x 2d,
+// row-major), with global indices.
+// ---------------------------------------------------------------------------
+template <typename Pt>
+void buildBlocks(dsgdpp_ctx* c, int64_t nnz, const int32_t* I, const int32_t* J, const double* X,
+                 std::vector<Pt>& out, std::vector<int64_t>& start) {
+    const uint32_t d = c->d, d2 = 2 * d;
+    const size_t nb = (size_t)d * d2;
+    std::vector<uint32_t> rb(c->m), cb(c->n);
+    for (uint32_t b = 0; b < d; ++b)
+        for (int64_t i = c->off1[b]; i < c->off1[b + 1]; ++i) rb[i] = b;
+    for (uint32_t b = 0; b < d2; ++b)
+        for (int64_t j = c->off2[b]; j < c->off2[b + 1]; ++j) cb[j] = b;
+    std::vector<int64_t> cnt(nb + 1, 0);
+    for (int64_t p = 0; p < nnz; ++p) cnt[(size_t)rb[I[p]] * d2 + cb[J[p]] + 1]++;
+    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    std::vector<int64_t> order(nnz);
+    {
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int64_t p = 0; p < nnz; ++p) order[pos[(size_t)rb[I[p]] * d2 + cb[J[p]]]++] = p;
+    }
+    std::vector<int64_t> kept(nb, 0);
+    out.resize(nnz);
+    parallelFor((int64_t)nb, [&](int64_t lo, int64_t hi) {
+        for (int64_t b = lo; b < hi; ++b) {
+            int64_t* first = order.data() + cnt[b];
+            int64_t* last = order.data() + cnt[b + 1];
+            std::stable_sort(first, last, [&](int64_t a, int64_t q) {
+                return I[a] != I[q] ? I[a] < I[q] : J[a] < J[q];
+            });
+            Pt* dst = out.data() + cnt[b];
+            int64_t k = 0;
+            for (int64_t* it = first; it != last; ++it) {
+                const int64_t p = *it;
+                if (k > 0 && dst[k - 1].i == I[p] && dst[k - 1].j == J[p]) {
+                    dst[k - 1].x = (decltype(dst[0].x))((double)dst[k - 1].x + X[p]);
+                } else {
+                    Pt e{};
+                    e.i = I[p];
+                    e.j = J[p];
+                    e.x = (decltype(e.x))X[p];
+                    dst[k++] = e;
+                }
+            }
+            kept[b] = k;
+        }
+    });
+    // compact (duplicates removed)
+    start.assign(nb + 1, 0);
+    for (size_t b = 0; b < nb; ++b) start[b + 1] = start[b] + kept[b];
+    if (start[nb] != nnz) {
+        for (size_t b = 0; b < nb; ++b)
+            if (start[b] != cnt[b]) std::copy(out.begin() + cnt[b], out.begin() + cnt[b] + kept[b], out.begin() + start[b]);
+        out.resize(start[nb]);
+    }
+}
+
+template <typename Pt>
+void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const int32_t* J,
+                  const double* X) {
+    std::vector<Pt> pts;
+    HostBlocks& hb = c->blocks[which];
+    buildBlocks<Pt>(c, nnz, I, J, X, pts, hb.start);
+    hb.nnz = (int64_t)pts.size();
+    dalloc(c->dPts[which], pts.size() * sizeof(Pt));
+    HIPC(hipMemcpy(c->dPts[which], pts.data(), pts.size() * sizeof(Pt), hipMemcpyHostToDevice));
+    dalloc(c->dBlk[which], hb.start.size() * sizeof(int64_t));
+    HIPC(hipMemcpy(c->dBlk[which], hb.start.data(), hb.start.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    hb.loaded = true;
+    if (which == DSGDPP_TRAIN) {
+        // nnz12 (mf/matrix/op/nnz.h:34-58)
+        c->nnz1.assign(c->m, 0);
+        c->nnz2.assign(c->n, 0);
+        for (const Pt& e : pts) { c->nnz1[e.i]++; c->nnz2[e.j]++; }
+        std::vector<double> cnt1(c->m), cnt2(c->n);
+        for (int64_t i = 0; i < c->m; ++i) cnt1[i] = (double)c->nnz1[i];
+        for (int64_t j = 0; j < c->n; ++j) cnt2[j] = (double)c->nnz2[j];
+        auto upInv = [&](const std::vector<double>& cntv, void*& dst) {
+            if (c->fp64()) {   // f3 = 1./nnz1[i] (update-nzsl-l2.h:47-48), same rounding as on the host
+                std::vector<double> inv(cntv.size());
+                for (size_t k = 0; k < cntv.size(); ++k) inv[k] = 1. / cntv[k];
+                dalloc(dst, inv.size() * sizeof(double));
+                HIPC(hipMemcpy(dst, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> inv(cntv.size());
+                for (size_t k = 0; k < cntv.size(); ++k) inv[k] = (float)(1. / cntv[k]);
+                dalloc(dst, inv.size() * sizeof(float));
+                HIPC(hipMemcpy(dst, inv.data(), inv.size() * sizeof(float), hipMemcpyHostToDevice));
+            }
+        };
+        void* inv1 = const_cast<void*>(c->P.inv1);
+        void* inv2 = const_cast<void*>(c->P.inv2);
+        upInv(cnt1, inv1);
+        upInv(cnt2, inv2);
+        c->P.inv1 = inv1;
+        c->P.inv2 = inv2;
+        double* dc1 = const_cast<double*>(c->P.cnt1);
+        double* dc2 = const_cast<double*>(c->P.cnt2);
+        dalloc(dc1, cnt1.size() * sizeof(double));
+        dalloc(dc2, cnt2.size() * sizeof(double));
+        HIPC(hipMemcpy(dc1, cnt1.data(), cnt1.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(dc2, cnt2.data(), cnt2.size() * sizeof(double), hipMemcpyHostToDevice));
+        c->P.cnt1 = dc1;
+        c->P.cnt2 = dc2;
+        dalloc(c->P.perm, (size_t)hb.nnz * sizeof(uint32_t));
+        c->P.pts = c->dPts[0];
+        c->P.blk = c->dBlk[0];
+        c->P.nnz = hb.nnz;
+    } else {
+        c->P.test_pts = c->dPts[1];
+        c->P.test_blk = c->dBlk[1];
+        c->P.test_nnz = hb.nnz;
+    }
+}
+
+void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const int32_t* I,
+             const int32_t* J, const double* X) {
+    if (which != DSGDPP_TRAIN && which != DSGDPP_TEST) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+    if (m <= 0 || n <= 0 || nnz < 0) throw Error(DSGDPP_E_ARG, "invalid matrix dimensions");
+    if (m >= INT32_MAX || n >= INT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "dimensions must fit in int32");
+    if (nnz > 0 && (!I || !J || !X)) throw Error(DSGDPP_E_ARG, "null index or value array");
+    if (c->m >= 0 && (c->m != m || c->n != n)) throw Error(DSGDPP_E_ARG, "matrix dimensions differ from the loaded data");
+    if (m < (int64_t)c->d || n < (int64_t)(2 * c->d)) throw Error(DSGDPP_E_ARG, "matrix is smaller than the d x 2d block grid");
+    for (int64_t p = 0; p < nnz; ++p)
+        if (I[p] < 0 || I[p] >= m || J[p] < 0 || J[p] >= n)
+            throw Error(DSGDPP_E_ARG, "index out of range at entry " + std::to_string(p));
+    if (which == DSGDPP_TRAIN && nnz >= (int64_t)UINT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "too many nonzeros");
+    const bool first = c->m < 0;
+    c->m = m;
+    c->n = n;
+    c->off1 = defaultOffsets(m, c->d);
+    c->off2 = defaultOffsets(n, 2 * c->d);
+    if (first) {
+        c->P.m = m;
+        c->P.n = n;
+        c->P.r = c->r;
+        c->P.d = (int)c->d;
+        c->P.precision = c->precision;
+        const int64_t vec = 16 / (int64_t)c->factorBytes();
+        c->P.ld = (c->r + vec - 1) / vec * vec;
+    }
+    if (c->fp64()) uploadBlocks<Pt64>(c, which, nnz, I, J, X);
+    else uploadBlocks<Pt32>(c, which, nnz, I, J, X);
+}
+
+// factors: host doubles <-> device storage (W row-major, H column-major r x n)
+uint16_t toBf16(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+float fromBf16(uint16_t u) {
+    uint32_t b = (uint32_t)u << 16;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+void putFactor(dsgdpp_ctx* c, void*& dst, const double* src, int64_t rows) {
+    const int64_t r = c->r, ld = c->P.ld;
+    const size_t es = c->factorBytes();
+    std::vector<uint8_t> buf((size_t)rows * ld * es, 0);
+    for (int64_t i = 0; i < rows; ++i)
+        for (int64_t z = 0; z < r; ++z) {
+            const double v = src[i * r + z];
+            uint8_t* p = buf.data() + ((size_t)i * ld + z) * es;
+            if (es == 8) memcpy(p, &v, 8);
+            else if (es == 4) { const float f = (float)v; memcpy(p, &f, 4); }
+            else { const uint16_t h = toBf16((float)v); memcpy(p, &h, 2); }
+        }
+    if (!dst) dalloc(dst, buf.size());
+    HIPC(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+}
+
+void getFactor(dsgdpp_ctx* c, const void* src, double* dst, int64_t rows) {
+    const int64_t r = c->r, ld = c->P.ld;
+    const size_t es = c->factorBytes();
+    std::vector<uint8_t> buf((size_t)rows * ld * es);
+    HIPC(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < rows; ++i)
+        for (int64_t z = 0; z < r; ++z) {
+            const uint8_t* p = buf.data() + ((size_t)i * ld + z) * es;
+            double v;
+            if (es == 8) memcpy(&v, p, 8);
+            else if (es == 4) { float f; memcpy(&f, p, 4); v = f; }
+            else { uint16_t h; memcpy(&h, p, 2); v = fromBf16(h); }
+            dst[i * r + z] = v;
+        }
+}
+
+void requireReady(dsgdpp_ctx* c) {
+    if (!c->blocks[0].loaded) throw Error(DSGDPP_E_STATE, "load the training matrix first");
+    if (!c->haveFactors) throw Error(DSGDPP_E_STATE, "set the factors first");
+}
+
+// ---------------------------------------------------------------------------
+// one epoch: DsgdPpRunner::epoch (mf/sgd/dsgdpp_impl.h:282-349)
+// ---------------------------------------------------------------------------
+void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
+    requireReady(c);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t d = c->d, d2 = 2 * d;
+    Stream& rng = c->master;
+    // column-block halves for even / odd sub-epochs (:289-323)
+    std::vector<uint32_t> even(d), odd(d);
+    if (c->stratumOrder == DSGDPP_STRATUM_SEQ) {
+        for (uint32_t b = 0; b < d; ++b) { even[b] = 2 * b; odd[b] = 2 * b + 1; }
+    } else {
+        // rg::sample(d, 2d) stand-in: sorted, by selection sampling
+        std::vector<uint8_t> chosen(d2, 0);
+        uint32_t need = d;
+        for (uint32_t b = 0; b < d2 && need > 0; ++b)
+            if (rng.below(d2 - b) < need) { chosen[b] = 1; --need; }
+        uint32_t e = 0, o = 0;
+        for (uint32_t b = 0; b < d2; ++b) (chosen[b] ? even[e++] : odd[o++]) = b;
+    }
+    // two d x d schedules interleaved into 2d x d (:327-336)
+    const std::vector<uint32_t> s0 = dsgdSchedule(c->w, c->t, c->stratumOrder, rng);
+    const std::vector<uint32_t> s1 = dsgdSchedule(c->w, c->t, c->stratumOrder, rng);
+    c->schedule.assign((size_t)d2 * d, 0);
+    for (uint32_t i = 0; i < d; ++i)
+        for (uint32_t id = 0; id < d; ++id) {
+            c->schedule[(size_t)(2 * i) * d + id] = even[s0[(size_t)i * d + id]];
+            c->schedule[(size_t)(2 * i + 1) * d + id] = odd[s1[(size_t)i * d + id]];
+        }
+    // task seeds (mpi2::seed, :344): one master draw per task, in task order
+    c->seeds.resize(d);
+    for (uint32_t id = 0; id < d; ++id) c->seeds[id] = rng.u64();
+
+    // visit-order work: each task's stream is consumed block after block in
+    // sub-epoch order (one SgdRunner per task per epoch, :112)
+    const std::vector<int64_t>& bs = c->blocks[0].start;
+    std::vector<PermJob> jobs;
+    std::vector<int32_t> chains;
+    int64_t maxBlock = 0;
+    if (c->sgdOrder != DSGDPP_SGD_SEQ) {
+        std::vector<std::vector<PermJob>> perTask(d);
+        parallelFor(d, [&](int64_t lo, int64_t hi) {
+            for (int64_t id = lo; id < hi; ++id) {
+                uint32_t k0, k1;
+                task_key(c->seeds[id], &k0, &k1);
+                uint64_t draw = 0;
+                int64_t lastN = 0, lastBase = -1;
+                for (uint32_t s = 0; s < d2; ++s) {
+                    const size_t b = (size_t)id * d2 + c->schedule[(size_t)s * d + id];
+                    const int64_t base = bs[b], nb = bs[b + 1] - bs[b];
+                    if (c->sgdOrder == DSGDPP_SGD_WR) {
+                        if (nb > 0) perTask[id].push_back(PermJob{base, nb, -1, draw, k0, k1});
+                        draw += (uint64_t)nb + 2;
+                    } else {
+                        if (nb == 0) continue;
+                        const int64_t carry = (nb == lastN) ? lastBase : -1;
+                        perTask[id].push_back(PermJob{base, nb, carry, draw, k0, k1});
+                        draw += (uint64_t)nb;
+                        lastN = nb;
+                        lastBase = base;
+                    }
+                }
+            }
+        });
+        for (uint32_t id = 0; id < d; ++id)
+            for (const PermJob& j : perTask[id]) {
+                if (c->sgdOrder == DSGDPP_SGD_WOR && j.carry_base < 0) chains.push_back((int32_t)jobs.size());
+                jobs.push_back(j);
+                maxBlock = std::max(maxBlock, j.n);
+            }
+        chains.push_back((int32_t)jobs.size());
+    }
+    if (jobs.size() > c->jobsCap) {
+        c->jobsCap = jobs.size() * 2;
+        dalloc(c->dJobs, c->jobsCap * sizeof(PermJob));
+    }
+    if (chains.size() > c->chainsCap) {
+        c->chainsCap = chains.size() * 2;
+        dalloc(c->dChains, c->chainsCap * sizeof(int32_t));
+    }
+    if (!c->dSched) dalloc(c->dSched, (size_t)d2 * d * sizeof(uint32_t));
+    HIPC(hipMemcpyAsync(c->dSched, c->schedule.data(), c->schedule.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, c->stream));
+    if (!jobs.empty())
+        HIPC(hipMemcpyAsync(c->dJobs, jobs.data(), jobs.size() * sizeof(PermJob), hipMemcpyHostToDevice, c->stream));
+    if (!chains.empty())
+        HIPC(hipMemcpyAsync(c->dChains, chains.data(), chains.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+
+    HIPC(hipEventRecord(c->ev[0], c->stream));
+    if (c->sgdOrder == DSGDPP_SGD_WOR)
+        launch_permute_wor(c->dJobs, c->dChains, (int)chains.size() - 1, c->P.perm, maxBlock, c->stream);
+    else if (c->sgdOrder == DSGDPP_SGD_WR)
+        launch_visit_wr(c->dJobs, (int)jobs.size(), c->P.perm, c->stream);
+    HIPC(hipEventRecord(c->ev[1], c->stream));
+    for (uint32_t s = 0; s < d2; ++s)
+        launch_stratum(c->P, c->upd, c->dSched + (size_t)s * d, c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1, eps, c->stream);
+    HIPC(hipEventRecord(c->ev[2], c->stream));
+    HIPC(hipEventSynchronize(c->ev[2]));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        float a = 0, b = 0;
+        HIPC(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIPC(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->perm_seconds = a / 1e3;
+        st->sgd_seconds = b / 1e3;
+        st->device_seconds = (a + b) / 1e3;
+        st->updates = c->blocks[0].nnz;
+        st->launches = d2;
+    }
+}
+
+double trainLoss(dsgdpp_ctx* c) {
+    requireReady(c);
+    double l = device_nzsl(c->P, 0, c->dPartials, c->hPinned, c->stream);
+    if (c->upd.kind == UPD_NZSL_L2) {
+        const double a = device_sumsq(c->P, 0, nullptr, 0, c->m, c->dPartials, c->hPinned, c->stream);
+        const double b = device_sumsq(c->P, 1, nullptr, 0, c->n, c->dPartials, c->hPinned, c->stream);
+        l = l + c->lossLambda * (a + b);   // SumLoss + L2Loss (loss.h:99, l2.h:165-167)
+    } else if (c->upd.kind == UPD_NZSL_NZL2) {
+        const double a = device_sumsq(c->P, 0, c->P.cnt1, 0, c->m, c->dPartials, c->hPinned, c->stream);
+        const double b = device_sumsq(c->P, 1, c->P.cnt2, 0, c->n, c->dPartials, c->hPinned, c->stream);
+        l = l + c->lossLambda * (a + b);
+    }
+    return l;
+}
+
+double testLoss(dsgdpp_ctx* c) {
+    if (!c->blocks[1].loaded) throw Error(DSGDPP_E_STATE, "no test matrix loaded");
+    if (!c->haveFactors) throw Error(DSGDPP_E_STATE, "set the factors first");
+    return device_nzsl(c->P, 1, c->dPartials, c->hPinned, c->stream);
+}
+
+// balanceSimple (mf/matrix/op/balance_impl.cc:102-127)
+void balance(dsgdpp_ctx* c, int type) {
+    if (type == DSGDPP_BALANCE_NONE) return;
+    if (type != DSGDPP_BALANCE_L2 && type != DSGDPP_BALANCE_NZL2) throw Error(DSGDPP_E_ARG, "invalid balance type");
+    requireReady(c);
+    const double* w1 = type == DSGDPP_BALANCE_NZL2 ? c->P.cnt1 : nullptr;
+    const double* w2 = type == DSGDPP_BALANCE_NZL2 ? c->P.cnt2 : nullptr;
+    const double regW = device_sumsq(c->P, 0, w1, 0, c->m, c->dPartials, c->hPinned, c->stream);
+    const double regH = device_sumsq(c->P, 1, w2, 0, c->n, c->dPartials, c->hPinned, c->stream);
+    double wf = std::sqrt(std::sqrt(regH / regW)), hf;
+    if (std::isnan(wf)) wf = hf = 0;
+    else hf = 1. / wf;
+    device_scale(c->P, 0, wf, c->stream);
+    device_scale(c->P, 1, hf, c->stream);
+    HIPC(hipStreamSynchronize(c->stream));
+}
+
+// ---------------------------------------------------------------------------
+// Matrix Market (read_impl.h:46-256, write_impl.h:94-118)
+// ---------------------------------------------------------------------------
+std::string readAll(const std::string& fn) {
+    std::ifstream in(fn, std::ios::binary);
+    if (!in.is_open()) throw Error(DSGDPP_E_IO, "Cannot open file " + fn);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return ss.str();
+}
+
+struct LineReader {
+    const std::string& s;
+    size_t pos = 0;
+    int64_t lineNo = 0;
+    explicit LineReader(const std::string& t) : s(t) {}
+    bool next(const char*& b, const char*& e) {
+        if (pos >= s.size()) return false;
+        size_t nl = s.find('\n', pos);
+        if (nl == std::string::npos) nl = s.size();
+        b = s.data() + pos;
+        e = s.data() + nl;
+        pos = nl + 1;
+        ++lineNo;
+        return true;
+    }
+};
+
+bool onlySpace(const char* b, const char* e) {
+    for (; b < e; ++b)
+        if (!isspace((unsigned char)*b)) return false;
+    return true;
+}
+
+void readHeader(LineReader& lr, const std::string& fn, const char* banner, std::string& dims) {
+    const char *b, *e;
+    if (!lr.next(b, e)) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
+    std::string first(b, e);
+    while (!first.empty() && isspace((unsigned char)first.back())) first.pop_back();
+    if (first != banner) throw Error(DSGDPP_E_IO, "Wrong matrix-market banner in file " + fn + ": " + std::string(b, e));
+    while (true) {
+        if (!lr.next(b, e)) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
+        if (b < e && *b == '%') continue;
+        dims.assign(b, e);
+        return;
+    }
+}
+
+void loadMmc(dsgdpp_ctx* c, int which, const std::string& fn) {
+    const std::string text = readAll(fn);
+    LineReader lr(text);
+    std::string dims;
+    readHeader(lr, fn, "%%MatrixMarket matrix coordinate real general", dims);
+    long long m, n, nnz;
+    char junk[2];
+    if (sscanf(dims.c_str(), "%lld %lld %lld %1s", &m, &n, &nnz, junk) != 3 || m <= 0 || n <= 0 || nnz < 0)
+        throw Error(DSGDPP_E_IO, "Invalid matrix dimensions in file " + fn + ": " + dims);
+    std::vector<int32_t> I(nnz), J(nnz);
+    std::vector<double> X(nnz);
+    std::string line;
+    for (long long p = 0; p < nnz; ++p) {
+        const char *b, *e;
+        if (!lr.next(b, e)) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
+        line.assign(b, e);
+        const char* s = line.c_str();
+        char* q = nullptr;
+        const unsigned long i = strtoul(s, &q, 10);
+        if (q == s) throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        s = q;
+        const unsigned long j = strtoul(s, &q, 10);
+        if (q == s) throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        s = q;
+        const double x = strtod(s, &q);
+        if (q == s || !onlySpace(q, line.c_str() + line.size()))
+            throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        if (i < 1 || j < 1 || (long long)i > m || (long long)j > n)
+            throw Error(DSGDPP_E_IO, "Index out of range at line " + std::to_string(lr.lineNo) + " of " + fn);
+        I[p] = (int32_t)(i - 1);
+        J[p] = (int32_t)(j - 1);
+        X[p] = x;
+    }
+    const char *b, *e;
+    while (lr.next(b, e))
+        if (!onlySpace(b, e)) throw Error(DSGDPP_E_IO, "Unexpected input at line " + std::to_string(lr.lineNo) + " of " + fn);
+    loadCoo(c, which, m, n, nnz, I.data(), J.data(), X.data());
+}
+
+// column-major values of a "matrix array" file
+void readMma(const std::string& fn, int64_t& rows, int64_t& cols, std::vector<double>& cm) {
+    const std::string text = readAll(fn);
+    LineReader lr(text);
+    std::string dims;
+    readHeader(lr, fn, "%%MatrixMarket matrix array real general", dims);
+    long long a, b2;
+    char junk[2];
+    if (sscanf(dims.c_str(), "%lld %lld %1s", &a, &b2, junk) != 2 || a <= 0 || b2 <= 0)
+        throw Error(DSGDPP_E_IO, "Invalid matrix dimensions in file " + fn + ": " + dims);
+    rows = a;
+    cols = b2;
+    cm.resize((size_t)a * b2);
+    std::string line;
+    for (size_t p = 0; p < cm.size(); ++p) {
+        const char *b, *e;
+        if (!lr.next(b, e)) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
+        line.assign(b, e);
+        char* q = nullptr;
+        const double x = strtod(line.c_str(), &q);
+        if (q == line.c_str() || !onlySpace(q, line.c_str() + line.size()))
+            throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        cm[p] = x;
+    }
+    const char *b, *e;
+    while (lr.next(b, e))
+        if (!onlySpace(b, e)) throw Error(DSGDPP_E_IO, "Unexpected input at line " + std::to_string(lr.lineNo) + " of " + fn);
+}
+
+void writeMma(const std::string& fn, int64_t rows, int64_t cols, const std::vector<double>& cm) {
+    std::ofstream out(fn);
+    if (!out.is_open()) throw Error(DSGDPP_E_IO, "Cannot open file " + fn);
+    out.precision(16);   // digits10 + 1 (write_impl.h:99)
+    out << "%%MatrixMarket matrix array real general\n"
+        << "% First line: ROWS COLUMNS\n"
+        << "% Subsequent lines: entries in column-major order\n"
+        << rows << " " << cols << "\n";
+    for (double v : cm) out << v << "\n";
+    if (!out) throw Error(DSGDPP_E_IO, "Cannot write file " + fn);
+}
+
+// ---------------------------------------------------------------------------
+// R trace (mf/trace.h:46-262)
+// ---------------------------------------------------------------------------
+struct TraceEntry {
+    int64_t it;
+    double loss, test, eps, tEps, tEpoch, tLoss, tTest;   // times in ns
+};
+
+std::string formatTrace(const std::vector<TraceEntry>& es, const std::string& var,
+                        const std::map<std::string, double>& nums,
+                        const std::map<std::string, std::string>& strs) {
+    std::stringstream o;
+    o << var << " <- list(trace=list(";
+    double elapsed = 0;
+    for (size_t k = 0; k < es.size(); ++k) {
+        const TraceEntry& e = es[k];
+        o << (k ? ", " : "") << std::endl;
+        o << "list(it=" << e.it << ", epoch=" << e.it << ", loss=" << e.loss << ", loss.test=";
+        if (std::isnan(e.test)) o << "NA, ";
+        else o << e.test << ", ";
+        elapsed += (e.tEps + e.tEpoch + e.tLoss + e.tTest) / 1E9;
+        o << "time=list(elapsed=" << elapsed << ", eps=" << e.tEps / 1E9 << ", epoch=" << e.tEpoch / 1E9
+          << ", loss=" << e.tLoss / 1E9 << ", loss.test=" << e.tTest / 1E9 << "),  eps=";
+        if (std::isnan(e.eps)) o << "NA";
+        else o << e.eps;
+        o << ")";
+    }
+    o << std::endl << ")";
+    if (!nums.empty() || !strs.empty()) {
+        o << ", info=list(";
+        bool any = false;
+        for (const auto& kv : nums) { o << (any ? ", " : "") << kv.first << " = " << kv.second; any = true; }
+        for (const auto& kv : strs) { o << (any ? ", " : "") << kv.first << " = \"" << kv.second << "\""; any = true; }
+        o << ")";
+    }
+    o << ");";
+    return o.str();
+}
+
+void run(dsgdpp_ctx* c, const dsgdpp_run_cfg* rc) {
+    if (!rc) throw Error(DSGDPP_E_ARG, "null run configuration");
+    requireReady(c);
+    std::string dn;
+    std::vector<double> da;
+    if (!splitFunctor(rc->decay ? rc->decay : "", dn, da)) throw Error(DSGDPP_E_ARG, "invalid decay");
+    bool bold = false;
+    double eps0 = 0, dec = 0.5, inc = 1.05;
+    if (dn == "BoldDriver" && (da.size() == 1 || da.size() == 3)) {   // parse.h:171-198
+        bold = true;
+        eps0 = da[0];
+        if (da.size() == 3) { dec = da[1]; inc = da[2]; }
+    } else if (dn == "Const" && da.size() == 1) {
+        eps0 = da[0];
+    } else {
+        throw Error(DSGDPP_E_ARG, "Error in argument 'decay': Unknown or invalid decay '" + std::string(rc->decay ? rc->decay : "") + "'");
+    }
+    if (rc->epochs < 0) throw Error(DSGDPP_E_ARG, "negative epoch count");
+    using clk = std::chrono::steady_clock;
+    auto ns = [](clk::time_point a, clk::time_point b) {
+        return (double)std::chrono::duration_cast<std::chrono::nanoseconds>(b - a).count();
+    };
+    const bool verbose = rc->verbose != 0;
+    const bool haveTest = c->blocks[1].loaded;
+    std::vector<TraceEntry> es;
+    // defaultRunner (mf/sgd/sgd_impl.h:34-145)
+    auto a = clk::now();
+    double cur = trainLoss(c);
+    auto b = clk::now();
+    double tl = ns(a, b);
+    if (verbose) printf("Loss: %g (%.6fs)\n", cur, tl / 1e9);
+    double test = NAN, tt = 0;
+    if (haveTest) {
+        a = clk::now(); test = testLoss(c); b = clk::now(); tt = ns(a, b);
+        if (verbose) printf("Test loss: %g (%.6fs)\n", test, tt / 1e9);
+    }
+    es.push_back({0, cur, test, NAN, 0, 0, tl, tt});
+    double prev = 0, eps = eps0;
+    for (int64_t ep = 0; ep < rc->epochs; ++ep) {
+        a = clk::now();
+        if (ep > 0 && bold) eps = (prev <= cur) ? eps * dec : eps * inc;   // decay_bolddriver.h:59-67
+        b = clk::now();
+        const double te = ns(a, b);
+        if (verbose) printf("Step size: %g\nStarting epoch %lld\n", eps, (long long)ep + 1);
+        a = clk::now();
+        epoch(c, eps, nullptr);
+        b = clk::now();
+        const double tep = ns(a, b);
+        if (verbose) printf("Finished epoch %lld (%.6fs)\n", (long long)ep + 1, tep / 1e9);
+        balance(c, rc->balance);
+        prev = cur;
+        a = clk::now(); cur = trainLoss(c); b = clk::now(); tl = ns(a, b);
+        if (verbose) printf("Loss: %g (%.6fs)\n", cur, tl / 1e9);
+        test = NAN;
+        tt = 0;
+        if (haveTest) {
+            a = clk::now(); test = testLoss(c); b = clk::now(); tt = ns(a, b);
+            if (verbose) printf("Test loss: %g (%.6fs)\n", test, tt / 1e9);
+        }
+        es.push_back({ep + 1, cur, test, eps, te, tep, tl, tt});
+        if (verbose) fflush(stdout);
+    }
+    for (size_t k = 0; k < es.size(); ++k) {
+        if (rc->loss) rc->loss[k] = es[k].loss;
+        if (rc->test_loss) rc->test_loss[k] = es[k].test;
+        if (rc->eps) rc->eps[k] = es[k].eps;
+        if (rc->time_epoch) rc->time_epoch[k] = es[k].tEpoch / 1e9;
+    }
+    if (rc->trace_path && *rc->trace_path) {
+        // Args::createTraceFields (tools/detail/mfdsgd-args.h:53-63)
+        const std::map<std::string, double> nums{{"rank", (double)c->r}, {"nodes", (double)c->w}, {"threads", (double)c->t}};
+        const std::map<std::string, std::string> strs{{"update", c->updateStr},
+                                                      {"regularize", c->regularizeStr},
+                                                      {"input_file", rc->input_file ? rc->input_file : ""},
+                                                      {"sample_matrix", ""},
+                                                      {"barrier", "false"}};
+        std::ofstream out(rc->trace_path);
+        if (!out.is_open()) throw Error(DSGDPP_E_IO, std::string("Cannot open file ") + rc->trace_path);
+        out << formatTrace(es, rc->trace_var && *rc->trace_var ? rc->trace_var : "trace", nums, strs);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* dsgdpp_create_error(void) { return g_createError.c_str(); }
+
+int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out) {
+    if (!out) return DSGDPP_E_ARG;
+    *out = nullptr;
+    dsgdpp_ctx* c = new dsgdpp_ctx();
+    try {
+        configure(c, cfg);
+        int count = 0;
+        HIPC(hipGetDeviceCount(&count));
+        if (c->device < 0 || c->device >= count) throw Error(DSGDPP_E_DEVICE, "no such HIP device");
+        HIPC(hipSetDevice(c->device));
+        HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+        dalloc(c->dPartials, (size_t)partials_capacity() * sizeof(double));
+        HIPC(hipHostMalloc((void**)&c->hPinned, sizeof(double) * 8, hipHostMallocDefault));
+    } catch (const std::exception& e) {
+        g_createError = e.what();
+        const int code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : DSGDPP_E_DEVICE;
+        dsgdpp_destroy(c);
+        return code;
+    }
+    *out = c;
+    return DSGDPP_OK;
+}
+
+void dsgdpp_destroy(dsgdpp_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; ++k) { dfree(c->dPts[k]); void* p = c->dBlk[k]; dfree(p); }
+    dfree(c->P.W);
+    dfree(c->P.H);
+    void* v;
+    v = const_cast<void*>(c->P.inv1); dfree(v);
+    v = const_cast<void*>(c->P.inv2); dfree(v);
+    v = const_cast<double*>(c->P.cnt1); dfree(v);
+    v = const_cast<double*>(c->P.cnt2); dfree(v);
+    v = c->P.perm; dfree(v);
+    v = c->dPartials; dfree(v);
+    v = c->dSched; dfree(v);
+    v = c->dJobs; dfree(v);
+    v = c->dChains; dfree(v);
+    if (c->hPinned) (void)hipHostFree(c->hPinned);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* dsgdpp_last_error(const dsgdpp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int dsgdpp_load_coo(dsgdpp_ctx* c, int32_t which, int64_t m, int64_t n, int64_t nnz, const int32_t* i,
+                    const int32_t* j, const double* x) {
+    return guarded(c, [&] { loadCoo(c, which, m, n, nnz, i, j, x); });
+}
+
+int dsgdpp_load_mmc(dsgdpp_ctx* c, int32_t which, const char* path) {
+    return guarded(c, [&] {
+        if (!path) throw Error(DSGDPP_E_ARG, "null path");
+        loadMmc(c, which, path);
+    });
+}
+
+int dsgdpp_set_factors(dsgdpp_ctx* c, const double* W, const double* H) {
+    return guarded(c, [&] {
+        if (c->m < 0) throw Error(DSGDPP_E_STATE, "load the training matrix first");
+        if (!W || !H) throw Error(DSGDPP_E_ARG, "null factor array");
+        putFactor(c, c->P.W, W, c->m);
+        putFactor(c, c->P.H, H, c->n);
+        c->haveFactors = true;
+    });
+}
+
+int dsgdpp_get_factors(dsgdpp_ctx* c, double* W, double* H) {
+    return guarded(c, [&] {
+        if (!c->haveFactors) throw Error(DSGDPP_E_STATE, "no factors");
+        HIPC(hipStreamSynchronize(c->stream));
+        if (W) getFactor(c, c->P.W, W, c->m);
+        if (H) getFactor(c, c->P.H, H, c->n);
+    });
+}
+
+int dsgdpp_load_factors_mma(dsgdpp_ctx* c, const char* wpath, const char* hpath) {
+    return guarded(c, [&] {
+        if (c->m < 0) throw Error(DSGDPP_E_STATE, "load the training matrix first");
+        int64_t a, b;
+        std::vector<double> cm;
+        readMma(wpath ? wpath : "", a, b, cm);
+        if (a != c->m || b != c->r) throw Error(DSGDPP_E_ARG, "row factor matrix is " + std::to_string(a) + " x " + std::to_string(b));
+        std::vector<double> W((size_t)a * b);
+        for (int64_t z = 0; z < b; ++z)
+            for (int64_t i = 0; i < a; ++i) W[i * b + z] = cm[z * a + i];
+        readMma(hpath ? hpath : "", a, b, cm);
+        if (a != c->r || b != c->n) throw Error(DSGDPP_E_ARG, "column factor matrix is " + std::to_string(a) + " x " + std::to_string(b));
+        putFactor(c, c->P.W, W.data(), c->m);
+        putFactor(c, c->P.H, cm.data(), c->n);
+        c->haveFactors = true;
+    });
+}
+
+int dsgdpp_write_factors_mma(dsgdpp_ctx* c, const char* wpath, const char* hpath) {
+    return guarded(c, [&] {
+        if (!c->haveFactors) throw Error(DSGDPP_E_STATE, "no factors");
+        HIPC(hipStreamSynchronize(c->stream));
+        const int64_t r = c->r;
+        if (wpath && *wpath) {
+            std::vector<double> W((size_t)c->m * r), cm((size_t)c->m * r);
+            getFactor(c, c->P.W, W.data(), c->m);
+            for (int64_t z = 0; z < r; ++z)
+                for (int64_t i = 0; i < c->m; ++i) cm[z * c->m + i] = W[i * r + z];
+            writeMma(wpath, c->m, r, cm);
+        }
+        if (hpath && *hpath) {
+            std::vector<double> H((size_t)c->n * r);
+            getFactor(c, c->P.H, H.data(), c->n);
+            writeMma(hpath, r, c->n, H);
+        }
+    });
+}
+
+int dsgdpp_epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* stats) {
+    return guarded(c, [&] { epoch(c, eps, stats); });
+}
+
+int dsgdpp_loss(dsgdpp_ctx* c, int32_t which, double* out) {
+    return guarded(c, [&] {
+        if (!out) throw Error(DSGDPP_E_ARG, "null output");
+        if (which == DSGDPP_TRAIN) *out = trainLoss(c);
+        else if (which == DSGDPP_TEST) *out = testLoss(c);
+        else throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+    });
+}
+
+int dsgdpp_balance(dsgdpp_ctx* c, int32_t type) {
+    return guarded(c, [&] { balance(c, type); });
+}
+
+int dsgdpp_run(dsgdpp_ctx* c, const dsgdpp_run_cfg* rc) {
+    return guarded(c, [&] { run(c, rc); });
+}
+
+int dsgdpp_get_schedule(dsgdpp_ctx* c, uint32_t* out) {
+    return guarded(c, [&] {
+        if (c->schedule.empty()) throw Error(DSGDPP_E_STATE, "no epoch has run");
+        if (!out) throw Error(DSGDPP_E_ARG, "null output");
+        std::copy(c->schedule.begin(), c->schedule.end(), out);
+    });
+}
+
+int dsgdpp_get_task_seeds(dsgdpp_ctx* c, uint64_t* out) {
+    return guarded(c, [&] {
+        if (c->seeds.empty()) throw Error(DSGDPP_E_STATE, "no epoch has run");
+        if (!out) throw Error(DSGDPP_E_ARG, "null output");
+        std::copy(c->seeds.begin(), c->seeds.end(), out);
+    });
+}
+
+int dsgdpp_get_permutation(dsgdpp_ctx* c, int32_t task, int32_t subepoch, uint32_t* out, int64_t cap,
+                           int64_t* n) {
+    return guarded(c, [&] {
+        if (c->schedule.empty()) throw Error(DSGDPP_E_STATE, "no epoch has run");
+        if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)(2 * c->d))
+            throw Error(DSGDPP_E_ARG, "task or sub-epoch out of range");
+        const size_t b = (size_t)task * 2 * c->d + c->schedule[(size_t)subepoch * c->d + task];
+        const int64_t base = c->blocks[0].start[b], nb = c->blocks[0].start[b + 1] - base;
+        if (n) *n = nb;
+        if (!out) return;
+        if (cap < nb) throw Error(DSGDPP_E_ARG, "output buffer too small");
+        if (c->sgdOrder == DSGDPP_SGD_SEQ) {
+            for (int64_t k = 0; k < nb; ++k) out[k] = (uint32_t)k;
+        } else if (nb > 0) {
+            HIPC(hipStreamSynchronize(c->stream));
+            HIPC(hipMemcpy(out, c->P.perm + base, (size_t)nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int dsgdpp_block_nnz(dsgdpp_ctx* c, int32_t which, int64_t* out) {
+    return guarded(c, [&] {
+        if (which != 0 && which != 1) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+        if (!c->blocks[which].loaded) throw Error(DSGDPP_E_STATE, "matrix not loaded");
+        const auto& s = c->blocks[which].start;
+        for (size_t b = 0; b + 1 < s.size(); ++b) out[b] = s[b + 1] - s[b];
+    });
+}
+
+int dsgdpp_dims(dsgdpp_ctx* c, int64_t* out) {
+    return guarded(c, [&] {
+        out[0] = c->m;
+        out[1] = c->n;
+        out[2] = c->r;
+        out[3] = c->d;
+        out[4] = c->blocks[0].nnz;
+        out[5] = c->blocks[1].nnz;
+    });
+}
+
+}  // extern "C"

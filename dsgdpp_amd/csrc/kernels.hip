@@ -1,0 +1,469 @@
+// gfx950 kernels of the DSGD++ epoch: per-block visit orders (WOR / WR),
+// one launch per stratum walking each conflict-free block in its visit order,
+// and FP64 loss reductions.
+//
+// Compiled with -ffp-contract=off: every expression rounds as written, which is
+// what the FP64_STRICT path needs to reproduce UpdateNzslL2 bit for bit
+// (mf/sgd/functions/update-nzsl-l2.h:34-57).  The other precisions call fma()
+// explicitly where they want it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "internal.h"
+#include "philox.h"
+
+namespace dsgdpp {
+
+#define HIPK(call)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                     " at " #call);                                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// storage types
+// ---------------------------------------------------------------------------
+struct bf16_t {
+    uint16_t u;
+};
+
+template <typename T>
+struct St;
+template <>
+struct St<double> {
+    using C = double;
+    static __device__ __forceinline__ C load(const double* p) { return *p; }
+    static __device__ __forceinline__ void store(double* p, C v) { *p = v; }
+    static __device__ __forceinline__ double wide(const double* p) { return *p; }
+};
+template <>
+struct St<float> {
+    using C = float;
+    static __device__ __forceinline__ C load(const float* p) { return *p; }
+    static __device__ __forceinline__ void store(float* p, C v) { *p = v; }
+    static __device__ __forceinline__ double wide(const float* p) { return (double)*p; }
+};
+__device__ __forceinline__ float bf16_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even
+    uint32_t b = __float_as_uint(f);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);   // quiet NaN
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+template <>
+struct St<bf16_t> {
+    using C = float;
+    static __device__ __forceinline__ C load(const bf16_t* p) { return bf16_to_f32(p->u); }
+    static __device__ __forceinline__ void store(bf16_t* p, C v) { p->u = f32_to_bf16(v); }
+    static __device__ __forceinline__ double wide(const bf16_t* p) { return (double)bf16_to_f32(p->u); }
+};
+
+template <typename T>
+struct PtOf {
+    using type = Pt32;
+};
+template <>
+struct PtOf<double> {
+    using type = Pt64;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ double readlane(double v, int lane) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = readlane_u32((uint32_t)u, lane);
+    const uint32_t hi = readlane_u32((uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float readlane(float v, int lane) {
+    return __uint_as_float(readlane_u32(__float_as_uint(v), lane));
+}
+
+// Butterfly sum: every lane ends with the bitwise identical total.
+template <typename C>
+__device__ __forceinline__ C wave_sum(C v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Visit orders.
+// WOR: SgdRunner::permute (mf/sgd/sgd_impl.h:255-278).  For a block of n points
+// the swap partner of position s is s + nextInt(n - s) (draw s of the task
+// stream, counted from the block's first draw); positions 0..n-2 are swapped
+// in order, the draw for n-1 (nextInt(1)) is consumed but unused.  A block
+// starts from the identity unless the task's previous non-empty block had the
+// same size, in which case it starts from that block's permutation (:257-262).
+// The 64 lanes draw the swap partners of 64 positions at once; lane 0 performs
+// the dependent swaps in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_permute_wor(const PermJob* __restrict__ jobs,
+                                                    const int32_t* __restrict__ chain_start,
+                                                    uint32_t* __restrict__ perm, int64_t lds_cap) {
+    extern __shared__ uint32_t sp[];
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    for (int jx = chain_start[c]; jx < chain_start[c + 1]; ++jx) {
+        const PermJob J = jobs[jx];
+        const int64_t n = J.n;
+        uint32_t* out = perm + J.base;
+        const bool in_lds = n <= lds_cap;
+        uint32_t* a = in_lds ? sp : out;
+        if (J.carry_base >= 0) {
+            const uint32_t* src = perm + J.carry_base;
+            for (int64_t k = lane; k < n; k += 64) a[k] = src[k];
+        } else {
+            for (int64_t k = lane; k < n; k += 64) a[k] = (uint32_t)k;
+        }
+        __syncthreads();
+        for (int64_t base = 0; base < n - 1; base += 64) {
+            const int64_t s = base + lane;
+            uint32_t tg = 0;
+            if (s < n) tg = (uint32_t)(s + (int64_t)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)(n - s)));
+            const int cnt = (int)min<int64_t>(64, n - 1 - base);
+            for (int l = 0; l < cnt; ++l) {
+                const uint32_t t = readlane_u32(tg, l);
+                if (lane == 0) {
+                    const uint32_t u = a[base + l];
+                    const uint32_t v = a[t];
+                    a[base + l] = v;
+                    a[t] = u;
+                }
+            }
+            __syncthreads();
+        }
+        if (in_lds) {
+            for (int64_t k = lane; k < n; k += 64) out[k] = a[k];
+        }
+        __syncthreads();
+    }
+}
+
+// WR: SgdRunner::updateWr (sgd_impl.h:224-251): step s visits nextInt(n) of
+// draw s; two draws beyond the last step are consumed but unused.
+__global__ __launch_bounds__(256) void k_visit_wr(const PermJob* __restrict__ jobs,
+                                                 uint32_t* __restrict__ perm) {
+    const PermJob J = jobs[blockIdx.x];
+    for (int64_t s = threadIdx.x; s < J.n; s += 256)
+        perm[J.base + s] = (uint32_t)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)J.n);
+}
+
+// ---------------------------------------------------------------------------
+// Stratum kernel: one wave per block of the stratum.  Workgroup t runs block
+// (t, sched[t]) -- task t's work in this sub-epoch (dsgdpp_impl.h:122-123) --
+// visiting its points in the block's visit order (sgd_impl.h:293-351) and
+// applying the update functor to W row i and H column j, lane z holding
+// element z (+64k) of both vectors.
+// ---------------------------------------------------------------------------
+template <typename T, int K, bool STRICT>
+__global__ __launch_bounds__(64) void k_stratum_seq(
+    const typename PtOf<T>::type* __restrict__ pts, const int64_t* __restrict__ blk,
+    const uint32_t* __restrict__ perm, const uint32_t* __restrict__ sched, T* W, T* H,
+    const typename St<T>::C* __restrict__ inv1, const typename St<T>::C* __restrict__ inv2,
+    int d2, int r, int64_t ld, typename St<T>::C eps, typename St<T>::C lambda, int kind,
+    int truncate, typename St<T>::C tmin, typename St<T>::C tmax, int use_perm) {
+    using C = typename St<T>::C;
+    using Pt = typename PtOf<T>::type;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x * d2 + (int)sched[blockIdx.x];
+    const int64_t s0 = blk[b];
+    const int64_t n = blk[b + 1] - s0;
+    const C m2eps = eps * (C)-2.;
+    const C f2 = eps * (C)2. * lambda;
+    for (int64_t p = 0; p < n; ++p) {
+        const uint32_t q = use_perm ? perm[s0 + p] : (uint32_t)p;
+        const Pt pt = pts[s0 + q];
+        T* wr = W + (int64_t)pt.i * ld;
+        T* hr = H + (int64_t)pt.j * ld;
+        C w[K], h[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int z = lane + 64 * k;
+            w[k] = z < r ? St<T>::load(wr + z) : (C)0;
+            h[k] = z < r ? St<T>::load(hr + z) : (C)0;
+        }
+        C wh;
+        if constexpr (STRICT) {
+            // wh accumulated sequentially over z (update-nzsl-l2.h:40-43)
+            C prod[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) prod[k] = w[k] * h[k];
+            wh = (C)0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int lim = min(64, r - 64 * k);
+                for (int l = 0; l < lim; ++l) wh = wh + readlane(prod[k], l);
+            }
+        } else {
+            C part = (C)0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) part = fma(w[k], h[k], part);
+            wh = wave_sum(part);
+        }
+        const C x = (C)pt.x;
+        const C f1 = m2eps * (x - wh);
+        C a = (C)0, bb = (C)0;
+        if (kind == UPD_NZSL_L2) {
+            a = f2 * inv1[pt.i];
+            bb = f2 * inv2[pt.j];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const C t = w[k];
+            C wn, hn;
+            if constexpr (STRICT) {
+                if (kind == UPD_NZSL) {                 // update-nzsl.h:44-48
+                    wn = w[k] - f1 * h[k];
+                    hn = h[k] - f1 * t;
+                } else if (kind == UPD_NZSL_L2) {       // update-nzsl-l2.h:49-55
+                    wn = w[k] - (f1 * h[k] + f2 * w[k] * inv1[pt.i]);
+                    hn = h[k] - (f1 * t + f2 * h[k] * inv2[pt.j]);
+                } else {                                // update-nzsl-nzl2.h:46-51
+                    wn = w[k] - (f1 * h[k] + f2 * w[k]);
+                    hn = h[k] - (f1 * t + f2 * h[k]);
+                }
+            } else {
+                if (kind == UPD_NZSL) {
+                    wn = fma(-f1, h[k], w[k]);
+                    hn = fma(-f1, t, h[k]);
+                } else if (kind == UPD_NZSL_L2) {
+                    wn = fma(-f1, h[k], fma(-a, w[k], w[k]));
+                    hn = fma(-f1, t, fma(-bb, h[k], h[k]));
+                } else {
+                    wn = fma(-f1, h[k], fma(-f2, w[k], w[k]));
+                    hn = fma(-f1, t, fma(-f2, h[k], h[k]));
+                }
+            }
+            if (truncate) {                             // update-truncate.h:50-64
+                if (wn <= tmin) wn = tmin + eps;
+                else if (wn >= tmax) wn = tmax - eps;
+                if (hn <= tmin) hn = tmin + eps;
+                else if (hn >= tmax) hn = tmax - eps;
+            }
+            const int z = lane + 64 * k;
+            if (z < r) {
+                St<T>::store(wr + z, wn);
+                St<T>::store(hr + z, hn);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Loss reductions, FP64 accumulation, deterministic: fixed grid, per-workgroup
+// partials, then one workgroup sums the partials in index order.
+// ---------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 2048;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
+}
+
+// NZSL = sum (x - w_i . h_j)^2 (mf/loss/nzsl.h:30-53)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_nzsl(const typename PtOf<T>::type* __restrict__ pts,
+                                                      int64_t nnz, const T* __restrict__ W,
+                                                      const T* __restrict__ H, int r, int64_t ld,
+                                                      double* __restrict__ partials) {
+    __shared__ double sh[RED_THREADS / 64];
+    double acc = 0;
+    for (int64_t p = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; p < nnz;
+         p += (int64_t)gridDim.x * RED_THREADS) {
+        const auto pt = pts[p];
+        const T* w = W + (int64_t)pt.i * ld;
+        const T* h = H + (int64_t)pt.j * ld;
+        double ip = 0;
+        for (int z = 0; z < r; ++z) ip += St<T>::wide(w + z) * St<T>::wide(h + z);
+        const double diff = (double)pt.x - ip;
+        acc += diff * diff;
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// sum_i weight_i * sum_z A[i,z]^2 over rows [lo, hi) (l2.h:40-44, nzl2.h:29-76)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_sumsq(const T* __restrict__ A, int64_t lo, int64_t hi,
+                                                       int r, int64_t ld,
+                                                       const double* __restrict__ weight,
+                                                       double* __restrict__ partials) {
+    __shared__ double sh[RED_THREADS / 64];
+    double acc = 0;
+    for (int64_t i = lo + (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < hi;
+         i += (int64_t)gridDim.x * RED_THREADS) {
+        const T* a = A + i * ld;
+        double v = 0;
+        for (int z = 0; z < r; ++z) {
+            const double e = St<T>::wide(a + z);
+            v += e * e;
+        }
+        acc += weight ? weight[i] * v : v;
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ partials, int n,
+                                                     double* __restrict__ out) {
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < n; ++k) t += partials[k];
+        *out = t;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_scale(T* A, int64_t rows, int r, int64_t ld,
+                                                       double f) {
+    for (int64_t e = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; e < rows * r;
+         e += (int64_t)gridDim.x * RED_THREADS) {
+        const int64_t i = e / r;
+        const int z = (int)(e - i * r);
+        T* p = A + i * ld + z;
+        St<T>::store(p, (typename St<T>::C)(St<T>::wide(p) * f));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int partials_capacity() { return RED_BLOCKS + 1; }
+
+void launch_permute_wor(const PermJob* d_jobs, const int32_t* d_chain_start, int nchains,
+                        uint32_t* perm, int64_t max_block, hipStream_t s) {
+    if (nchains <= 0) return;
+    const int64_t lds_cap = 32768;   // 128 KiB of LDS; larger blocks shuffle in place in HBM
+    const int64_t words = std::min<int64_t>(max_block, lds_cap);
+    const size_t bytes = (size_t)std::max<int64_t>(words, 1) * sizeof(uint32_t);
+    if (bytes > 64 * 1024)
+        HIPK(hipFuncSetAttribute((const void*)k_permute_wor,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    k_permute_wor<<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, perm, words);
+    HIPK(hipGetLastError());
+}
+
+void launch_visit_wr(const PermJob* d_jobs, int njobs, uint32_t* perm, hipStream_t s) {
+    if (njobs <= 0) return;
+    k_visit_wr<<<njobs, 256, 0, s>>>(d_jobs, perm);
+    HIPK(hipGetLastError());
+}
+
+template <typename T, int K, bool STRICT>
+static void stratum_t(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
+                      int use_perm, double eps, hipStream_t s) {
+    using C = typename St<T>::C;
+    k_stratum_seq<T, K, STRICT><<<P.d, 64, 0, s>>>(
+        (const typename PtOf<T>::type*)P.pts, P.blk, P.perm, d_sched_row, (T*)P.W, (T*)P.H,
+        (const C*)P.inv1, (const C*)P.inv2, 2 * P.d, (int)P.r, P.ld, (C)eps, (C)U.lambda, U.kind,
+        U.truncate ? 1 : 0, (C)U.tmin, (C)U.tmax, use_perm);
+    HIPK(hipGetLastError());
+}
+
+template <typename T, bool STRICT>
+static void stratum_k(const DevProblem& P, const UpdParams& U, const uint32_t* row, int use_perm,
+                      double eps, hipStream_t s) {
+    if (P.r <= 64) stratum_t<T, 1, STRICT>(P, U, row, use_perm, eps, s);
+    else if (P.r <= 128) stratum_t<T, 2, STRICT>(P, U, row, use_perm, eps, s);
+    else if (P.r <= 256) stratum_t<T, 4, STRICT>(P, U, row, use_perm, eps, s);
+    else throw std::invalid_argument("rank above 256 is not supported by the GPU path");
+}
+
+void launch_stratum(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
+                    int use_perm, double eps, hipStream_t s) {
+    switch (P.precision) {
+    case PREC_FP64_STRICT: stratum_k<double, true>(P, U, d_sched_row, use_perm, eps, s); break;
+    case PREC_FP64: stratum_k<double, false>(P, U, d_sched_row, use_perm, eps, s); break;
+    case PREC_FP32: stratum_k<float, false>(P, U, d_sched_row, use_perm, eps, s); break;
+    case PREC_BF16: stratum_k<bf16_t, false>(P, U, d_sched_row, use_perm, eps, s); break;
+    default: throw std::invalid_argument("unknown precision");
+    }
+}
+
+template <typename T>
+static double nzsl_t(const DevProblem& P, int which, double* d_partials, double* h_pinned,
+                     hipStream_t s) {
+    const void* pts = which == 0 ? P.pts : P.test_pts;
+    const int64_t nnz = which == 0 ? P.nnz : P.test_nnz;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (nnz + RED_THREADS - 1) / RED_THREADS));
+    k_nzsl<T><<<blocks, RED_THREADS, 0, s>>>((const typename PtOf<T>::type*)pts, nnz,
+                                             (const T*)P.W, (const T*)P.H, (int)P.r, P.ld, d_partials);
+    HIPK(hipGetLastError());
+    k_sum_partials<<<1, 64, 0, s>>>(d_partials, blocks, d_partials + RED_BLOCKS);
+    HIPK(hipGetLastError());
+    HIPK(hipMemcpyAsync(h_pinned, d_partials + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPK(hipStreamSynchronize(s));
+    return *h_pinned;
+}
+
+double device_nzsl(const DevProblem& P, int which, double* d_partials, double* h_pinned,
+                   hipStream_t s) {
+    switch (P.precision) {
+    case PREC_FP64_STRICT:
+    case PREC_FP64: return nzsl_t<double>(P, which, d_partials, h_pinned, s);
+    case PREC_FP32: return nzsl_t<float>(P, which, d_partials, h_pinned, s);
+    default: return nzsl_t<bf16_t>(P, which, d_partials, h_pinned, s);
+    }
+}
+
+template <typename T>
+static double sumsq_t(const DevProblem& P, int which_factor, const double* d_weights, int64_t lo,
+                      int64_t hi, double* d_partials, double* h_pinned, hipStream_t s) {
+    const T* A = (const T*)(which_factor == 0 ? P.W : P.H);
+    const int64_t rows = hi - lo;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (rows + RED_THREADS - 1) / RED_THREADS));
+    k_sumsq<T><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, (int)P.r, P.ld, d_weights, d_partials);
+    HIPK(hipGetLastError());
+    k_sum_partials<<<1, 64, 0, s>>>(d_partials, blocks, d_partials + RED_BLOCKS);
+    HIPK(hipGetLastError());
+    HIPK(hipMemcpyAsync(h_pinned, d_partials + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPK(hipStreamSynchronize(s));
+    return *h_pinned;
+}
+
+double device_sumsq(const DevProblem& P, int which_factor, const double* d_weights, int64_t lo,
+                    int64_t hi, double* d_partials, double* h_pinned, hipStream_t s) {
+    switch (P.precision) {
+    case PREC_FP64_STRICT:
+    case PREC_FP64: return sumsq_t<double>(P, which_factor, d_weights, lo, hi, d_partials, h_pinned, s);
+    case PREC_FP32: return sumsq_t<float>(P, which_factor, d_weights, lo, hi, d_partials, h_pinned, s);
+    default: return sumsq_t<bf16_t>(P, which_factor, d_weights, lo, hi, d_partials, h_pinned, s);
+    }
+}
+
+template <typename T>
+static void scale_t(const DevProblem& P, int which_factor, double f, hipStream_t s) {
+    T* A = (T*)(which_factor == 0 ? P.W : P.H);
+    const int64_t rows = which_factor == 0 ? P.m : P.n;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (rows * P.r + RED_THREADS - 1) / RED_THREADS));
+    k_scale<T><<<blocks, RED_THREADS, 0, s>>>(A, rows, (int)P.r, P.ld, f);
+    HIPK(hipGetLastError());
+}
+
+void device_scale(const DevProblem& P, int which_factor, double f, hipStream_t s) {
+    switch (P.precision) {
+    case PREC_FP64_STRICT:
+    case PREC_FP64: scale_t<double>(P, which_factor, f, s); break;
+    case PREC_FP32: scale_t<float>(P, which_factor, f, s); break;
+    default: scale_t<bf16_t>(P, which_factor, f, s); break;
+    }
+}
+
+}  // namespace dsgdpp

@@ -1,0 +1,164 @@
+/*
+ * dsgdpp_amd -- C ABI of the MI355X (gfx950) DSGD++ stratified-SGD epoch.
+ *
+ * Drop-in boundary for the reference's DSGD++ path (uma-pi1/DSGDpp):
+ *   tools/mfdsgdpp.cc           (options, functor strings, blocking d x 2d)
+ *   mf/sgd/dsgdpp.h:82-134      (DsgdPpRunner::run / epoch)
+ *   mf/sgd/dsgdpp_impl.h:74-349 (DsgdPpTask, per-epoch schedule)
+ *   mf/sgd/sgd_impl.h:34-351    (defaultRunner, SgdRunner::updateWor/permute)
+ *   mf/loss/{loss,nzsl,l2,nzl2}.h, mf/sgd/decay/decay_bolddriver.h, mf/trace.h
+ *
+ * Conventions: plain pointers and sizes; every call returns 0 on success and a
+ * negative status on failure (message via dsgdpp_last_error); no exceptions
+ * cross the ABI.  The caller owns host buffers, which are copied in and out;
+ * the context owns all device memory and one HIP stream.  One host thread per
+ * context; every call is synchronous at return.
+ */
+#ifndef DSGDPP_H
+#define DSGDPP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSGDPP_ABI_VERSION 1
+
+/* status codes */
+#define DSGDPP_OK 0
+#define DSGDPP_E_ARG -1      /* invalid argument (reference: rg::InvalidArgumentException) */
+#define DSGDPP_E_STATE -2    /* call out of order (data/factors missing) */
+#define DSGDPP_E_IO -3       /* file errors (reference: rg::IOException, read_impl.h) */
+#define DSGDPP_E_DEVICE -4   /* HIP runtime error */
+#define DSGDPP_E_UNSUPPORTED -5
+
+/* SGD point order, mf/sgd/sgd.h:39 */
+#define DSGDPP_SGD_SEQ 0
+#define DSGDPP_SGD_WR 1
+#define DSGDPP_SGD_WOR 2
+/* stratum order, mf/sgd/dsgd.h:29 */
+#define DSGDPP_STRATUM_SEQ 0
+#define DSGDPP_STRATUM_RSEQ 1
+#define DSGDPP_STRATUM_WR 2
+#define DSGDPP_STRATUM_WOR 3
+#define DSGDPP_STRATUM_COWOR 4
+/* factor precision (extension; the reference is FP64 only) */
+#define DSGDPP_FP64_STRICT 0 /* FP64, reference association, sequential dot: bit-exact */
+#define DSGDPP_FP64 1        /* FP64, lane-parallel dot */
+#define DSGDPP_FP32 2        /* FP32 factors and update math */
+#define DSGDPP_BF16 3        /* BF16 factor storage, FP32 update math */
+/* matrices */
+#define DSGDPP_TRAIN 0
+#define DSGDPP_TEST 1
+/* balance, mf/matrix/op/balance.h:27 (simple method) */
+#define DSGDPP_BALANCE_NONE 0
+#define DSGDPP_BALANCE_L2 1
+#define DSGDPP_BALANCE_NZL2 2
+
+typedef struct dsgdpp_ctx dsgdpp_ctx;
+
+/* Replaces the mfdsgdpp option set (tools/mfdsgdpp.cc:380-404, defaults :430-441)
+ * and DsgdPpJob (mf/sgd/dsgdpp.h:39). */
+typedef struct dsgdpp_cfg {
+    int32_t abi_version;     /* DSGDPP_ABI_VERSION */
+    const char* update;      /* "Nzsl" | "Nzsl_L2(l)" | "Nzsl_Nzl2(l)"  (parse.h:39-61) */
+    const char* regularize;  /* "None"                                   (mfdsgdpp.cc:162-358) */
+    const char* loss;        /* "Nzsl" | "Nzsl_L2(l)" | "Nzsl_Nzl2(l)" */
+    const char* truncate;    /* NULL/"" or "(min,max)"                   (parse.h:115-139) */
+    int64_t rank;            /* --rank */
+    int32_t world_size;      /* logical ranks w (COWOR grouping); 1 on one GPU */
+    int32_t tasks_per_rank;  /* --tasks-per-rank t; d = w*t row blocks, 2d column blocks */
+    int32_t sgd_order;       /* --sgd-order, default WOR */
+    int32_t stratum_order;   /* --stratum-order, default COWOR */
+    uint64_t seed;           /* --seed */
+    int32_t precision;       /* DSGDPP_FP64_STRICT .. DSGDPP_BF16 */
+    int32_t device;          /* HIP device ordinal */
+    int32_t kernel;          /* 0 = auto; 1 = one wave per block, sequential; 2 = dataflow */
+    int32_t reserved[5];
+} dsgdpp_cfg;
+
+typedef struct dsgdpp_epoch_stats {
+    double seconds;          /* wall time of the epoch incl. schedule generation (trace time$epoch) */
+    double device_seconds;   /* HIP-event time of the device work */
+    double perm_seconds;     /* HIP-event time of the permutation kernel */
+    double sgd_seconds;      /* HIP-event time of the stratum kernels */
+    int64_t updates;         /* points visited */
+    int64_t launches;        /* stratum kernel launches */
+} dsgdpp_epoch_stats;
+
+/* Creates a context (replaces mfInit + DsgdPpJob construction). */
+int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out);
+void dsgdpp_destroy(dsgdpp_ctx* ctx);
+const char* dsgdpp_last_error(const dsgdpp_ctx* ctx);
+/* message of the last dsgdpp_create failure in this thread */
+const char* dsgdpp_create_error(void);
+
+/* Data matrix: 0-based COO triples; the library blocks it d x 2d with the
+ * reference offsets (distributed_matrix_impl.cc:24-40) and sorts every block
+ * row-major, summing duplicates (read_impl.h:163, :365-486).  Replaces
+ * getDataMatrices / loadMatrix (generateDistributedMatrix_impl.h:693-744). */
+int dsgdpp_load_coo(dsgdpp_ctx* ctx, int32_t which, int64_t m, int64_t n, int64_t nnz,
+                    const int32_t* i, const int32_t* j, const double* x);
+/* Same from a "%%MatrixMarket matrix coordinate real general" file
+ * (read_impl.h:46-116). */
+int dsgdpp_load_mmc(dsgdpp_ctx* ctx, int32_t which, const char* path);
+
+/* Factors: W row-major m x r, H column-major r x n (column j's r values
+ * contiguous), as DenseMatrix / DenseMatrixCM (mf/types.h:37-39). */
+int dsgdpp_set_factors(dsgdpp_ctx* ctx, const double* W, const double* H);
+int dsgdpp_get_factors(dsgdpp_ctx* ctx, double* W, double* H);
+/* "%%MatrixMarket matrix array real general" files (read_impl.h:195-256,
+ * write_impl.h:94-118); replaces getFactors (generateDistributedMatrix_impl.h:655-687)
+ * and the --output-*-file writes (mfdsgdpp.cc:145-159). */
+int dsgdpp_load_factors_mma(dsgdpp_ctx* ctx, const char* wpath, const char* hpath);
+int dsgdpp_write_factors_mma(dsgdpp_ctx* ctx, const char* wpath, const char* hpath);
+
+/* One DSGD++ epoch with step size eps.  Replaces DsgdPpRunner::epoch
+ * (dsgdpp_impl.h:282-349) and the d DsgdPpTask::run bodies (:78-276). */
+int dsgdpp_epoch(dsgdpp_ctx* ctx, double eps, dsgdpp_epoch_stats* stats);
+
+/* Loss in FP64.  which = TRAIN: the --loss functor (SumLoss of NzslLoss and
+ * L2Loss/Nzl2Loss, loss.h:89-112); which = TEST: NzslLoss on the test matrix
+ * (mfdsgdpp.cc:60-66). */
+int dsgdpp_loss(dsgdpp_ctx* ctx, int32_t which, double* out);
+
+/* balance (balance_impl.cc:102-208, simple method) */
+int dsgdpp_balance(dsgdpp_ctx* ctx, int32_t type);
+
+/* The whole defaultRunner loop (sgd_impl.h:34-145): initial loss/test loss,
+ * then per epoch decay -> epoch -> balance -> loss -> test loss -> trace entry.
+ * decay: "BoldDriver(e)", "BoldDriver(e,dec,inc)" or "Const(e)" (parse.h:142-235).
+ * trace_path (optional) gets the R trace (trace.h:230-262).  The optional
+ * arrays receive epochs+1 values each. */
+typedef struct dsgdpp_run_cfg {
+    int64_t epochs;
+    const char* decay;
+    int32_t balance;
+    int32_t verbose;         /* print the reference's log lines */
+    const char* trace_path;
+    const char* trace_var;
+    const char* input_file;  /* trace info field */
+    double* loss;
+    double* test_loss;
+    double* eps;
+    double* time_epoch;
+} dsgdpp_run_cfg;
+int dsgdpp_run(dsgdpp_ctx* ctx, const dsgdpp_run_cfg* run);
+
+/* Parity hooks: the last epoch's 2d x d schedule (row = sub-epoch, column =
+ * task, entry = column block; dsgdpp_impl.h:330-336), the seeds handed to the
+ * d tasks (:344), and one block's visit order (block-local point indices in
+ * stored order; sgd_impl.h:255-351). */
+int dsgdpp_get_schedule(dsgdpp_ctx* ctx, uint32_t* out);
+int dsgdpp_get_task_seeds(dsgdpp_ctx* ctx, uint64_t* out);
+int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint32_t* out,
+                           int64_t cap, int64_t* n);
+/* nnz per block (d x 2d, row-major) and dims {m, n, r, d, nnz_train, nnz_test} */
+int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);
+int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,198 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+FP64_STRICT: schedules, task seeds and every block's visit order must match
+exactly, and the factors must be bitwise identical after every epoch; the
+FP64 losses (GPU tree reduction vs the oracle's sequential sums) agree to
+1e-12 relative.  Other precisions are checked against the FP64 oracle with the
+tolerance written in each test.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from conftest import random_problem
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+LOSS_RTOL = 1e-12
+
+
+def _dsgdpp():
+    import dsgdpp_amd
+    return dsgdpp_amd
+
+
+def make_pair(p, update="Nzsl_L2(1)", loss=None, truncate=None, tasks=2, world=1, sgd="WOR",
+              stratum="COWOR", seed=7, precision="FP64_STRICT", test=None):
+    D = _dsgdpp()
+    loss = loss or update
+    g = D.DsgdPp(update=update, regularize="None", loss=loss, rank=p["rank"], tasks_per_rank=tasks,
+                 world_size=world, sgd_order=sgd, stratum_order=stratum, seed=seed,
+                 precision=precision, truncate=truncate)
+    o = po.Oracle(update=update, regularize="None", loss=loss, truncate=truncate, rank=p["rank"],
+                  world_size=world, tasks_per_rank=tasks, sgd_order=sgd, stratum_order=stratum, seed=seed,
+                  threads=2)
+    for h in (g, o):
+        h.load_coo(0, p["m"], p["n"], *p["train"])
+        if test is not None:
+            h.load_coo(1, p["m"], p["n"], *test)
+        h.set_factors(p["W0"], p["H0"])
+    o.record(True)
+    return g, o
+
+
+def assert_epoch_parity(g, o, eps, check_perms=True):
+    g.epoch(eps)
+    o.epoch(eps)
+    np.testing.assert_array_equal(g.schedule(), o.schedule())
+    np.testing.assert_array_equal(g.task_seeds(), o.task_seeds())
+    d = g.d
+    if check_perms:
+        for t in range(d):
+            for s in range(2 * d):
+                np.testing.assert_array_equal(g.permutation(t, s), o.permutation(t, s),
+                                              err_msg=f"task {t} sub-epoch {s}")
+    Wg, Hg = g.get_factors()
+    Wo, Ho = o.get_factors()
+    np.testing.assert_array_equal(Wg, Wo)
+    np.testing.assert_array_equal(Hg, Ho)
+    lg, lo = g.loss(0), o.loss(0)
+    assert abs(lg - lo) <= LOSS_RTOL * abs(lo), (lg, lo)
+
+
+def test_tiny_strict_bitexact():
+    print("case: tiny 12x16 r=3 d=2", flush=True)
+    p = random_problem(12, 16, 60, 3, seed=1)
+    g, o = make_pair(p, tasks=2)
+    for e in range(4):
+        assert_epoch_parity(g, o, 0.01 * (1 + e))
+
+
+@pytest.mark.parametrize("sgd,stratum", list(itertools.product(["SEQ", "WR", "WOR"],
+                                                               ["SEQ", "RSEQ", "WR", "WOR", "COWOR"])))
+def test_orders_strict(sgd, stratum):
+    print(f"case: orders sgd={sgd} stratum={stratum}", flush=True)
+    p = random_problem(40, 60, 500, 4, seed=2)
+    g, o = make_pair(p, tasks=3, sgd=sgd, stratum=stratum, seed=11)
+    for e in range(3):
+        assert_epoch_parity(g, o, 0.02)
+
+
+@pytest.mark.parametrize("update,truncate", [("Nzsl", None), ("Nzsl_L2(0.5)", None),
+                                             ("Nzsl_Nzl2(0.05)", None), ("Nzsl_L2(0.5)", "(-0.3,0.3)"),
+                                             ("Nzsl", "(-0.2,0.25)")])
+def test_update_functors_strict(update, truncate):
+    print(f"case: functor {update} truncate={truncate}", flush=True)
+    p = random_problem(50, 70, 900, 5, seed=3)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=2, seed=5)
+    for e in range(3):
+        assert_epoch_parity(g, o, 0.05)
+
+
+def test_cowor_world_groups_strict():
+    print("case: COWOR w=2 t=3", flush=True)
+    p = random_problem(60, 80, 1200, 4, seed=4)
+    g, o = make_pair(p, tasks=3, world=2, seed=9)
+    for e in range(2):
+        assert_epoch_parity(g, o, 0.03)
+
+
+@pytest.mark.parametrize("rank", [1, 63, 64, 65, 128, 130])
+def test_ranks_strict(rank):
+    print(f"case: rank {rank}", flush=True)
+    p = random_problem(30, 40, 300, rank, seed=5)
+    g, o = make_pair(p, tasks=2, seed=3)
+    for e in range(2):
+        assert_epoch_parity(g, o, 0.01)
+
+
+def test_empty_blocks_and_duplicates_strict():
+    print("case: empty blocks + duplicate entries", flush=True)
+    p = random_problem(64, 64, 40, 3, seed=6, dup=10)   # d=8: 128 blocks, most empty
+    g, o = make_pair(p, tasks=8, seed=13)
+    np.testing.assert_array_equal(g.block_nnz(), o.block_nnz())
+    for e in range(3):
+        assert_epoch_parity(g, o, 0.05)
+
+
+def test_carry_over_equal_block_sizes_strict():
+    """Every block has the same nnz, so each task's permutation carries over
+    from block to block (sgd_impl.h:257-262)."""
+    print("case: equal-size blocks (permutation carry-over)", flush=True)
+    m, n, d = 8, 16, 2
+    rng = np.random.default_rng(8)
+    i, j = [], []
+    for b1 in range(d):
+        for b2 in range(2 * d):
+            cells = rng.choice(16, size=5, replace=False)
+            i += list(b1 * 4 + cells // 4)
+            j += list(b2 * 4 + cells % 4)
+    p = dict(m=m, n=n, train=(np.array(i), np.array(j), rng.normal(size=len(i))), rank=3,
+             W0=rng.uniform(-0.5, 0.5, (m, 3)), H0=rng.uniform(-0.5, 0.5, (3, n)))
+    g, o = make_pair(p, tasks=d, seed=17)
+    assert (g.block_nnz() == 5).all()
+    for e in range(3):
+        assert_epoch_parity(g, o, 0.05)
+
+
+def test_medium_many_blocks_strict():
+    print("case: 3000x3000 nnz=60000 r=10 d=16", flush=True)
+    p = random_problem(3000, 3000, 60000, 10, seed=9)
+    g, o = make_pair(p, tasks=16, seed=21)
+    for e in range(2):
+        assert_epoch_parity(g, o, 0.005, check_perms=(e == 0))
+
+
+def test_run_loop_trace_and_test_loss_strict(tmp_path):
+    print("case: defaultRunner loop with test matrix", flush=True)
+    p = random_problem(200, 300, 6000, 6, seed=10)
+    rng = np.random.default_rng(99)
+    cells = rng.choice(200 * 300, size=600, replace=False)
+    test = ((cells // 300).astype(np.int32), (cells % 300).astype(np.int32), rng.normal(size=600))
+    g, o = make_pair(p, tasks=4, seed=31, test=test)
+    rg = g.run_native(6, "BoldDriver(0.01)", trace_path=str(tmp_path / "g.R"))
+    ro = o.run(6, "BoldDriver(0.01)")
+    np.testing.assert_allclose(rg["loss"], ro["loss"], rtol=LOSS_RTOL)
+    np.testing.assert_allclose(rg["test_loss"], ro["test_loss"], rtol=LOSS_RTOL)
+    np.testing.assert_array_equal(rg["eps"][1:], ro["eps"][1:])
+    txt = (tmp_path / "g.R").read_text()
+    assert txt.startswith("trace <- list(trace=list(\nlist(it=0, epoch=0, loss=")
+    assert txt.count("list(it=") == 7
+
+
+def test_config1_shape_strict_per_epoch_loss():
+    """The quick-start shape (config 1 family, reduced 4x) for 8 epochs:
+    bit-identical factors and per-epoch loss within 1e-9 relative."""
+    print("case: config1/4 strict, 8 epochs", flush=True)
+    from dsgdpp_amd import data
+    p = data.config1(shrink=4, seed=3)
+    g, o = make_pair(p, tasks=4, seed=1234, test=p["test"])
+    rg = g.run_native(8, "BoldDriver(0.001)")
+    ro = o.run(8, "BoldDriver(0.001)")
+    np.testing.assert_allclose(rg["loss"], ro["loss"], rtol=1e-9)
+    np.testing.assert_allclose(rg["test_loss"], ro["test_loss"], rtol=1e-9)
+    Wg, Hg = g.get_factors()
+    Wo, Ho = o.get_factors()
+    np.testing.assert_array_equal(Wg, Wo)
+    np.testing.assert_array_equal(Hg, Ho)
+
+
+@pytest.mark.parametrize("precision,rtol", [("FP64", 1e-9), ("FP32", 2e-3), ("BF16", 5e-2)])
+def test_precisions_vs_fp64_oracle(precision, rtol):
+    """Same schedules/visit orders as the FP64 oracle; per-epoch training loss
+    relative error bounded by `rtol` (FP64 lane-parallel dot: 1e-9;
+    FP32 factors: 2e-3; BF16 factor storage: 5e-2)."""
+    print(f"case: precision {precision}", flush=True)
+    p = random_problem(400, 500, 20000, 16, seed=12, values="int")
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=4, seed=41, precision=precision)
+    for e in range(5):
+        g.epoch(0.01)
+        o.epoch(0.01)
+        np.testing.assert_array_equal(g.schedule(), o.schedule())
+        np.testing.assert_array_equal(g.permutation(1, 3), o.permutation(1, 3))
+        lg, lo = g.loss(0), o.loss(0)
+        assert abs(lg - lo) <= rtol * abs(lo), (e, lg, lo, abs(lg - lo) / abs(lo))
