@@ -185,9 +185,12 @@ struct dsgdpp_ctx {
     uint32_t* dSched = nullptr;
     PermJob* dJobs = nullptr;
     int32_t* dChains = nullptr;
+    int64_t* dVoff = nullptr;
     size_t jobsCap = 0, chainsCap = 0;
+    int64_t permCap = 0;
 
     std::vector<uint32_t> schedule;
+    std::vector<int64_t> voff;   // 2d x d: first perm entry of each (sub-epoch, task) visit
     std::vector<uint64_t> seeds;
 
     size_t factorBytes() const {
@@ -403,7 +406,6 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
         HIPC(hipMemcpy(dc2, cnt2.data(), cnt2.size() * sizeof(double), hipMemcpyHostToDevice));
         c->P.cnt1 = dc1;
         c->P.cnt2 = dc2;
-        dalloc(c->P.perm, (size_t)hb.nnz * sizeof(uint32_t));
         c->P.pts = c->dPts[0];
         c->P.blk = c->dBlk[0];
         c->P.nnz = hb.nnz;
@@ -531,8 +533,21 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     for (uint32_t id = 0; id < d; ++id) c->seeds[id] = rng.u64();
 
     // visit-order work: each task's stream is consumed block after block in
-    // sub-epoch order (one SgdRunner per task per epoch, :112)
+    // sub-epoch order (one SgdRunner per task per epoch, :112).  Every
+    // (task, sub-epoch) visit gets its own run of the perm buffer: a WR stratum
+    // order may send a task to the same block twice in one epoch.
     const std::vector<int64_t>& bs = c->blocks[0].start;
+    auto visitBlock = [&](uint32_t id, uint32_t s) {
+        return (size_t)id * d2 + c->schedule[(size_t)s * d + id];
+    };
+    c->voff.assign((size_t)d2 * d, 0);
+    int64_t total = 0;
+    for (uint32_t id = 0; id < d; ++id)
+        for (uint32_t s = 0; s < d2; ++s) {
+            const size_t b = visitBlock(id, s);
+            c->voff[(size_t)s * d + id] = total;
+            total += bs[b + 1] - bs[b];
+        }
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
     int64_t maxBlock = 0;
@@ -545,8 +560,9 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
                 uint64_t draw = 0;
                 int64_t lastN = 0, lastBase = -1;
                 for (uint32_t s = 0; s < d2; ++s) {
-                    const size_t b = (size_t)id * d2 + c->schedule[(size_t)s * d + id];
-                    const int64_t base = bs[b], nb = bs[b + 1] - bs[b];
+                    const size_t b = visitBlock((uint32_t)id, s);
+                    const int64_t nb = bs[b + 1] - bs[b];
+                    const int64_t base = c->voff[(size_t)s * d + id];
                     if (c->sgdOrder == DSGDPP_SGD_WR) {
                         if (nb > 0) perTask[id].push_back(PermJob{base, nb, -1, draw, k0, k1});
                         draw += (uint64_t)nb + 2;
@@ -568,6 +584,10 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
                 maxBlock = std::max(maxBlock, j.n);
             }
         chains.push_back((int32_t)jobs.size());
+        if (total > c->permCap) {
+            c->permCap = total;
+            dalloc(c->P.perm, (size_t)total * sizeof(uint32_t));
+        }
     }
     if (jobs.size() > c->jobsCap) {
         c->jobsCap = jobs.size() * 2;
@@ -578,8 +598,11 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
         dalloc(c->dChains, c->chainsCap * sizeof(int32_t));
     }
     if (!c->dSched) dalloc(c->dSched, (size_t)d2 * d * sizeof(uint32_t));
+    if (!c->dVoff) dalloc(c->dVoff, (size_t)d2 * d * sizeof(int64_t));
     HIPC(hipMemcpyAsync(c->dSched, c->schedule.data(), c->schedule.size() * sizeof(uint32_t),
                         hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->dVoff, c->voff.data(), c->voff.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                        c->stream));
     if (!jobs.empty())
         HIPC(hipMemcpyAsync(c->dJobs, jobs.data(), jobs.size() * sizeof(PermJob), hipMemcpyHostToDevice, c->stream));
     if (!chains.empty())
@@ -592,7 +615,8 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
         launch_visit_wr(c->dJobs, (int)jobs.size(), c->P.perm, c->stream);
     HIPC(hipEventRecord(c->ev[1], c->stream));
     for (uint32_t s = 0; s < d2; ++s)
-        launch_stratum(c->P, c->upd, c->dSched + (size_t)s * d, c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1, eps, c->stream);
+        launch_stratum(c->P, c->upd, c->dSched + (size_t)s * d, c->dVoff + (size_t)s * d,
+                       c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1, eps, c->stream);
     HIPC(hipEventRecord(c->ev[2], c->stream));
     HIPC(hipEventSynchronize(c->ev[2]));
     const auto t1 = std::chrono::steady_clock::now();
@@ -946,6 +970,7 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     v = c->dSched; dfree(v);
     v = c->dJobs; dfree(v);
     v = c->dChains; dfree(v);
+    v = c->dVoff; dfree(v);
     if (c->hPinned) (void)hipHostFree(c->hPinned);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1068,7 +1093,8 @@ int dsgdpp_get_permutation(dsgdpp_ctx* c, int32_t task, int32_t subepoch, uint32
         if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)(2 * c->d))
             throw Error(DSGDPP_E_ARG, "task or sub-epoch out of range");
         const size_t b = (size_t)task * 2 * c->d + c->schedule[(size_t)subepoch * c->d + task];
-        const int64_t base = c->blocks[0].start[b], nb = c->blocks[0].start[b + 1] - base;
+        const int64_t nb = c->blocks[0].start[b + 1] - c->blocks[0].start[b];
+        const int64_t base = c->voff[(size_t)subepoch * c->d + task];
         if (n) *n = nb;
         if (!out) return;
         if (cap < nb) throw Error(DSGDPP_E_ARG, "output buffer too small");

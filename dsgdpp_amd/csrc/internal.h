@@ -41,7 +41,8 @@ struct DevProblem {
     const void* inv2 = nullptr;             // 1/nnz2[j]
     const double* cnt1 = nullptr;           // nnz1[i] as double (Nzl2 loss, balance)
     const double* cnt2 = nullptr;
-    uint32_t* perm = nullptr;               // nnz: visit order of each block (block-local ids)
+    uint32_t* perm = nullptr;               // visit orders (block-local ids), one run per (task, sub-epoch)
+    const int64_t* voff = nullptr;          // 2d x d: first perm entry of (sub-epoch, task)
 };
 
 struct UpdParams {
@@ -53,9 +54,9 @@ struct UpdParams {
 
 // Work item of the permutation kernel: one non-empty block of one task.
 struct PermJob {
-    int64_t base;        // first point of the block
+    int64_t base;        // first entry of this visit in the perm buffer
     int64_t n;           // points in the block
-    int64_t carry_base;  // first point of the block whose permutation carries over, or -1
+    int64_t carry_base;  // perm entry of the visit whose permutation carries over, or -1
     uint64_t draw;       // first draw index in the task stream
     uint32_t k0, k1;     // task stream key
 };
@@ -65,7 +66,7 @@ void launch_permute_wor(const PermJob* d_jobs, const int32_t* d_chain_start, int
                         uint32_t* perm, int64_t max_block, hipStream_t s);
 void launch_visit_wr(const PermJob* d_jobs, int njobs, uint32_t* perm, hipStream_t s);
 void launch_stratum(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
-                    int use_perm, double eps, hipStream_t s);
+                    const int64_t* d_voff_row, int use_perm, double eps, hipStream_t s);
 double device_nzsl(const DevProblem& P, int which, double* d_partials, double* h_pinned,
                    hipStream_t s);
 double device_sumsq(const DevProblem& P, int which_factor, const double* d_weights,

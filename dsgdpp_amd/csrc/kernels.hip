@@ -168,7 +168,8 @@ __global__ __launch_bounds__(256) void k_visit_wr(const PermJob* __restrict__ jo
 template <typename T, int K, bool STRICT>
 __global__ __launch_bounds__(64) void k_stratum_seq(
     const typename PtOf<T>::type* __restrict__ pts, const int64_t* __restrict__ blk,
-    const uint32_t* __restrict__ perm, const uint32_t* __restrict__ sched, T* W, T* H,
+    const uint32_t* __restrict__ perm, const uint32_t* __restrict__ sched,
+    const int64_t* __restrict__ voff, T* W, T* H,
     const typename St<T>::C* __restrict__ inv1, const typename St<T>::C* __restrict__ inv2,
     int d2, int r, int64_t ld, typename St<T>::C eps, typename St<T>::C lambda, int kind,
     int truncate, typename St<T>::C tmin, typename St<T>::C tmax, int use_perm) {
@@ -178,10 +179,11 @@ __global__ __launch_bounds__(64) void k_stratum_seq(
     const int b = blockIdx.x * d2 + (int)sched[blockIdx.x];
     const int64_t s0 = blk[b];
     const int64_t n = blk[b + 1] - s0;
+    const uint32_t* order = perm + (use_perm ? voff[blockIdx.x] : 0);
     const C m2eps = eps * (C)-2.;
     const C f2 = eps * (C)2. * lambda;
     for (int64_t p = 0; p < n; ++p) {
-        const uint32_t q = use_perm ? perm[s0 + p] : (uint32_t)p;
+        const uint32_t q = use_perm ? order[p] : (uint32_t)p;
         const Pt pt = pts[s0 + q];
         T* wr = W + (int64_t)pt.i * ld;
         T* hr = H + (int64_t)pt.j * ld;
@@ -368,31 +370,31 @@ void launch_visit_wr(const PermJob* d_jobs, int njobs, uint32_t* perm, hipStream
 
 template <typename T, int K, bool STRICT>
 static void stratum_t(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
-                      int use_perm, double eps, hipStream_t s) {
+                      const int64_t* d_voff_row, int use_perm, double eps, hipStream_t s) {
     using C = typename St<T>::C;
     k_stratum_seq<T, K, STRICT><<<P.d, 64, 0, s>>>(
-        (const typename PtOf<T>::type*)P.pts, P.blk, P.perm, d_sched_row, (T*)P.W, (T*)P.H,
+        (const typename PtOf<T>::type*)P.pts, P.blk, P.perm, d_sched_row, d_voff_row, (T*)P.W, (T*)P.H,
         (const C*)P.inv1, (const C*)P.inv2, 2 * P.d, (int)P.r, P.ld, (C)eps, (C)U.lambda, U.kind,
         U.truncate ? 1 : 0, (C)U.tmin, (C)U.tmax, use_perm);
     HIPK(hipGetLastError());
 }
 
 template <typename T, bool STRICT>
-static void stratum_k(const DevProblem& P, const UpdParams& U, const uint32_t* row, int use_perm,
-                      double eps, hipStream_t s) {
-    if (P.r <= 64) stratum_t<T, 1, STRICT>(P, U, row, use_perm, eps, s);
-    else if (P.r <= 128) stratum_t<T, 2, STRICT>(P, U, row, use_perm, eps, s);
-    else if (P.r <= 256) stratum_t<T, 4, STRICT>(P, U, row, use_perm, eps, s);
+static void stratum_k(const DevProblem& P, const UpdParams& U, const uint32_t* row, const int64_t* vrow,
+                      int use_perm, double eps, hipStream_t s) {
+    if (P.r <= 64) stratum_t<T, 1, STRICT>(P, U, row, vrow, use_perm, eps, s);
+    else if (P.r <= 128) stratum_t<T, 2, STRICT>(P, U, row, vrow, use_perm, eps, s);
+    else if (P.r <= 256) stratum_t<T, 4, STRICT>(P, U, row, vrow, use_perm, eps, s);
     else throw std::invalid_argument("rank above 256 is not supported by the GPU path");
 }
 
 void launch_stratum(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
-                    int use_perm, double eps, hipStream_t s) {
+                    const int64_t* d_voff_row, int use_perm, double eps, hipStream_t s) {
     switch (P.precision) {
-    case PREC_FP64_STRICT: stratum_k<double, true>(P, U, d_sched_row, use_perm, eps, s); break;
-    case PREC_FP64: stratum_k<double, false>(P, U, d_sched_row, use_perm, eps, s); break;
-    case PREC_FP32: stratum_k<float, false>(P, U, d_sched_row, use_perm, eps, s); break;
-    case PREC_BF16: stratum_k<bf16_t, false>(P, U, d_sched_row, use_perm, eps, s); break;
+    case PREC_FP64_STRICT: stratum_k<double, true>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
+    case PREC_FP64: stratum_k<double, false>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
+    case PREC_FP32: stratum_k<float, false>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
+    case PREC_BF16: stratum_k<bf16_t, false>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
     default: throw std::invalid_argument("unknown precision");
     }
 }
