@@ -59,6 +59,8 @@ class EpochStats(ctypes.Structure):
         ("sgd_seconds", ctypes.c_double),
         ("updates", ctypes.c_int64),
         ("launches", ctypes.c_int64),
+        ("kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

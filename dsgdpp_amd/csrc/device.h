@@ -1,0 +1,115 @@
+// Device-side helpers shared by the gfx950 kernels: storage types (FP64,
+// FP32, BF16 with FP32 math), wave64 reductions, and the agent-scope
+// completion counters of the dataflow epoch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "internal.h"
+
+namespace dsgdpp {
+
+#define HIPK(call)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                     " at " #call);                                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// storage types
+// ---------------------------------------------------------------------------
+struct bf16_t {
+    uint16_t u;
+};
+
+template <typename T>
+struct St;
+template <>
+struct St<double> {
+    using C = double;
+    static __device__ __forceinline__ C load(const double* p) { return *p; }
+    static __device__ __forceinline__ void store(double* p, C v) { *p = v; }
+    static __device__ __forceinline__ double wide(const double* p) { return *p; }
+};
+template <>
+struct St<float> {
+    using C = float;
+    static __device__ __forceinline__ C load(const float* p) { return *p; }
+    static __device__ __forceinline__ void store(float* p, C v) { *p = v; }
+    static __device__ __forceinline__ double wide(const float* p) { return (double)*p; }
+};
+__device__ __forceinline__ float bf16_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even
+    uint32_t b = __float_as_uint(f);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);   // quiet NaN
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+template <>
+struct St<bf16_t> {
+    using C = float;
+    static __device__ __forceinline__ C load(const bf16_t* p) { return bf16_to_f32(p->u); }
+    static __device__ __forceinline__ void store(bf16_t* p, C v) { p->u = f32_to_bf16(v); }
+    static __device__ __forceinline__ double wide(const bf16_t* p) { return (double)bf16_to_f32(p->u); }
+};
+
+template <typename T>
+struct PtOf {
+    using type = Pt32;
+};
+template <>
+struct PtOf<double> {
+    using type = Pt64;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ double readlane(double v, int lane) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = readlane_u32((uint32_t)u, lane);
+    const uint32_t hi = readlane_u32((uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float readlane(float v, int lane) {
+    return __uint_as_float(readlane_u32(__float_as_uint(v), lane));
+}
+
+// Butterfly sum: every lane ends with the bitwise identical total.
+template <typename C>
+__device__ __forceinline__ C wave_sum(C v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void dataflow_wait(int32_t* ready, int b2, int32_t need) {
+    if (need <= 0) return;
+    while (__hip_atomic_load(ready + b2, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+        __builtin_amdgcn_s_sleep(1);
+}
+
+__device__ __forceinline__ void dataflow_signal(int32_t* ready, int b2) {
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(ready + b2, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+
+// Sum over the 16 lanes of a DPP row; every lane of the row gets the bitwise
+// identical total (each step adds a lane pair in both orders).
+__device__ __forceinline__ float row16_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+    return v;
+}
+
+}  // namespace dsgdpp

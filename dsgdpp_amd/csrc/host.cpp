@@ -155,6 +155,30 @@ struct HostBlocks {
     bool loaded = false;
 };
 
+// Everything one epoch needs besides eps and the factors.
+struct EpochPlan {
+    bool drawn = false, built = false, grouped = false, dataflow = false;
+    uint64_t dataVersion = 0;
+    std::vector<uint32_t> schedule;   // 2d x d
+    std::vector<uint64_t> seeds;      // d
+    std::vector<int64_t> voff;        // 2d x d
+    std::vector<int32_t> need;        // 2d x d
+    std::vector<PermJob> jobs;
+    std::vector<int32_t> chains;
+    uint32_t* dSched = nullptr;
+    int64_t* dVoff = nullptr;
+    int32_t* dNeed = nullptr;
+    PermJob* dJobs = nullptr;
+    int32_t* dChains = nullptr;
+    uint32_t* perm = nullptr;
+    Pt32* grec = nullptr;
+    uint2* ginfo = nullptr;
+    int32_t* ngroups = nullptr;
+    size_t jobsCap = 0, chainsCap = 0;
+    int64_t permCap = 0, groupCap = 0;
+    hipEvent_t evStart = nullptr, evEnd = nullptr;
+};
+
 }  // namespace
 
 struct dsgdpp_ctx {
@@ -182,16 +206,16 @@ struct dsgdpp_ctx {
     int64_t* dBlk[2] = {nullptr, nullptr};
     double* dPartials = nullptr;
     double* hPinned = nullptr;
-    uint32_t* dSched = nullptr;
-    PermJob* dJobs = nullptr;
-    int32_t* dChains = nullptr;
-    int64_t* dVoff = nullptr;
-    size_t jobsCap = 0, chainsCap = 0;
-    int64_t permCap = 0;
-
-    std::vector<uint32_t> schedule;
-    std::vector<int64_t> voff;   // 2d x d: first perm entry of each (sub-epoch, task) visit
-    std::vector<uint64_t> seeds;
+    int32_t* dReady = nullptr;
+    bool dataflowOk = false;
+    bool groupedOk = false;
+    std::string groupedWhy;
+    GroupedBuffers G;
+    hipStream_t prepStream = nullptr;
+    EpochPlan plan[2];
+    int cur = 0, lastPlan = -1;
+    bool lookahead = true;
+    uint64_t dataVersion = 0;
 
     size_t factorBytes() const {
         return precision <= DSGDPP_FP64 ? 8 : precision == DSGDPP_FP32 ? 4 : 2;
@@ -409,6 +433,23 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
         c->P.pts = c->dPts[0];
         c->P.blk = c->dBlk[0];
         c->P.nnz = hb.nnz;
+        // grouped epoch: block extents and eligibility
+        int64_t maxBlock = 0;
+        for (size_t b = 0; b + 1 < hb.start.size(); ++b) maxBlock = std::max(maxBlock, hb.start[b + 1] - hb.start[b]);
+        int maxRows = 0, maxCols = 0;
+        for (size_t b = 0; b + 1 < c->off1.size(); ++b) maxRows = std::max<int>(maxRows, (int)(c->off1[b + 1] - c->off1[b]));
+        for (size_t b = 0; b + 1 < c->off2.size(); ++b) maxCols = std::max<int>(maxCols, (int)(c->off2[b + 1] - c->off2[b]));
+        c->G.max_rows = maxRows;
+        c->G.max_cols = maxCols;
+        c->groupedOk = grouped_supported(c->P, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy);
+        if (c->groupedOk) {
+            std::vector<int32_t> o2(c->off2.begin(), c->off2.end());
+            int32_t* d2p = const_cast<int32_t*>(c->G.off2);
+            dalloc(d2p, o2.size() * sizeof(int32_t));
+            HIPC(hipMemcpy(d2p, o2.data(), o2.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+            c->G.off2 = d2p;
+        }
+        if (c->kernel == 3 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "grouped kernel: " + c->groupedWhy);
     } else {
         c->P.test_pts = c->dPts[1];
         c->P.test_blk = c->dBlk[1];
@@ -429,6 +470,8 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
             throw Error(DSGDPP_E_ARG, "index out of range at entry " + std::to_string(p));
     if (which == DSGDPP_TRAIN && nnz >= (int64_t)UINT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "too many nonzeros");
     const bool first = c->m < 0;
+    HIPC(hipStreamSynchronize(c->prepStream));   // a look-ahead plan may still read the old blocks
+    ++c->dataVersion;
     c->m = m;
     c->n = n;
     c->off1 = defaultOffsets(m, c->d);
@@ -441,6 +484,7 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
         c->P.precision = c->precision;
         const int64_t vec = 16 / (int64_t)c->factorBytes();
         c->P.ld = (c->r + vec - 1) / vec * vec;
+        c->dataflowOk = dataflow_fits(c->P, c->device);
     }
     if (c->fp64()) uploadBlocks<Pt64>(c, which, nnz, I, J, X);
     else uploadBlocks<Pt32>(c, which, nnz, I, J, X);
@@ -500,10 +544,14 @@ void requireReady(dsgdpp_ctx* c) {
 
 // ---------------------------------------------------------------------------
 // one epoch: DsgdPpRunner::epoch (mf/sgd/dsgdpp_impl.h:282-349)
+//
+// An epoch's plan -- schedule, task seeds, visit orders and groups -- depends
+// only on the master stream and the data, not on eps or the factors.  So while
+// epoch e runs on the main stream, the plan of epoch e+1 is drawn (in the same
+// master-stream order as without look-ahead) and its visit-order kernel runs
+// on a second stream; epoch e+1 then only waits for that kernel's event.
 // ---------------------------------------------------------------------------
-void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
-    requireReady(c);
-    const auto t0 = std::chrono::steady_clock::now();
+void drawPlan(dsgdpp_ctx* c, EpochPlan& P) {
     const uint32_t d = c->d, d2 = 2 * d;
     Stream& rng = c->master;
     // column-block halves for even / odd sub-epochs (:289-323)
@@ -522,54 +570,66 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     // two d x d schedules interleaved into 2d x d (:327-336)
     const std::vector<uint32_t> s0 = dsgdSchedule(c->w, c->t, c->stratumOrder, rng);
     const std::vector<uint32_t> s1 = dsgdSchedule(c->w, c->t, c->stratumOrder, rng);
-    c->schedule.assign((size_t)d2 * d, 0);
+    P.schedule.assign((size_t)d2 * d, 0);
     for (uint32_t i = 0; i < d; ++i)
         for (uint32_t id = 0; id < d; ++id) {
-            c->schedule[(size_t)(2 * i) * d + id] = even[s0[(size_t)i * d + id]];
-            c->schedule[(size_t)(2 * i + 1) * d + id] = odd[s1[(size_t)i * d + id]];
+            P.schedule[(size_t)(2 * i) * d + id] = even[s0[(size_t)i * d + id]];
+            P.schedule[(size_t)(2 * i + 1) * d + id] = odd[s1[(size_t)i * d + id]];
         }
     // task seeds (mpi2::seed, :344): one master draw per task, in task order
-    c->seeds.resize(d);
-    for (uint32_t id = 0; id < d; ++id) c->seeds[id] = rng.u64();
+    P.seeds.resize(d);
+    for (uint32_t id = 0; id < d; ++id) P.seeds[id] = rng.u64();
+    P.drawn = true;
+    P.built = false;
+}
 
-    // visit-order work: each task's stream is consumed block after block in
-    // sub-epoch order (one SgdRunner per task per epoch, :112).  Every
-    // (task, sub-epoch) visit gets its own run of the perm buffer: a WR stratum
-    // order may send a task to the same block twice in one epoch.
+// Visit-order jobs, dataflow dependencies and the visit-order kernel of a
+// drawn plan, all enqueued on `s`.
+void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
+    const uint32_t d = c->d, d2 = 2 * d;
     const std::vector<int64_t>& bs = c->blocks[0].start;
-    auto visitBlock = [&](uint32_t id, uint32_t s) {
-        return (size_t)id * d2 + c->schedule[(size_t)s * d + id];
-    };
-    c->voff.assign((size_t)d2 * d, 0);
+    auto visitBlock = [&](uint32_t id, uint32_t sub) { return (size_t)id * d2 + P.schedule[(size_t)sub * d + id]; };
+    // every (task, sub-epoch) visit gets its own run of the visit-order buffers:
+    // a WR stratum order may send a task to the same block twice in one epoch
+    P.voff.assign((size_t)d2 * d, 0);
     int64_t total = 0;
     for (uint32_t id = 0; id < d; ++id)
-        for (uint32_t s = 0; s < d2; ++s) {
-            const size_t b = visitBlock(id, s);
-            c->voff[(size_t)s * d + id] = total;
+        for (uint32_t sub = 0; sub < d2; ++sub) {
+            const size_t b = visitBlock(id, sub);
+            P.voff[(size_t)sub * d + id] = total;
             total += bs[b + 1] - bs[b];
         }
+    P.grouped = c->groupedOk && (c->kernel == 0 || c->kernel == 3);
+    P.dataflow = P.grouped || (c->kernel != 1 && c->dataflowOk);
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
     int64_t maxBlock = 0;
-    if (c->sgdOrder != DSGDPP_SGD_SEQ) {
+    if (c->sgdOrder != DSGDPP_SGD_SEQ || P.grouped) {
+        // each task's stream is consumed block after block in sub-epoch order
+        // (one SgdRunner per task per epoch, :112)
         std::vector<std::vector<PermJob>> perTask(d);
         parallelFor(d, [&](int64_t lo, int64_t hi) {
             for (int64_t id = lo; id < hi; ++id) {
                 uint32_t k0, k1;
-                task_key(c->seeds[id], &k0, &k1);
+                task_key(P.seeds[id], &k0, &k1);
                 uint64_t draw = 0;
                 int64_t lastN = 0, lastBase = -1;
-                for (uint32_t s = 0; s < d2; ++s) {
-                    const size_t b = visitBlock((uint32_t)id, s);
+                for (uint32_t sub = 0; sub < d2; ++sub) {
+                    const size_t b = visitBlock((uint32_t)id, sub);
                     const int64_t nb = bs[b + 1] - bs[b];
-                    const int64_t base = c->voff[(size_t)s * d + id];
+                    const int64_t base = P.voff[(size_t)sub * d + id];
+                    const uint32_t b2 = P.schedule[(size_t)sub * d + id];
+                    PermJob j{base, nb, -1, draw, k0, k1, bs[b], (int32_t)c->off1[id], (int32_t)c->off2[b2],
+                              (int32_t)(sub * d + id), 0};
                     if (c->sgdOrder == DSGDPP_SGD_WR) {
-                        if (nb > 0) perTask[id].push_back(PermJob{base, nb, -1, draw, k0, k1});
+                        if (nb > 0) perTask[id].push_back(j);
                         draw += (uint64_t)nb + 2;
+                    } else if (c->sgdOrder == DSGDPP_SGD_SEQ) {
+                        if (nb > 0) perTask[id].push_back(j);
                     } else {
                         if (nb == 0) continue;
-                        const int64_t carry = (nb == lastN) ? lastBase : -1;
-                        perTask[id].push_back(PermJob{base, nb, carry, draw, k0, k1});
+                        j.carry_base = (nb == lastN) ? lastBase : -1;
+                        perTask[id].push_back(j);
                         draw += (uint64_t)nb;
                         lastN = nb;
                         lastBase = base;
@@ -579,57 +639,133 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
         });
         for (uint32_t id = 0; id < d; ++id)
             for (const PermJob& j : perTask[id]) {
-                if (c->sgdOrder == DSGDPP_SGD_WOR && j.carry_base < 0) chains.push_back((int32_t)jobs.size());
+                // a chain is a run of visits whose permutations carry over (WOR only)
+                if (!(c->sgdOrder == DSGDPP_SGD_WOR && j.carry_base >= 0)) chains.push_back((int32_t)jobs.size());
                 jobs.push_back(j);
                 maxBlock = std::max(maxBlock, j.n);
             }
         chains.push_back((int32_t)jobs.size());
-        if (total > c->permCap) {
-            c->permCap = total;
-            dalloc(c->P.perm, (size_t)total * sizeof(uint32_t));
+        if (total > P.permCap) {
+            P.permCap = total;
+            dalloc(P.perm, (size_t)total * sizeof(uint32_t));
         }
+        if (P.grouped && total > P.groupCap) {
+            P.groupCap = total;
+            dalloc(P.grec, (size_t)total * GROUP_SLOTS * sizeof(Pt32));
+            dalloc(P.ginfo, (size_t)total * sizeof(uint2));
+        }
+        if (P.grouped && !P.ngroups) dalloc(P.ngroups, (size_t)d2 * d * sizeof(int32_t));
     }
-    if (jobs.size() > c->jobsCap) {
-        c->jobsCap = jobs.size() * 2;
-        dalloc(c->dJobs, c->jobsCap * sizeof(PermJob));
+    if (jobs.size() > P.jobsCap) {
+        P.jobsCap = jobs.size() * 2;
+        dalloc(P.dJobs, P.jobsCap * sizeof(PermJob));
     }
-    if (chains.size() > c->chainsCap) {
-        c->chainsCap = chains.size() * 2;
-        dalloc(c->dChains, c->chainsCap * sizeof(int32_t));
+    if (chains.size() > P.chainsCap) {
+        P.chainsCap = chains.size() * 2;
+        dalloc(P.dChains, P.chainsCap * sizeof(int32_t));
     }
-    if (!c->dSched) dalloc(c->dSched, (size_t)d2 * d * sizeof(uint32_t));
-    if (!c->dVoff) dalloc(c->dVoff, (size_t)d2 * d * sizeof(int64_t));
-    HIPC(hipMemcpyAsync(c->dSched, c->schedule.data(), c->schedule.size() * sizeof(uint32_t),
-                        hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemcpyAsync(c->dVoff, c->voff.data(), c->voff.size() * sizeof(int64_t), hipMemcpyHostToDevice,
-                        c->stream));
-    if (!jobs.empty())
-        HIPC(hipMemcpyAsync(c->dJobs, jobs.data(), jobs.size() * sizeof(PermJob), hipMemcpyHostToDevice, c->stream));
-    if (!chains.empty())
-        HIPC(hipMemcpyAsync(c->dChains, chains.data(), chains.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-
-    HIPC(hipEventRecord(c->ev[0], c->stream));
-    if (c->sgdOrder == DSGDPP_SGD_WOR)
-        launch_permute_wor(c->dJobs, c->dChains, (int)chains.size() - 1, c->P.perm, maxBlock, c->stream);
+    if (!P.dSched) dalloc(P.dSched, (size_t)d2 * d * sizeof(uint32_t));
+    if (!P.dVoff) dalloc(P.dVoff, (size_t)d2 * d * sizeof(int64_t));
+    // dataflow dependencies: a visit to column block b2 at sub-epoch s waits for
+    // the non-empty visits to b2 in sub-epochs < s
+    P.need.assign((size_t)d2 * d, 0);
+    if (P.dataflow) {
+        std::vector<int32_t> cnt(d2, 0);
+        for (uint32_t sub = 0; sub < d2; ++sub) {
+            for (uint32_t id = 0; id < d; ++id) P.need[(size_t)sub * d + id] = cnt[P.schedule[(size_t)sub * d + id]];
+            for (uint32_t id = 0; id < d; ++id) {
+                const size_t b = visitBlock(id, sub);
+                if (bs[b + 1] > bs[b]) cnt[P.schedule[(size_t)sub * d + id]]++;
+            }
+        }
+        if (!P.dNeed) dalloc(P.dNeed, P.need.size() * sizeof(int32_t));
+    }
+    // host staging stays alive until the copies complete (the plan owns it)
+    P.jobs.swap(jobs);
+    P.chains.swap(chains);
+    HIPC(hipMemcpyAsync(P.dSched, P.schedule.data(), P.schedule.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(P.dVoff, P.voff.data(), P.voff.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (P.dataflow)
+        HIPC(hipMemcpyAsync(P.dNeed, P.need.data(), P.need.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (!P.jobs.empty())
+        HIPC(hipMemcpyAsync(P.dJobs, P.jobs.data(), P.jobs.size() * sizeof(PermJob), hipMemcpyHostToDevice, s));
+    if (!P.chains.empty())
+        HIPC(hipMemcpyAsync(P.dChains, P.chains.data(), P.chains.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    DevProblem Dp = c->P;
+    Dp.perm = P.perm;
+    GroupedBuffers G = c->G;
+    G.grec = P.grec;
+    G.ginfo = P.ginfo;
+    G.ngroups = P.ngroups;
+    const int nchains = (int)P.chains.size() - 1;
+    HIPC(hipEventRecord(P.evStart, s));
+    if (P.grouped)
+        launch_prep_grouped(Dp, G, P.dJobs, P.dChains, nchains, c->sgdOrder, maxBlock, s);
+    else if (c->sgdOrder == DSGDPP_SGD_WOR)
+        launch_permute_wor(P.dJobs, P.dChains, nchains, P.perm, maxBlock, s);
     else if (c->sgdOrder == DSGDPP_SGD_WR)
-        launch_visit_wr(c->dJobs, (int)jobs.size(), c->P.perm, c->stream);
+        launch_visit_wr(P.dJobs, (int)P.jobs.size(), P.perm, s);
+    HIPC(hipEventRecord(P.evEnd, s));
+    P.built = true;
+    P.dataVersion = c->dataVersion;
+}
+
+void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
+    requireReady(c);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t d = c->d, d2 = 2 * d;
+    EpochPlan& P = c->plan[c->cur];
+    if (!P.drawn) drawPlan(c, P);
+    if (!P.built || P.dataVersion != c->dataVersion) {
+        HIPC(hipStreamSynchronize(c->prepStream));
+        buildPlan(c, P, c->prepStream);
+    }
+    HIPC(hipStreamWaitEvent(c->stream, P.evEnd, 0));
+    if (P.dataflow && !c->dReady) dalloc(c->dReady, (size_t)d2 * sizeof(int32_t));
     HIPC(hipEventRecord(c->ev[1], c->stream));
-    for (uint32_t s = 0; s < d2; ++s)
-        launch_stratum(c->P, c->upd, c->dSched + (size_t)s * d, c->dVoff + (size_t)s * d,
-                       c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1, eps, c->stream);
+    EpochLaunch L;
+    L.sched = P.dSched;
+    L.voff = P.dVoff;
+    L.need = P.dNeed;
+    L.ready = c->dReady;
+    L.use_perm = c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1;
+    L.dataflow = P.dataflow ? 1 : 0;
+    L.eps = eps;
+    DevProblem Dp = c->P;
+    Dp.perm = P.perm;
+    if (P.grouped) {
+        GroupedBuffers G = c->G;
+        G.grec = P.grec;
+        G.ginfo = P.ginfo;
+        G.ngroups = P.ngroups;
+        launch_epoch_grouped(Dp, c->upd, G, L, c->stream);
+    } else {
+        launch_epoch(Dp, c->upd, L, c->stream);
+    }
     HIPC(hipEventRecord(c->ev[2], c->stream));
+    const int used = c->cur;
+    c->lastPlan = used;
+    c->cur ^= 1;
+    // look ahead: the next epoch's plan, drawn now in master-stream order
+    if (c->lookahead) {
+        EpochPlan& N = c->plan[c->cur];
+        drawPlan(c, N);
+        buildPlan(c, N, c->prepStream);
+    }
     HIPC(hipEventSynchronize(c->ev[2]));
     const auto t1 = std::chrono::steady_clock::now();
     if (st) {
         float a = 0, b = 0;
-        HIPC(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIPC(hipEventSynchronize(P.evEnd));
+        HIPC(hipEventElapsedTime(&a, P.evStart, P.evEnd));
         HIPC(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
         st->perm_seconds = a / 1e3;
         st->sgd_seconds = b / 1e3;
-        st->device_seconds = (a + b) / 1e3;
+        st->device_seconds = b / 1e3;
         st->updates = c->blocks[0].nnz;
-        st->launches = d2;
+        st->launches = P.dataflow ? 1 : d2;
+        st->kernel = P.grouped ? 3 : P.dataflow ? 2 : 1;
     }
 }
 
@@ -940,7 +1076,14 @@ int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out) {
         if (c->device < 0 || c->device >= count) throw Error(DSGDPP_E_DEVICE, "no such HIP device");
         HIPC(hipSetDevice(c->device));
         HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&c->prepStream, hipStreamNonBlocking));
         for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+        for (EpochPlan& P : c->plan) {
+            HIPC(hipEventCreate(&P.evStart));
+            HIPC(hipEventCreate(&P.evEnd));
+        }
+        const char* la = getenv("DSGDPP_LOOKAHEAD");
+        c->lookahead = !(la && la[0] == '0');
         dalloc(c->dPartials, (size_t)partials_capacity() * sizeof(double));
         HIPC(hipHostMalloc((void**)&c->hPinned, sizeof(double) * 8, hipHostMallocDefault));
     } catch (const std::exception& e) {
@@ -957,7 +1100,22 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->prepStream) (void)hipStreamSynchronize(c->prepStream);
     for (int k = 0; k < 2; ++k) { dfree(c->dPts[k]); void* p = c->dBlk[k]; dfree(p); }
+    for (EpochPlan& P : c->plan) {
+        void* q;
+        q = P.dSched; dfree(q);
+        q = P.dVoff; dfree(q);
+        q = P.dNeed; dfree(q);
+        q = P.dJobs; dfree(q);
+        q = P.dChains; dfree(q);
+        q = P.perm; dfree(q);
+        q = P.grec; dfree(q);
+        q = P.ginfo; dfree(q);
+        q = P.ngroups; dfree(q);
+        if (P.evStart) (void)hipEventDestroy(P.evStart);
+        if (P.evEnd) (void)hipEventDestroy(P.evEnd);
+    }
     dfree(c->P.W);
     dfree(c->P.H);
     void* v;
@@ -965,16 +1123,14 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     v = const_cast<void*>(c->P.inv2); dfree(v);
     v = const_cast<double*>(c->P.cnt1); dfree(v);
     v = const_cast<double*>(c->P.cnt2); dfree(v);
-    v = c->P.perm; dfree(v);
     v = c->dPartials; dfree(v);
-    v = c->dSched; dfree(v);
-    v = c->dJobs; dfree(v);
-    v = c->dChains; dfree(v);
-    v = c->dVoff; dfree(v);
+    v = const_cast<int32_t*>(c->G.off2); dfree(v);
+    v = c->dReady; dfree(v);
     if (c->hPinned) (void)hipHostFree(c->hPinned);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->prepStream) (void)hipStreamDestroy(c->prepStream);
     delete c;
 }
 
@@ -1072,29 +1228,32 @@ int dsgdpp_run(dsgdpp_ctx* c, const dsgdpp_run_cfg* rc) {
 
 int dsgdpp_get_schedule(dsgdpp_ctx* c, uint32_t* out) {
     return guarded(c, [&] {
-        if (c->schedule.empty()) throw Error(DSGDPP_E_STATE, "no epoch has run");
+        if (c->lastPlan < 0) throw Error(DSGDPP_E_STATE, "no epoch has run");
         if (!out) throw Error(DSGDPP_E_ARG, "null output");
-        std::copy(c->schedule.begin(), c->schedule.end(), out);
+        const auto& sch = c->plan[c->lastPlan].schedule;
+        std::copy(sch.begin(), sch.end(), out);
     });
 }
 
 int dsgdpp_get_task_seeds(dsgdpp_ctx* c, uint64_t* out) {
     return guarded(c, [&] {
-        if (c->seeds.empty()) throw Error(DSGDPP_E_STATE, "no epoch has run");
+        if (c->lastPlan < 0) throw Error(DSGDPP_E_STATE, "no epoch has run");
         if (!out) throw Error(DSGDPP_E_ARG, "null output");
-        std::copy(c->seeds.begin(), c->seeds.end(), out);
+        const auto& sd = c->plan[c->lastPlan].seeds;
+        std::copy(sd.begin(), sd.end(), out);
     });
 }
 
 int dsgdpp_get_permutation(dsgdpp_ctx* c, int32_t task, int32_t subepoch, uint32_t* out, int64_t cap,
                            int64_t* n) {
     return guarded(c, [&] {
-        if (c->schedule.empty()) throw Error(DSGDPP_E_STATE, "no epoch has run");
+        if (c->lastPlan < 0) throw Error(DSGDPP_E_STATE, "no epoch has run");
         if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)(2 * c->d))
             throw Error(DSGDPP_E_ARG, "task or sub-epoch out of range");
-        const size_t b = (size_t)task * 2 * c->d + c->schedule[(size_t)subepoch * c->d + task];
+        const EpochPlan& Pl = c->plan[c->lastPlan];
+        const size_t b = (size_t)task * 2 * c->d + Pl.schedule[(size_t)subepoch * c->d + task];
         const int64_t nb = c->blocks[0].start[b + 1] - c->blocks[0].start[b];
-        const int64_t base = c->voff[(size_t)subepoch * c->d + task];
+        const int64_t base = Pl.voff[(size_t)subepoch * c->d + task];
         if (n) *n = nb;
         if (!out) return;
         if (cap < nb) throw Error(DSGDPP_E_ARG, "output buffer too small");
@@ -1102,7 +1261,7 @@ int dsgdpp_get_permutation(dsgdpp_ctx* c, int32_t task, int32_t subepoch, uint32
             for (int64_t k = 0; k < nb; ++k) out[k] = (uint32_t)k;
         } else if (nb > 0) {
             HIPC(hipStreamSynchronize(c->stream));
-            HIPC(hipMemcpy(out, c->P.perm + base, (size_t)nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(out, Pl.perm + base, (size_t)nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
         }
     });
 }

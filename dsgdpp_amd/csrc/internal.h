@@ -59,14 +59,52 @@ struct PermJob {
     int64_t carry_base;  // perm entry of the visit whose permutation carries over, or -1
     uint64_t draw;       // first draw index in the task stream
     uint32_t k0, k1;     // task stream key
+    int64_t s0;          // first point of the block in the blocked point array
+    int32_t row0, col0;  // first global row / column of the block
+    int32_t visit;       // s * d + t
+    int32_t pad;
 };
 
 // kernels.hip
 void launch_permute_wor(const PermJob* d_jobs, const int32_t* d_chain_start, int nchains,
                         uint32_t* perm, int64_t max_block, hipStream_t s);
 void launch_visit_wr(const PermJob* d_jobs, int njobs, uint32_t* perm, hipStream_t s);
-void launch_stratum(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
-                    const int64_t* d_voff_row, int use_perm, double eps, hipStream_t s);
+// Device work of one epoch's SGD part.  dataflow: one persistent launch (a
+// workgroup per task, completion counters per column block); otherwise one
+// launch per stratum.
+struct EpochLaunch {
+    const uint32_t* sched = nullptr;   // 2d x d
+    const int64_t* voff = nullptr;     // 2d x d
+    const int32_t* need = nullptr;     // 2d x d (dataflow)
+    int32_t* ready = nullptr;          // 2d (dataflow)
+    int use_perm = 1;
+    int dataflow = 0;
+    double eps = 0;
+};
+void launch_epoch(const DevProblem& P, const UpdParams& U, const EpochLaunch& L, hipStream_t s);
+bool dataflow_fits(const DevProblem& P, int device);
+
+// kernels_grouped.hip: the grouped epoch.  A prep kernel turns every visit
+// into its visit order, the visited records in that order, and groups of up to
+// GROUP_SLOTS consecutive independent points; the epoch kernel runs one group
+// per wave step (16 lanes per point, H block staged in LDS).
+constexpr int GROUP_SLOTS = 4;
+constexpr int GROUP_RING = 16;    // W forwarding window, in groups
+struct GroupedBuffers {
+    void* grec = nullptr;         // point records in (group, slot) layout (Pt32), 4 per group
+    uint2* ginfo = nullptr;       // per group: size, forwarding bytes
+    int32_t* ngroups = nullptr;   // 2d x d
+    const int32_t* off2 = nullptr;   // 2d+1 column block offsets (device)
+    int max_cols = 0;             // widest column block
+    int max_rows = 0;             // tallest row block
+};
+bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols,
+                       int device, std::string* why);
+void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
+                         const int32_t* d_chain_start, int nchains, int order, int64_t max_block,
+                         hipStream_t s);
+void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                          const EpochLaunch& L, hipStream_t s);
 double device_nzsl(const DevProblem& P, int which, double* d_partials, double* h_pinned,
                    hipStream_t s);
 double device_sumsq(const DevProblem& P, int which_factor, const double* d_weights,

@@ -14,87 +14,9 @@
 
 #include "internal.h"
 #include "philox.h"
+#include "device.h"
 
 namespace dsgdpp {
-
-#define HIPK(call)                                                                  \
-    do {                                                                            \
-        hipError_t e_ = (call);                                                     \
-        if (e_ != hipSuccess)                                                       \
-            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
-                                     " at " #call);                                 \
-    } while (0)
-
-// ---------------------------------------------------------------------------
-// storage types
-// ---------------------------------------------------------------------------
-struct bf16_t {
-    uint16_t u;
-};
-
-template <typename T>
-struct St;
-template <>
-struct St<double> {
-    using C = double;
-    static __device__ __forceinline__ C load(const double* p) { return *p; }
-    static __device__ __forceinline__ void store(double* p, C v) { *p = v; }
-    static __device__ __forceinline__ double wide(const double* p) { return *p; }
-};
-template <>
-struct St<float> {
-    using C = float;
-    static __device__ __forceinline__ C load(const float* p) { return *p; }
-    static __device__ __forceinline__ void store(float* p, C v) { *p = v; }
-    static __device__ __forceinline__ double wide(const float* p) { return (double)*p; }
-};
-__device__ __forceinline__ float bf16_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even
-    uint32_t b = __float_as_uint(f);
-    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);   // quiet NaN
-    b += 0x7fffu + ((b >> 16) & 1u);
-    return (uint16_t)(b >> 16);
-}
-template <>
-struct St<bf16_t> {
-    using C = float;
-    static __device__ __forceinline__ C load(const bf16_t* p) { return bf16_to_f32(p->u); }
-    static __device__ __forceinline__ void store(bf16_t* p, C v) { p->u = f32_to_bf16(v); }
-    static __device__ __forceinline__ double wide(const bf16_t* p) { return (double)bf16_to_f32(p->u); }
-};
-
-template <typename T>
-struct PtOf {
-    using type = Pt32;
-};
-template <>
-struct PtOf<double> {
-    using type = Pt64;
-};
-
-// ---------------------------------------------------------------------------
-// wave helpers (wave64)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
-}
-__device__ __forceinline__ double readlane(double v, int lane) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const uint32_t lo = readlane_u32((uint32_t)u, lane);
-    const uint32_t hi = readlane_u32((uint32_t)(u >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ float readlane(float v, int lane) {
-    return __uint_as_float(readlane_u32(__float_as_uint(v), lane));
-}
-
-// Butterfly sum: every lane ends with the bitwise identical total.
-template <typename C>
-__device__ __forceinline__ C wave_sum(C v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // Visit orders.
@@ -159,34 +81,48 @@ __global__ __launch_bounds__(256) void k_visit_wr(const PermJob* __restrict__ jo
 }
 
 // ---------------------------------------------------------------------------
-// Stratum kernel: one wave per block of the stratum.  Workgroup t runs block
-// (t, sched[t]) -- task t's work in this sub-epoch (dsgdpp_impl.h:122-123) --
-// visiting its points in the block's visit order (sgd_impl.h:293-351) and
-// applying the update functor to W row i and H column j, lane z holding
-// element z (+64k) of both vectors.
+// Sequential block body: one wave walks a block in its visit order
+// (sgd_impl.h:293-351) and applies the update functor to W row i and H
+// column j, lane z holding element z (+64k) of both vectors.  Each lane only
+// touches its own elements, so a point's loads see the previous point's
+// stores through same-thread program order.
 // ---------------------------------------------------------------------------
+template <typename T>
+struct SgdArgs {
+    using C = typename St<T>::C;
+    const typename PtOf<T>::type* __restrict__ pts;
+    const int64_t* __restrict__ blk;
+    const uint32_t* __restrict__ perm;
+    const uint32_t* __restrict__ sched;   // 2d x d (dataflow) or one row (per stratum)
+    const int64_t* __restrict__ voff;     // same shape as sched
+    const int32_t* __restrict__ need;     // 2d x d: earlier non-empty visits of the column block
+    int32_t* ready;                       // 2d: completed visits per column block
+    T* W;
+    T* H;
+    const C* __restrict__ inv1;
+    const C* __restrict__ inv2;
+    int d, r;
+    int64_t ld;
+    C eps, lambda, tmin, tmax;
+    int kind, truncate, use_perm;
+};
+
 template <typename T, int K, bool STRICT>
-__global__ __launch_bounds__(64) void k_stratum_seq(
-    const typename PtOf<T>::type* __restrict__ pts, const int64_t* __restrict__ blk,
-    const uint32_t* __restrict__ perm, const uint32_t* __restrict__ sched,
-    const int64_t* __restrict__ voff, T* W, T* H,
-    const typename St<T>::C* __restrict__ inv1, const typename St<T>::C* __restrict__ inv2,
-    int d2, int r, int64_t ld, typename St<T>::C eps, typename St<T>::C lambda, int kind,
-    int truncate, typename St<T>::C tmin, typename St<T>::C tmax, int use_perm) {
+__device__ __forceinline__ void run_block_seq(const SgdArgs<T>& A, int64_t s0, int64_t n,
+                                              const uint32_t* order) {
     using C = typename St<T>::C;
     using Pt = typename PtOf<T>::type;
     const int lane = threadIdx.x;
-    const int b = blockIdx.x * d2 + (int)sched[blockIdx.x];
-    const int64_t s0 = blk[b];
-    const int64_t n = blk[b + 1] - s0;
-    const uint32_t* order = perm + (use_perm ? voff[blockIdx.x] : 0);
+    const C eps = A.eps;
     const C m2eps = eps * (C)-2.;
-    const C f2 = eps * (C)2. * lambda;
+    const C f2 = eps * (C)2. * A.lambda;
+    const int r = A.r;
+    const int kind = A.kind;
     for (int64_t p = 0; p < n; ++p) {
-        const uint32_t q = use_perm ? order[p] : (uint32_t)p;
-        const Pt pt = pts[s0 + q];
-        T* wr = W + (int64_t)pt.i * ld;
-        T* hr = H + (int64_t)pt.j * ld;
+        const uint32_t q = A.use_perm ? order[p] : (uint32_t)p;
+        const Pt pt = A.pts[s0 + q];
+        T* wr = A.W + (int64_t)pt.i * A.ld;
+        T* hr = A.H + (int64_t)pt.j * A.ld;
         C w[K], h[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -216,8 +152,8 @@ __global__ __launch_bounds__(64) void k_stratum_seq(
         const C f1 = m2eps * (x - wh);
         C a = (C)0, bb = (C)0;
         if (kind == UPD_NZSL_L2) {
-            a = f2 * inv1[pt.i];
-            bb = f2 * inv2[pt.j];
+            a = f2 * A.inv1[pt.i];
+            bb = f2 * A.inv2[pt.j];
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -228,8 +164,8 @@ __global__ __launch_bounds__(64) void k_stratum_seq(
                     wn = w[k] - f1 * h[k];
                     hn = h[k] - f1 * t;
                 } else if (kind == UPD_NZSL_L2) {       // update-nzsl-l2.h:49-55
-                    wn = w[k] - (f1 * h[k] + f2 * w[k] * inv1[pt.i]);
-                    hn = h[k] - (f1 * t + f2 * h[k] * inv2[pt.j]);
+                    wn = w[k] - (f1 * h[k] + f2 * w[k] * A.inv1[pt.i]);
+                    hn = h[k] - (f1 * t + f2 * h[k] * A.inv2[pt.j]);
                 } else {                                // update-nzsl-nzl2.h:46-51
                     wn = w[k] - (f1 * h[k] + f2 * w[k]);
                     hn = h[k] - (f1 * t + f2 * h[k]);
@@ -246,11 +182,11 @@ __global__ __launch_bounds__(64) void k_stratum_seq(
                     hn = fma(-f1, t, fma(-f2, h[k], h[k]));
                 }
             }
-            if (truncate) {                             // update-truncate.h:50-64
-                if (wn <= tmin) wn = tmin + eps;
-                else if (wn >= tmax) wn = tmax - eps;
-                if (hn <= tmin) hn = tmin + eps;
-                else if (hn >= tmax) hn = tmax - eps;
+            if (A.truncate) {                           // update-truncate.h:50-64
+                if (wn <= A.tmin) wn = A.tmin + eps;
+                else if (wn >= A.tmax) wn = A.tmax - eps;
+                if (hn <= A.tmin) hn = A.tmin + eps;
+                else if (hn >= A.tmax) hn = A.tmax - eps;
             }
             const int z = lane + 64 * k;
             if (z < r) {
@@ -259,6 +195,46 @@ __global__ __launch_bounds__(64) void k_stratum_seq(
             }
         }
     }
+}
+
+// One launch per stratum: workgroup t runs block (t, sched[t]) -- task t's
+// work in this sub-epoch (dsgdpp_impl.h:122-123); the launch boundary is the
+// sub-epoch barrier (:259-261).
+template <typename T, int K, bool STRICT>
+__global__ __launch_bounds__(64) void k_stratum_seq(SgdArgs<T> A) {
+    const int t = blockIdx.x;
+    const int b = t * 2 * A.d + (int)A.sched[t];
+    const int64_t s0 = A.blk[b];
+    run_block_seq<T, K, STRICT>(A, s0, A.blk[b + 1] - s0, A.perm + (A.use_perm ? A.voff[t] : 0));
+}
+
+// ---------------------------------------------------------------------------
+// Dataflow epoch: one persistent workgroup per task walks its 2d sub-epochs
+// (DsgdPpTask::run, dsgdpp_impl.h:117-267).  Instead of a barrier after every
+// sub-epoch, a visit to column block b2 waits until all earlier visits to b2
+// (sub-epochs < s) have finished -- the only other writers of those H
+// columns; the task's own W rows are only ever touched by this workgroup.
+// Every W row and H column therefore sees exactly the update sequence of the
+// barrier schedule.  Completion counters use agent-scope release/acquire, so
+// H written on one XCD is visible to the next user on another.
+// ---------------------------------------------------------------------------
+template <typename T, int K, bool STRICT>
+__global__ __launch_bounds__(64) void k_epoch_dataflow_seq(SgdArgs<T> A) {
+    const int t = blockIdx.x, d = A.d;
+    for (int s = 0; s < 2 * d; ++s) {
+        const int b2 = (int)A.sched[s * d + t];
+        const int b = t * 2 * d + b2;
+        const int64_t s0 = A.blk[b];
+        const int64_t n = A.blk[b + 1] - s0;
+        if (n == 0) continue;
+        dataflow_wait(A.ready, b2, A.need[s * d + t]);
+        run_block_seq<T, K, STRICT>(A, s0, n, A.perm + (A.use_perm ? A.voff[s * d + t] : 0));
+        dataflow_signal(A.ready, b2);
+    }
+}
+
+__global__ void k_fill_i32(int32_t* p, int n, int32_t v) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -368,35 +344,79 @@ void launch_visit_wr(const PermJob* d_jobs, int njobs, uint32_t* perm, hipStream
     HIPK(hipGetLastError());
 }
 
-template <typename T, int K, bool STRICT>
-static void stratum_t(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
-                      const int64_t* d_voff_row, int use_perm, double eps, hipStream_t s) {
+template <typename T>
+static SgdArgs<T> make_args(const DevProblem& P, const UpdParams& U, const uint32_t* sched,
+                            const int64_t* voff, const int32_t* need, int32_t* ready, int use_perm,
+                            double eps) {
     using C = typename St<T>::C;
-    k_stratum_seq<T, K, STRICT><<<P.d, 64, 0, s>>>(
-        (const typename PtOf<T>::type*)P.pts, P.blk, P.perm, d_sched_row, d_voff_row, (T*)P.W, (T*)P.H,
-        (const C*)P.inv1, (const C*)P.inv2, 2 * P.d, (int)P.r, P.ld, (C)eps, (C)U.lambda, U.kind,
-        U.truncate ? 1 : 0, (C)U.tmin, (C)U.tmax, use_perm);
-    HIPK(hipGetLastError());
+    SgdArgs<T> A;
+    A.pts = (const typename PtOf<T>::type*)P.pts;
+    A.blk = P.blk;
+    A.perm = P.perm;
+    A.sched = sched;
+    A.voff = voff;
+    A.need = need;
+    A.ready = ready;
+    A.W = (T*)P.W;
+    A.H = (T*)P.H;
+    A.inv1 = (const C*)P.inv1;
+    A.inv2 = (const C*)P.inv2;
+    A.d = P.d;
+    A.r = (int)P.r;
+    A.ld = P.ld;
+    A.eps = (C)eps;
+    A.lambda = (C)U.lambda;
+    A.tmin = (C)U.tmin;
+    A.tmax = (C)U.tmax;
+    A.kind = U.kind;
+    A.truncate = U.truncate ? 1 : 0;
+    A.use_perm = use_perm;
+    return A;
+}
+
+template <typename T, int K, bool STRICT>
+static void launch_t(const DevProblem& P, const UpdParams& U, const EpochLaunch& L, hipStream_t s) {
+    if (L.dataflow) {
+        SgdArgs<T> A = make_args<T>(P, U, L.sched, L.voff, L.need, L.ready, L.use_perm, L.eps);
+        k_fill_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d, 0);
+        HIPK(hipGetLastError());
+        void* args[] = {&A};
+        HIPK(hipLaunchCooperativeKernel((const void*)k_epoch_dataflow_seq<T, K, STRICT>, dim3(P.d),
+                                        dim3(64), args, 0, s));
+    } else {
+        for (int st = 0; st < 2 * P.d; ++st) {
+            SgdArgs<T> A = make_args<T>(P, U, L.sched + (size_t)st * P.d, L.voff + (size_t)st * P.d,
+                                        nullptr, nullptr, L.use_perm, L.eps);
+            k_stratum_seq<T, K, STRICT><<<P.d, 64, 0, s>>>(A);
+            HIPK(hipGetLastError());
+        }
+    }
 }
 
 template <typename T, bool STRICT>
-static void stratum_k(const DevProblem& P, const UpdParams& U, const uint32_t* row, const int64_t* vrow,
-                      int use_perm, double eps, hipStream_t s) {
-    if (P.r <= 64) stratum_t<T, 1, STRICT>(P, U, row, vrow, use_perm, eps, s);
-    else if (P.r <= 128) stratum_t<T, 2, STRICT>(P, U, row, vrow, use_perm, eps, s);
-    else if (P.r <= 256) stratum_t<T, 4, STRICT>(P, U, row, vrow, use_perm, eps, s);
+static void launch_k(const DevProblem& P, const UpdParams& U, const EpochLaunch& L, hipStream_t s) {
+    if (P.r <= 64) launch_t<T, 1, STRICT>(P, U, L, s);
+    else if (P.r <= 128) launch_t<T, 2, STRICT>(P, U, L, s);
+    else if (P.r <= 256) launch_t<T, 4, STRICT>(P, U, L, s);
     else throw std::invalid_argument("rank above 256 is not supported by the GPU path");
 }
 
-void launch_stratum(const DevProblem& P, const UpdParams& U, const uint32_t* d_sched_row,
-                    const int64_t* d_voff_row, int use_perm, double eps, hipStream_t s) {
+void launch_epoch(const DevProblem& P, const UpdParams& U, const EpochLaunch& L, hipStream_t s) {
     switch (P.precision) {
-    case PREC_FP64_STRICT: stratum_k<double, true>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
-    case PREC_FP64: stratum_k<double, false>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
-    case PREC_FP32: stratum_k<float, false>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
-    case PREC_BF16: stratum_k<bf16_t, false>(P, U, d_sched_row, d_voff_row, use_perm, eps, s); break;
+    case PREC_FP64_STRICT: launch_k<double, true>(P, U, L, s); break;
+    case PREC_FP64: launch_k<double, false>(P, U, L, s); break;
+    case PREC_FP32: launch_k<float, false>(P, U, L, s); break;
+    case PREC_BF16: launch_k<bf16_t, false>(P, U, L, s); break;
     default: throw std::invalid_argument("unknown precision");
     }
+}
+
+bool dataflow_fits(const DevProblem& P, int device) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_epoch_dataflow_seq<float, 1, false>, 64, 0) != hipSuccess)
+        return false;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return false;
+    return (int64_t)per_cu * cus >= P.d;
 }
 
 template <typename T>

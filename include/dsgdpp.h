@@ -74,7 +74,10 @@ typedef struct dsgdpp_cfg {
     uint64_t seed;           /* --seed */
     int32_t precision;       /* DSGDPP_FP64_STRICT .. DSGDPP_BF16 */
     int32_t device;          /* HIP device ordinal */
-    int32_t kernel;          /* 0 = auto; 1 = one wave per block, sequential; 2 = dataflow */
+    int32_t kernel;          /* 0 = auto (grouped dataflow when eligible, else dataflow);
+                                1 = one launch per stratum, one wave per block, sequential;
+                                2 = dataflow (one persistent launch per epoch), sequential blocks;
+                                3 = grouped dataflow (FP32/BF16, rank <= 64) */
     int32_t reserved[5];
 } dsgdpp_cfg;
 
@@ -85,6 +88,8 @@ typedef struct dsgdpp_epoch_stats {
     double sgd_seconds;      /* HIP-event time of the stratum kernels */
     int64_t updates;         /* points visited */
     int64_t launches;        /* stratum kernel launches */
+    int32_t kernel;          /* epoch kernel used: 1 per-stratum, 2 dataflow, 3 grouped dataflow */
+    int32_t reserved;
 } dsgdpp_epoch_stats;
 
 /* Creates a context (replaces mfInit + DsgdPpJob construction). */

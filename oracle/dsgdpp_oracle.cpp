@@ -319,6 +319,7 @@ struct Ctx {
     std::vector<uint64_t> seeds;
     bool record = false;
     std::vector<std::vector<uint32_t>> recPerm;   // [task*2d + subepoch]
+    int arith = 0;   // 0 FP64 reference; 1 FP32 grouped-GPU emulation; 2 BF16 grouped-GPU emulation
 };
 
 static void setErr(Ctx* c, const std::string& e) { c->err = e; }
@@ -446,6 +447,63 @@ static inline void updatePoint(const UpdateArgs& a, double* w, double* h, double
     }
 }
 
+// Float emulation of the GPU's grouped FP32/BF16 update (not part of the
+// reference): lane l of a 16-lane row holds elements [4l, 4l+4); its partial
+// dot is a 4-term fmaf chain; the row sum is ((q0+q1)+(q2+q3)) with
+// q_k = (d_4k + d_4k+1) + (d_4k+2 + d_4k+3); the update is
+// w' = fmaf(-f1, h, fmaf(-a, w, w)), h' = fmaf(-f1, w, fmaf(-b, h, h)); with
+// BF16 storage every result is rounded to bf16 (nearest even).
+static inline float bf16Round(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) b = (b | 0x00400000u) & 0xffff0000u;
+    else { b += 0x7fffu + ((b >> 16) & 1u); b &= 0xffff0000u; }
+    float o;
+    memcpy(&o, &b, 4);
+    return o;
+}
+
+static inline void updatePointF(const UpdateArgs& a, int mode, double* w, double* h, double xd, double epsd,
+                                double n1, double n2) {
+    const int64_t r = a.r;
+    const int64_t ld = (r + 3) / 4 * 4;
+    float wf[64], hf[64];
+    for (int64_t z = 0; z < 64; ++z) {
+        wf[z] = z < r ? (float)w[z] : 0.f;
+        hf[z] = z < r ? (float)h[z] : 0.f;
+    }
+    (void)ld;
+    float dl[16];
+    for (int l = 0; l < 16; ++l) {
+        float p = wf[4 * l] * hf[4 * l];
+        p = fmaf(wf[4 * l + 1], hf[4 * l + 1], p);
+        p = fmaf(wf[4 * l + 2], hf[4 * l + 2], p);
+        p = fmaf(wf[4 * l + 3], hf[4 * l + 3], p);
+        dl[l] = p;
+    }
+    float q[4];
+    for (int k = 0; k < 4; ++k) q[k] = (dl[4 * k] + dl[4 * k + 1]) + (dl[4 * k + 2] + dl[4 * k + 3]);
+    const float wh = (q[0] + q[1]) + (q[2] + q[3]);
+    const float eps = (float)epsd, x = (float)xd, lam = (float)a.lambda;
+    const float f1 = (eps * -2.f) * (x - wh);
+    const float f2 = (eps * 2.f) * lam;
+    float aa = 0.f, bb = 0.f;
+    if (a.kind == U_NZSL_L2) { aa = f2 * (float)(1. / n1); bb = f2 * (float)(1. / n2); }
+    else if (a.kind == U_NZSL_NZL2) { aa = f2; bb = f2; }
+    const float tmin = (float)a.tmin, tmax = (float)a.tmax;
+    for (int64_t z = 0; z < r; ++z) {
+        float wn = fmaf(-f1, hf[z], fmaf(-aa, wf[z], wf[z]));
+        float hn = fmaf(-f1, wf[z], fmaf(-bb, hf[z], hf[z]));
+        if (a.truncate) {
+            wn = (wn <= tmin) ? tmin + eps : ((wn >= tmax) ? tmax - eps : wn);
+            hn = (hn <= tmin) ? tmin + eps : ((hn >= tmax) ? tmax - eps : hn);
+        }
+        if (mode == 2) { wn = bf16Round(wn); hn = bf16Round(hn); }
+        w[z] = wn;
+        h[z] = hn;
+    }
+}
+
 // One task's SgdRunner (sgd.h:111; one per task per epoch, dsgdpp_impl.h:112).
 struct TaskRunner {
     Rng rng;
@@ -463,10 +521,13 @@ static void runBlock(Ctx* c, TaskRunner& tr, uint32_t b1, uint32_t b2, double ep
     UpdateArgs ua{c->updKind, c->updLambda, r, c->truncate, c->tmin, c->tmax};
     double* W = c->W.data();
     double* H = c->H.data();
+    const int mode = c->arith;
     auto step = [&](uint64_t p) {
         int64_t gi = o1 + B.i[p], gj = o2 + B.j[p];
-        updatePoint(ua, W + gi * r, H + gj * r, B.x[p], eps, (double)c->nnz1[gi],
-                    (double)c->nnz2[gj]);
+        if (mode == 0)
+            updatePoint(ua, W + gi * r, H + gj * r, B.x[p], eps, (double)c->nnz1[gi], (double)c->nnz2[gj]);
+        else
+            updatePointF(ua, mode, W + gi * r, H + gj * r, B.x[p], eps, (double)c->nnz1[gi], (double)c->nnz2[gj]);
     };
     if (c->sgdOrder == 0) {                                   // SEQ :198-217
         for (uint64_t p = 0; p < n; ++p) step(p);
@@ -927,6 +988,21 @@ int orc_get_schedule(orc_ctx* c, uint32_t* out) {
 }
 
 int orc_set_record(orc_ctx* c, int on) { c->record = on != 0; return 0; }
+
+int orc_set_arith(orc_ctx* c, int mode) {
+    if (mode < 0 || mode > 2) { setErr(c, "invalid arithmetic mode"); return -1; }
+    if (mode != 0 && c->r > 64) { setErr(c, "float emulation needs rank <= 64"); return -1; }
+    c->arith = mode;
+    // values as the GPU stores them: x as float, factors rounded to the storage type
+    if (mode != 0) {
+        for (auto* blocks : {&c->train, &c->test})
+            for (auto& B : *blocks)
+                for (auto& v : B.x) v = (float)v;
+        for (auto* A : {&c->W, &c->H})
+            for (auto& v : *A) v = mode == 2 ? bf16Round((float)v) : (float)v;
+    }
+    return 0;
+}
 
 int orc_get_permutation(orc_ctx* c, int task, int subepoch, uint32_t* out, int64_t cap, int64_t* n) {
     if (!c->record || c->recPerm.empty()) { setErr(c, "recording off"); return -1; }

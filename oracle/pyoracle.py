@@ -46,6 +46,7 @@ def lib():
             "orc_balance": (i32, [P, i32]),
             "orc_get_schedule": (i32, [P, pu32]),
             "orc_set_record": (i32, [P, i32]),
+            "orc_set_arith": (i32, [P, i32]),
             "orc_get_permutation": (i32, [P, i32, i32, pu32, i64, pi64]),
             "orc_get_task_seeds": (i32, [P, pu64]),
             "orc_block_nnz": (i32, [P, i32, pi64]),
@@ -136,6 +137,11 @@ class Oracle:
 
     def balance(self, kind):
         self._check(lib().orc_balance(self._h, {"None": 0, "L2": 1, "NZL2": 2}[kind]))
+
+    def set_arith(self, mode):
+        """0 FP64 reference; 'FP32'/'BF16': emulate the GPU grouped update."""
+        m = {"FP64": 0, "FP32": 1, "BF16": 2}.get(mode, mode)
+        self._check(lib().orc_set_arith(self._h, int(m)))
 
     def record(self, on=True):
         self._check(lib().orc_set_record(self._h, 1 if on else 0))

@@ -27,12 +27,12 @@ def _dsgdpp():
 
 
 def make_pair(p, update="Nzsl_L2(1)", loss=None, truncate=None, tasks=2, world=1, sgd="WOR",
-              stratum="COWOR", seed=7, precision="FP64_STRICT", test=None):
+              stratum="COWOR", seed=7, precision="FP64_STRICT", test=None, kernel=0):
     D = _dsgdpp()
     loss = loss or update
     g = D.DsgdPp(update=update, regularize="None", loss=loss, rank=p["rank"], tasks_per_rank=tasks,
                  world_size=world, sgd_order=sgd, stratum_order=stratum, seed=seed,
-                 precision=precision, truncate=truncate)
+                 precision=precision, truncate=truncate, kernel=kernel)
     o = po.Oracle(update=update, regularize="None", loss=loss, truncate=truncate, rank=p["rank"],
                   world_size=world, tasks_per_rank=tasks, sgd_order=sgd, stratum_order=stratum, seed=seed,
                   threads=2)
@@ -196,3 +196,123 @@ def test_precisions_vs_fp64_oracle(precision, rtol):
         np.testing.assert_array_equal(g.permutation(1, 3), o.permutation(1, 3))
         lg, lo = g.loss(0), o.loss(0)
         assert abs(lg - lo) <= rtol * abs(lo), (e, lg, lo, abs(lg - lo) / abs(lo))
+
+
+# ---------------------------------------------------------------------------
+# kernel coverage: per-stratum launches (kernel 1) and the dataflow epoch
+# (kernel 2, the default for FP64) give bit-identical factors
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("sgd,stratum", [("WOR", "COWOR"), ("WR", "WR"), ("SEQ", "WOR")])
+def test_kernels_strict(kernel, sgd, stratum):
+    print(f"case: kernel {kernel} sgd={sgd} stratum={stratum}", flush=True)
+    p = random_problem(80, 120, 3000, 6, seed=21)
+    g, o = make_pair(p, tasks=5, sgd=sgd, stratum=stratum, seed=23, kernel=kernel)
+    for e in range(3):
+        g.epoch(0.02)
+        o.epoch(0.02)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo)
+        np.testing.assert_array_equal(Hg, Ho)
+
+
+def skewed_problem(m, n, nnz, rank, seed, hot=0.3):
+    """Cells where a few hot rows and columns carry a large share, so groups
+    break often and W rows recur within a few groups (forwarding)."""
+    rng = np.random.default_rng(seed)
+    rw = rng.pareto(1.2, m) + 1
+    cw = rng.pareto(1.2, n) + 1
+    cells = set()
+    while len(cells) < nnz:
+        k = nnz - len(cells)
+        i = rng.choice(m, size=2 * k, p=rw / rw.sum())
+        j = rng.choice(n, size=2 * k, p=cw / cw.sum())
+        for a, b in zip(i, j):
+            cells.add(int(a) * n + int(b))
+            if len(cells) >= nnz:
+                break
+    cells = np.array(sorted(cells))
+    rng.shuffle(cells)
+    i = (cells // n).astype(np.int32)
+    j = (cells % n).astype(np.int32)
+    x = rng.integers(1, 6, size=nnz).astype(np.float64)
+    return dict(m=m, n=n, train=(i, j, x), rank=rank, W0=rng.uniform(-0.5, 0.5, (m, rank)),
+                H0=rng.uniform(-0.5, 0.5, (rank, n)))
+
+
+@pytest.mark.parametrize("precision", ["FP32", "BF16"])
+@pytest.mark.parametrize("update,truncate", [("Nzsl_L2(0.05)", None), ("Nzsl", None),
+                                             ("Nzsl_Nzl2(0.02)", None), ("Nzsl_L2(0.05)", "(-0.4,0.4)")])
+def test_grouped_bitexact_vs_emulation(precision, update, truncate):
+    """Grouped kernel (kernel 3) against the oracle replaying the same float
+    association point by point in visit order: any grouping, forwarding or
+    dataflow mistake shows up as a bit difference."""
+    print(f"case: grouped {precision} {update} {truncate}", flush=True)
+    p = skewed_problem(300, 200, 12000, 50, seed=31)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=43, precision=precision, kernel=3)
+    o.set_arith(precision)
+    for e in range(3):
+        st = g.epoch(0.01)
+        assert st["kernel"] == 3
+        o.epoch(0.01)
+        np.testing.assert_array_equal(g.permutation(2, 5), o.permutation(2, 5))
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+        lg, lo = g.loss(0), o.loss(0)
+        assert abs(lg - lo) <= 1e-10 * abs(lo)
+
+
+@pytest.mark.parametrize("rank", [1, 13, 32, 64])
+@pytest.mark.parametrize("sgd,stratum", [("WOR", "COWOR"), ("WR", "WR"), ("SEQ", "SEQ")])
+def test_grouped_orders_ranks(rank, sgd, stratum):
+    print(f"case: grouped rank {rank} sgd={sgd} stratum={stratum}", flush=True)
+    p = skewed_problem(120, 90, 3000, rank, seed=7)
+    g, o = make_pair(p, update="Nzsl_L2(0.1)", tasks=3, seed=5, precision="FP32", kernel=3,
+                     sgd=sgd, stratum=stratum)
+    o.set_arith("FP32")
+    for e in range(2):
+        g.epoch(0.02)
+        o.epoch(0.02)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo)
+        np.testing.assert_array_equal(Hg, Ho)
+
+
+def test_grouped_carry_over_and_many_tasks():
+    print("case: grouped, equal-size blocks + d=16", flush=True)
+    m, n, d = 32, 64, 4
+    rng = np.random.default_rng(3)
+    i, j = [], []
+    for b1 in range(d):
+        for b2 in range(2 * d):
+            cells = rng.choice(64, size=20, replace=False)
+            i += list(b1 * 8 + cells // 8)
+            j += list(b2 * 8 + cells % 8)
+    p = dict(m=m, n=n, train=(np.array(i), np.array(j), rng.normal(size=len(i))), rank=8,
+             W0=rng.uniform(-0.5, 0.5, (m, 8)), H0=rng.uniform(-0.5, 0.5, (8, n)))
+    g, o = make_pair(p, update="Nzsl_L2(0.1)", tasks=d, seed=9, precision="FP32", kernel=3)
+    o.set_arith("FP32")
+    for e in range(3):
+        g.epoch(0.05)
+        o.epoch(0.05)
+        for t in range(d):
+            for s_ in range(2 * d):
+                np.testing.assert_array_equal(g.permutation(t, s_), o.permutation(t, s_))
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo)
+        np.testing.assert_array_equal(Hg, Ho)
+    p2 = skewed_problem(800, 600, 40000, 24, seed=11)
+    g, o = make_pair(p2, update="Nzsl_L2(0.05)", tasks=16, seed=19, precision="FP32", kernel=3)
+    o.set_arith("FP32")
+    for e in range(2):
+        g.epoch(0.01)
+        o.epoch(0.01)
+    Wg, Hg = g.get_factors()
+    Wo, Ho = o.get_factors()
+    np.testing.assert_array_equal(Wg, Wo)
+    np.testing.assert_array_equal(Hg, Ho)
