@@ -42,7 +42,12 @@ def main():
     ap.add_argument("--tasks", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -157,8 +162,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_stratum",
+            "traffic": measured_traffic(name, precision, r, stats[-1]["kernel"]),
+            "kernel": KERNEL_NAMES.get(stats[-1]["kernel"], "?"),
             "bytes_per_update": b_upd,
             "avg_launch_us": avg_launch_s * 1e6,
         },
@@ -172,6 +177,34 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+KERNEL_NAMES = {1: "k_stratum_seq", 2: "k_epoch_dataflow_seq", 3: "k_epoch_grouped"}
+
+
+def calibrate_copy():
+    """A plain device copy of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE of
+    this kernel give the counter-to-bytes factor (1 GiB read + 1 GiB written)."""
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4, "launches": 3}))
+
+
+def measured_traffic(name, precision, rank, kernel):
+    """HBM bytes per epoch launch from the committed rocprofv3 --pmc summary
+    for this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES.get(kernel, '?')}_{name}_{precision}_r{rank}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(p, update, tasks, threads):

@@ -35,7 +35,9 @@ class DsgdPpError(RuntimeError):
 # functors (rendered to the reference's functor strings, tools/parse.h:39-61)
 # ---------------------------------------------------------------------------
 def _num(x):
-    return "%.17g" % float(x)
+    """Shortest round-trip decimal ("1", "0.05"), as a user writes it."""
+    r = repr(float(x))
+    return r[:-2] if r.endswith(".0") else r
 
 
 class UpdateNzsl:
