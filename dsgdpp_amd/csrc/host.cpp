@@ -162,17 +162,18 @@ struct EpochPlan {
     std::vector<uint32_t> schedule;   // 2d x d
     std::vector<uint64_t> seeds;      // d
     std::vector<int64_t> voff;        // 2d x d
+    std::vector<int64_t> goff;        // 2d x d: first group of each visit (grouped epoch)
     std::vector<int32_t> need;        // 2d x d
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
     uint32_t* dSched = nullptr;
     int64_t* dVoff = nullptr;
+    int64_t* dGoff = nullptr;
     int32_t* dNeed = nullptr;
     PermJob* dJobs = nullptr;
     int32_t* dChains = nullptr;
     uint32_t* perm = nullptr;
     Pt32* grec = nullptr;
-    uint2* ginfo = nullptr;
     int32_t* ngroups = nullptr;
     size_t jobsCap = 0, chainsCap = 0;
     int64_t permCap = 0, groupCap = 0;
@@ -215,6 +216,7 @@ struct dsgdpp_ctx {
     EpochPlan plan[2];
     int cur = 0, lastPlan = -1;
     bool lookahead = true;
+    bool kprof = false;
     uint64_t dataVersion = 0;
 
     size_t factorBytes() const {
@@ -508,7 +510,8 @@ float fromBf16(uint16_t u) {
 void putFactor(dsgdpp_ctx* c, void*& dst, const double* src, int64_t rows) {
     const int64_t r = c->r, ld = c->P.ld;
     const size_t es = c->factorBytes();
-    std::vector<uint8_t> buf((size_t)rows * ld * es, 0);
+    // FACTOR_TAIL_ROWS spare rows after the last one (zero rows, then the dummy row of the grouped kernel)
+    std::vector<uint8_t> buf((size_t)(rows + FACTOR_TAIL_ROWS) * ld * es, 0);
     for (int64_t i = 0; i < rows; ++i)
         for (int64_t z = 0; z < r; ++z) {
             const double v = src[i * r + z];
@@ -524,7 +527,7 @@ void putFactor(dsgdpp_ctx* c, void*& dst, const double* src, int64_t rows) {
 void getFactor(dsgdpp_ctx* c, const void* src, double* dst, int64_t rows) {
     const int64_t r = c->r, ld = c->P.ld;
     const size_t es = c->factorBytes();
-    std::vector<uint8_t> buf((size_t)rows * ld * es);
+    std::vector<uint8_t> buf((size_t)rows * ld * es);   // the spare rows stay on the device
     HIPC(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < rows; ++i)
         for (int64_t z = 0; z < r; ++z) {
@@ -592,12 +595,16 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
     // every (task, sub-epoch) visit gets its own run of the visit-order buffers:
     // a WR stratum order may send a task to the same block twice in one epoch
     P.voff.assign((size_t)d2 * d, 0);
-    int64_t total = 0;
+    P.goff.assign((size_t)d2 * d, 0);
+    int64_t total = 0, gtotal = 0;
     for (uint32_t id = 0; id < d; ++id)
         for (uint32_t sub = 0; sub < d2; ++sub) {
             const size_t b = visitBlock(id, sub);
+            const int64_t nb = bs[b + 1] - bs[b];
             P.voff[(size_t)sub * d + id] = total;
-            total += bs[b + 1] - bs[b];
+            P.goff[(size_t)sub * d + id] = gtotal;
+            total += nb;
+            gtotal += (nb + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;   // groups <= points, padded
         }
     P.grouped = c->groupedOk && (c->kernel == 0 || c->kernel == 3);
     P.dataflow = P.grouped || (c->kernel != 1 && c->dataflowOk);
@@ -620,7 +627,7 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
                     const int64_t base = P.voff[(size_t)sub * d + id];
                     const uint32_t b2 = P.schedule[(size_t)sub * d + id];
                     PermJob j{base, nb, -1, draw, k0, k1, bs[b], (int32_t)c->off1[id], (int32_t)c->off2[b2],
-                              (int32_t)(sub * d + id), 0};
+                              (int32_t)(sub * d + id), 0, P.goff[(size_t)sub * d + id]};
                     if (c->sgdOrder == DSGDPP_SGD_WR) {
                         if (nb > 0) perTask[id].push_back(j);
                         draw += (uint64_t)nb + 2;
@@ -649,10 +656,9 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
             P.permCap = total;
             dalloc(P.perm, (size_t)total * sizeof(uint32_t));
         }
-        if (P.grouped && total > P.groupCap) {
-            P.groupCap = total;
-            dalloc(P.grec, (size_t)total * GROUP_SLOTS * sizeof(Pt32));
-            dalloc(P.ginfo, (size_t)total * sizeof(uint2));
+        if (P.grouped && gtotal > P.groupCap) {
+            P.groupCap = gtotal;
+            dalloc(P.grec, (size_t)gtotal * GROUP_SLOTS * sizeof(Pt32));
         }
         if (P.grouped && !P.ngroups) dalloc(P.ngroups, (size_t)d2 * d * sizeof(int32_t));
     }
@@ -666,6 +672,7 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
     }
     if (!P.dSched) dalloc(P.dSched, (size_t)d2 * d * sizeof(uint32_t));
     if (!P.dVoff) dalloc(P.dVoff, (size_t)d2 * d * sizeof(int64_t));
+    if (P.grouped && !P.dGoff) dalloc(P.dGoff, (size_t)d2 * d * sizeof(int64_t));
     // dataflow dependencies: a visit to column block b2 at sub-epoch s waits for
     // the non-empty visits to b2 in sub-epochs < s
     P.need.assign((size_t)d2 * d, 0);
@@ -685,6 +692,8 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
     P.chains.swap(chains);
     HIPC(hipMemcpyAsync(P.dSched, P.schedule.data(), P.schedule.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIPC(hipMemcpyAsync(P.dVoff, P.voff.data(), P.voff.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (P.grouped)
+        HIPC(hipMemcpyAsync(P.dGoff, P.goff.data(), P.goff.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (P.dataflow)
         HIPC(hipMemcpyAsync(P.dNeed, P.need.data(), P.need.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (!P.jobs.empty())
@@ -695,7 +704,6 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
     Dp.perm = P.perm;
     GroupedBuffers G = c->G;
     G.grec = P.grec;
-    G.ginfo = P.ginfo;
     G.ngroups = P.ngroups;
     const int nchains = (int)P.chains.size() - 1;
     HIPC(hipEventRecord(P.evStart, s));
@@ -726,6 +734,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     EpochLaunch L;
     L.sched = P.dSched;
     L.voff = P.dVoff;
+    L.goff = P.dGoff;
     L.need = P.dNeed;
     L.ready = c->dReady;
     L.use_perm = c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1;
@@ -736,8 +745,13 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     if (P.grouped) {
         GroupedBuffers G = c->G;
         G.grec = P.grec;
-        G.ginfo = P.ginfo;
-        G.ngroups = P.ngroups;
+            G.ngroups = P.ngroups;
+        if (c->kprof && !c->G.prof) {
+            unsigned long long* q = nullptr;
+            dalloc(q, (size_t)d * 5 * sizeof(unsigned long long));
+            c->G.prof = q;
+        }
+        G.prof = c->G.prof;
         launch_epoch_grouped(Dp, c->upd, G, L, c->stream);
     } else {
         launch_epoch(Dp, c->upd, L, c->stream);
@@ -754,6 +768,21 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     }
     HIPC(hipEventSynchronize(c->ev[2]));
     const auto t1 = std::chrono::steady_clock::now();
+    if (c->kprof && P.grouped && c->G.prof) {
+        // clock counters of the grouped epoch kernel: the slowest task and the mean
+        std::vector<unsigned long long> h((size_t)d * 5);
+        HIPC(hipMemcpy(h.data(), c->G.prof, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        double mw = 0, ms = 0, ml = 0, mt = 0, mg = 0;
+        size_t worst = 0;
+        for (size_t t = 0; t < d; ++t) {
+            mw += h[5 * t]; ms += h[5 * t + 1]; ml += h[5 * t + 2]; mt += h[5 * t + 3]; mg += h[5 * t + 4];
+            if (h[5 * t + 2] > h[5 * worst + 2]) worst = t;
+        }
+        fprintf(stderr, "[kprof] mean clocks/task: wait %.3g stage %.3g groups %.3g total %.3g; groups/task %.0f; "
+                        "busiest task %zu: loop %.3g clocks, %llu groups (%.1f clocks/group)\n",
+                mw / d, ms / d, ml / d, mt / d, mg / d, worst, (double)h[5 * worst + 2], h[5 * worst + 4],
+                (double)h[5 * worst + 2] / std::max<unsigned long long>(1, h[5 * worst + 4]));
+    }
     if (st) {
         float a = 0, b = 0;
         HIPC(hipEventSynchronize(P.evEnd));
@@ -1084,6 +1113,8 @@ int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out) {
         }
         const char* la = getenv("DSGDPP_LOOKAHEAD");
         c->lookahead = !(la && la[0] == '0');
+        const char* kp = getenv("DSGDPP_KPROF");
+        c->kprof = kp && kp[0] == '1';
         dalloc(c->dPartials, (size_t)partials_capacity() * sizeof(double));
         HIPC(hipHostMalloc((void**)&c->hPinned, sizeof(double) * 8, hipHostMallocDefault));
     } catch (const std::exception& e) {
@@ -1106,12 +1137,12 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
         void* q;
         q = P.dSched; dfree(q);
         q = P.dVoff; dfree(q);
+        q = P.dGoff; dfree(q);
         q = P.dNeed; dfree(q);
         q = P.dJobs; dfree(q);
         q = P.dChains; dfree(q);
         q = P.perm; dfree(q);
         q = P.grec; dfree(q);
-        q = P.ginfo; dfree(q);
         q = P.ngroups; dfree(q);
         if (P.evStart) (void)hipEventDestroy(P.evStart);
         if (P.evEnd) (void)hipEventDestroy(P.evEnd);
@@ -1125,6 +1156,7 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     v = const_cast<double*>(c->P.cnt2); dfree(v);
     v = c->dPartials; dfree(v);
     v = const_cast<int32_t*>(c->G.off2); dfree(v);
+    v = c->G.prof; dfree(v);
     v = c->dReady; dfree(v);
     if (c->hPinned) (void)hipHostFree(c->hPinned);
     for (auto& e : c->ev)

@@ -63,6 +63,7 @@ struct PermJob {
     int32_t row0, col0;  // first global row / column of the block
     int32_t visit;       // s * d + t
     int32_t pad;
+    int64_t gbase;       // first group of this visit in the grouped record buffer
 };
 
 // kernels.hip
@@ -75,6 +76,7 @@ void launch_visit_wr(const PermJob* d_jobs, int njobs, uint32_t* perm, hipStream
 struct EpochLaunch {
     const uint32_t* sched = nullptr;   // 2d x d
     const int64_t* voff = nullptr;     // 2d x d
+    const int64_t* goff = nullptr;     // 2d x d: first group of each visit (grouped epoch)
     const int32_t* need = nullptr;     // 2d x d (dataflow)
     int32_t* ready = nullptr;          // 2d (dataflow)
     int use_perm = 1;
@@ -85,18 +87,23 @@ void launch_epoch(const DevProblem& P, const UpdParams& U, const EpochLaunch& L,
 bool dataflow_fits(const DevProblem& P, int device);
 
 // kernels_grouped.hip: the grouped epoch.  A prep kernel turns every visit
-// into its visit order, the visited records in that order, and groups of up to
-// GROUP_SLOTS consecutive independent points; the epoch kernel runs one group
-// per wave step (16 lanes per point, H block staged in LDS).
+// into its visit order and groups of up to GROUP_SLOTS independent points
+// (packed 16-byte records in (group, slot) order); the epoch kernel runs one
+// group per wave step (16 lanes per point, H block staged in LDS).
 constexpr int GROUP_SLOTS = 4;
 constexpr int GROUP_RING = 16;    // W forwarding window, in groups
+constexpr int GROUP_PAD = 24;     // every visit's group list is padded to a multiple of this
+// Factor buffers carry FACTOR_TAIL_ROWS spare rows after the last one: the
+// first half stays zero (read by lanes past the row length), the second half
+// takes the stores of empty group slots.
+constexpr int FACTOR_TAIL_ROWS = 128;
 struct GroupedBuffers {
-    void* grec = nullptr;         // point records in (group, slot) layout (Pt32), 4 per group
-    uint2* ginfo = nullptr;       // per group: size, forwarding bytes
+    void* grec = nullptr;         // packed point records, GROUP_SLOTS per group (16 bytes each)
     int32_t* ngroups = nullptr;   // 2d x d
     const int32_t* off2 = nullptr;   // 2d+1 column block offsets (device)
     int max_cols = 0;             // widest column block
     int max_rows = 0;             // tallest row block
+    unsigned long long* prof = nullptr;   // optional kernel clock counters (5 per task)
 };
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols,
                        int device, std::string* why);

@@ -1,28 +1,32 @@
 // Grouped DSGD++ epoch for FP32 / BF16 factors (rank <= 64).
 //
 // Exactness argument.  Two SGD steps commute bit for bit when they share
-// neither the W row nor the H column.  Walking a block's visit order
-// (sgd_impl.h:293-351), the prep kernel packs consecutive points into a group
-// until the next point shares a row or column with a member (or the group holds
-// GROUP_SLOTS points).  Running the members of a group at the same time is then
-// the same computation as running them one after another, and groups run in
-// visit order, so every W row and H column sees exactly the sequential update
-// sequence.
+// neither the W row nor the H column.  The prep kernel walks a block's visit
+// order (sgd_impl.h:293-351) and places every point in a group of at most
+// GROUP_SLOTS points: the earliest group after the groups of the previous
+// points on its row and on its column (list scheduling).  Points of one group
+// are therefore independent, every point runs after all earlier points it
+// depends on, and groups run in order, so every W row and H column sees
+// exactly the sequential update sequence with the same operands.
 //
-// Layout on the wave: slot g = lane / 16 holds member g; lane l16 = lane % 16
-// holds elements [4 l16, 4 l16 + 4) of W_i and H_j, so a rank-64 row is one
-// 16-byte load per lane and the dot product is a 4-term FMA chain plus a
-// 4-step DPP reduction inside the 16-lane row.
+// Layout on the wave: slot s = lane / 16 holds the member placed in slot s;
+// lane l16 = lane % 16 holds elements [4 l16, 4 l16 + 4) of W_i and H_j, so a
+// rank-64 row is one 16-byte load per lane and the dot product is a 4-term FMA
+// chain plus a 4-step DPP reduction inside the 16-lane row.  Empty slots carry
+// a dummy record (a spare W row and a spare LDS column), so every load and
+// store is unconditional.
 //
 // Data movement per visit of block (t, b2):
-//   * H columns of b2 (<= max_cols x 64 floats) are staged in LDS after the
-//     dataflow wait and written back before the completion signal;
-//   * W rows are prefetched GPREF groups ahead from HBM.  A row that a member
-//     of one of the last GROUP_RING groups updated is read from an LDS ring of
-//     recent W results instead (the prep kernel records that group distance and
-//     slot), so a prefetch never returns a value older than the sequential
-//     order requires;
-//   * visit-ordered point records and group descriptors are streamed.
+//   * the H columns of b2 are staged in LDS after the dataflow wait and written
+//     back before the completion signal;
+//   * the H operand of group K+1 is read from LDS while group K computes.  A
+//     column that group K updates is never read that way: the prep kernel puts
+//     its next point in the same slot and the value is taken from registers;
+//   * W rows are prefetched GPREF groups ahead from HBM.  A row that one of the
+//     last GROUP_RING groups updated is read from an LDS ring of recent W
+//     results instead (the prep kernel records which ring entry);
+//   * point records, already in (group, slot) order, are streamed GQ groups
+//     ahead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,27 +38,50 @@
 
 namespace dsgdpp {
 
-constexpr int GPREF = 8;            // W prefetch distance (groups)
-constexpr int GQ = 2 * GPREF;       // record/descriptor prefetch distance
+constexpr int GPREF = 12;           // W prefetch distance (groups), <= GROUP_RING
+constexpr int GQ = 2 * GPREF;       // record prefetch distance = unroll of the group loop
+static_assert(GPREF <= GROUP_RING, "a prefetched W row may be at most GROUP_RING groups old");
+static_assert(GQ == GROUP_PAD, "group lists are padded to whole unrolled iterations");
+
+// packed record of one slot (uint4):
+//   .x  global row (bits 0-23) | REC_HREG | REC_WFWD | ring entry << 26
+//   .y  column index inside the column block; bit 31 of slot 0's record: some
+//       member of the group takes W_i from the ring (REC_GFWD)
+//   .z  value x (float bits)
+//   .w  1/nnz of the row (float bits)
+constexpr uint32_t REC_HREG = 1u << 24;   // H_j: the previous group's result in this slot
+constexpr uint32_t REC_WFWD = 1u << 25;   // W_i: ring entry (group & 15) * 4 + slot of its last update
+constexpr int REC_RING_SHIFT = 26;
+constexpr uint32_t REC_GFWD = 1u << 31;   // in .y: LDS offsets (.y << 8, .y << 2) drop this bit
+constexpr uint32_t GM_FWD = 16u;          // group mask bit: some member forwards W_i
 
 __global__ void k_zero_i32(int32_t* p, int n) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
 }
 
+// old with lane `lane` (uniform) replaced by v
+__device__ __forceinline__ uint32_t writelane_u32(uint32_t v, int lane, uint32_t old) {
+    return (int)threadIdx.x == lane ? v : old;
+}
+
 // ---------------------------------------------------------------------------
-// prep: visit order + greedy groups + records in group-slot layout
-//   grec[(base + g) * 4 + slot]  record of member `slot` of group g (unused slots
-//                                repeat member 0, so every address is valid)
-//   ginfo[base + g]              {size, forwarding bytes}
+// prep: visit order, groups and packed records
 // ---------------------------------------------------------------------------
+static size_t prep_lds(int64_t cap, int rows, int cols) {
+    return (size_t)(cap + rows + cols) * sizeof(uint32_t) + (size_t)((cap + GROUP_PAD + 3) & ~int64_t(3));
+}
+
 __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
                                                      const int32_t* __restrict__ chain_start, int order,
                                                      const Pt32* __restrict__ pts, uint32_t* __restrict__ perm,
-                                                     Pt32* __restrict__ grec, uint2* __restrict__ ginfo,
-                                                     int32_t* __restrict__ ngroups, int64_t cap, int rows_cap) {
+                                                     uint4* __restrict__ grec, int32_t* __restrict__ ngroups,
+                                                     const float* __restrict__ inv1, int64_t cap, int rows_cap,
+                                                     int cols_cap, uint32_t dummy_row, uint32_t dummy_col) {
     extern __shared__ uint32_t sm[];
-    uint32_t* vis = sm;              // visit order
-    uint32_t* lastw = sm + cap;      // per local row: (group << 2 | slot) + 1 of its last update
+    uint32_t* vis = sm;                          // visit order
+    uint32_t* lastR = vis + cap;                 // per local row: tag of its last update
+    uint32_t* lastC = lastR + rows_cap;          // per local column: the same
+    uint32_t* gmw = lastC + cols_cap;            // slot masks of the groups, 4 per word (+ padding groups)
     const int lane = threadIdx.x;
     for (int jx = chain_start[blockIdx.x]; jx < chain_start[blockIdx.x + 1]; ++jx) {
         const PermJob J = jobs[jx];
@@ -62,10 +89,12 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
         // 1. visit order (sgd_impl.h:198-351)
         if (order == 0) {             // SEQ
             for (int64_t k = lane; k < n; k += 64) vis[k] = (uint32_t)k;
+            __syncthreads();
         } else if (order == 1) {      // WR
             for (int64_t k = lane; k < n; k += 64)
                 vis[k] = (uint32_t)bounded(stream_draw(J.k0, J.k1, J.draw + k), (uint64_t)n);
-        } else {                      // WOR
+            __syncthreads();
+        } else {                      // WOR: Fisher-Yates, 64 steps at a time in registers
             if (J.carry_base >= 0) {
                 for (int64_t k = lane; k < n; k += 64) vis[k] = perm[J.carry_base + k];
             } else {
@@ -73,29 +102,47 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
             }
             __syncthreads();
             for (int64_t base = 0; base < n - 1; base += 64) {
-                const int64_t s = base + lane;
-                uint32_t tg = 0;
-                if (s < n) tg = (uint32_t)(s + (int64_t)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)(n - s)));
+                // lane l owns position base + l (cur) and, when step base + l swaps
+                // with a position past this chunk, the current value there (pre)
+                const int64_t s = base + lane, hi = base + 64;
+                int64_t t = s;
+                if (s < n - 1) t = s + (int64_t)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)(n - s));
+                const bool outside = t >= hi;
+                uint32_t cur = s < n ? vis[s] : 0u;
+                uint32_t pre = outside ? vis[t] : 0u;
                 const int cnt = (int)min<int64_t>(64, n - 1 - base);
                 for (int l = 0; l < cnt; ++l) {
-                    const uint32_t t = readlane_u32(tg, l);
-                    if (lane == 0) {
-                        const uint32_t u = vis[base + l];
-                        vis[base + l] = vis[t];
-                        vis[t] = u;
+                    const uint32_t T = readlane_u32((uint32_t)t, l);   // t < 2^32: n < 2^24
+                    const uint32_t a = readlane_u32(cur, l);
+                    if ((int64_t)T < hi) {
+                        const int tl = (int)((int64_t)T - base);
+                        const uint32_t b = readlane_u32(cur, tl);
+                        cur = writelane_u32(b, l, cur);
+                        cur = writelane_u32(a, tl, cur);
+                    } else {
+                        const uint32_t b = readlane_u32(pre, l);
+                        cur = writelane_u32(b, l, cur);
+                        pre = (uint32_t)t == T ? a : pre;   // every lane aimed at T sees its new value
                     }
                 }
+                if (s < n) vis[s] = cur;
+                if (outside) vis[t] = pre;   // lanes aimed at the same position store the same value
                 __syncthreads();
             }
         }
-        for (int k = lane; k < rows_cap; k += 64) lastw[k] = 0;
+        const int gwords = (int)((n + GROUP_PAD + 3) >> 2);
+        for (int k = lane; k < rows_cap; k += 64) lastR[k] = 0;
+        for (int k = lane; k < cols_cap; k += 64) lastC[k] = 0;
+        for (int k = lane; k < gwords; k += 64) gmw[k] = 0;
         __syncthreads();
-        // 2. greedy groups, 64 visit positions at a time: every lane fetches its
-        //    record, the scan runs on uniform values (readlane), and each lane
-        //    scatters its record to (group, slot)
-        uint32_t g = 0, size = 0, fw = 0;
-        uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-        Pt32 first = Pt32{0, 0, 0.f, 0u};   // member 0 of the open group (this lane's copy)
+        // 2. list scheduling, 64 visit positions at a time.  The scan runs on
+        //    uniform values.  Row and column states are tags ((group + 1) << 2 |
+        //    slot, 0 = none) held by the chunk's lanes; every placement updates
+        //    all lanes that share the row or column.  The slot masks of groups
+        //    maxg-62 .. maxg sit in lanes (group g in lane g % 64) and the lane of
+        //    maxg+1 is kept clear, so a point goes to a group in that range.
+        int maxg = -1;
+        uint32_t win = 0;
         for (int64_t v0 = 0; v0 < n; v0 += 64) {
             const int64_t v = v0 + lane;
             Pt32 e = Pt32{J.row0, J.col0, 0.f, 0u};
@@ -104,242 +151,315 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 perm[J.base + v] = q;
                 e = pts[J.s0 + q];
             }
-            const uint32_t rc = (uint32_t)(e.i - J.row0) | ((uint32_t)(e.j - J.col0) << 16);
-            uint32_t mine = 0;   // this lane's (group << 2 | slot)
+            const uint32_t r = (uint32_t)(e.i - J.row0), c = (uint32_t)(e.j - J.col0);
+            uint32_t myR = v < n ? lastR[r] : 0u, myC = v < n ? lastC[c] : 0u;
+            uint32_t res = 0;   // group << 10 | ring entry << 4 | wfwd << 3 | hreg << 2 | slot
             const int cnt = (int)min<int64_t>(64, n - v0);
             for (int l = 0; l < cnt; ++l) {
-                const uint32_t x = readlane_u32(rc, l);
-                const uint32_t r = x & 0xffffu, c = x >> 16;
-                bool clash = size == GROUP_SLOTS;
-                clash |= size > 0 && (r0 == r || c0 == c);
-                clash |= size > 1 && (r1 == r || c1 == c);
-                clash |= size > 2 && (r2 == r || c2 == c);
-                clash |= size > 3 && (r3 == r || c3 == c);
-                if (clash) {
-                    if (lane == 0) ginfo[J.base + g] = make_uint2(size, fw);
+                const uint32_t R = readlane_u32(r, l), C = readlane_u32(c, l);
+                const uint32_t LR = readlane_u32(myR, l), LC = readlane_u32(myC, l);
+                const int gR = (int)(LR >> 2) - 1, gC = (int)(LC >> 2) - 1;
+                const uint32_t sC = LC & 3u;
+                int g = max(max(gR, gC) + 1, maxg - 62);
+                uint32_t mask;
+                for (;;) {   // ends at maxg + 1 at the latest (its lane is clear)
+                    mask = readlane_u32(win, g & 63);
+                    const uint32_t need = (LC != 0u && gC == g - 1) ? (1u << sC) : 0u;
+                    if ((mask & 15u) != 15u && !(mask & need)) break;
                     ++g;
-                    size = 0;
-                    fw = 0;
                 }
-                const uint32_t le = lastw[r];
-                if (le != 0) {
-                    const uint32_t dist = g - ((le - 1) >> 2);
-                    if (dist < GROUP_RING) fw |= (0x80u | (dist << 2) | ((le - 1) & 3u)) << (8 * size);
+                // H_j from the previous group's result in the same slot (registers)
+                const uint32_t hreg = (LC != 0u && gC == g - 1) ? 1u : 0u;
+                const uint32_t fr = ~mask & 15u;
+                const uint32_t sl = (LC != 0u && ((fr >> sC) & 1u)) ? sC : (uint32_t)__builtin_ctz(fr);
+                if (g > maxg) {
+                    maxg = g;
+                    win = lane == ((g + 1) & 63) ? 0u : win;
                 }
-                __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): the read above precedes the write below
-                lastw[r] = ((g << 2) | size) + 1;
-                if (lane == l) mine = (g << 2) | size;
-                r0 = size == 0 ? r : r0; c0 = size == 0 ? c : c0;
-                r1 = size == 1 ? r : r1; c1 = size == 1 ? c : c1;
-                r2 = size == 2 ? r : r2; c2 = size == 2 ? c : c2;
-                r3 = size == 3 ? r : r3; c3 = size == 3 ? c : c3;
-                ++size;
+                win = writelane_u32(mask | (1u << sl), g & 63, win);
+                const uint32_t tag = ((uint32_t)(g + 1) << 2) | sl;
+                myR = r == R ? tag : myR;
+                myC = c == C ? tag : myC;
+                uint32_t f = sl | (hreg << 2);
+                if (gR >= 0 && g - gR < GROUP_RING) f |= 8u | (((((uint32_t)gR & 15u) << 2) | (LR & 3u)) << 4);
+                res = writelane_u32(((uint32_t)g << 10) | f, l, res);
             }
             if (v < n) {
-                const uint32_t gg = mine >> 2, sl = mine & 3u;
-                grec[(J.base + gg) * GROUP_SLOTS + sl] = e;
+                lastR[r] = myR;   // lanes of one row hold the same final tag
+                lastC[c] = myC;
+                const uint32_t g = res >> 10, sl = res & 3u;
+                atomicOr(gmw + (g >> 2), ((1u << sl) | ((res & 8u) ? GM_FWD : 0u)) << (8 * (g & 3u)));
+                uint4 o;
+                o.x = (uint32_t)e.i | (((res >> 2) & 1u) << 24) | (((res >> 3) & 1u) << 25) |
+                      (((res >> 4) & 63u) << REC_RING_SHIFT);
+                o.y = c;
+                o.z = __float_as_uint(e.x);
+                o.w = __float_as_uint(inv1[e.i]);
+                grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
             }
             __syncthreads();
         }
-        if (lane == 0 && n > 0) ginfo[J.base + g] = make_uint2(size, fw);
-        if (lane == 0) ngroups[J.visit] = n > 0 ? (int32_t)(g + 1) : 0;
-        __syncthreads();
-        // 3. unused slots repeat member 0 (valid indices for unconditional loads)
-        const int64_t ng = n > 0 ? (int64_t)g + 1 : 0;
-        for (int64_t k = lane; k < ng; k += 64) {
-            const uint32_t sz = ginfo[J.base + k].x;
-            if (sz < GROUP_SLOTS) {
-                const Pt32 m0 = grec[(J.base + k) * GROUP_SLOTS];
-                for (uint32_t sl = sz; sl < GROUP_SLOTS; ++sl) grec[(J.base + k) * GROUP_SLOTS + sl] = m0;
-            }
+        const int ng = maxg + 1;
+        const int ngp = (ng + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;
+        // 3. empty slots (and the padding groups) get the dummy record; slot 0
+        //    carries the group's W forwarding flag
+        for (int g = lane; g < ngp; g += 64) {
+            const uint32_t mk = (gmw[g >> 2] >> (8 * (g & 3))) & 0xffu;
+            uint4* gp = grec + ((uint64_t)J.gbase + g) * GROUP_SLOTS;
+            const uint32_t gf = (mk & GM_FWD) ? REC_GFWD : 0u;
+            for (int sl = 0; sl < GROUP_SLOTS; ++sl)
+                if (!((mk >> sl) & 1u)) gp[sl] = make_uint4(dummy_row, dummy_col | (sl == 0 ? gf : 0u), 0u, 0u);
+            if ((mk & 1u) && gf)   // slot 0 holds a point: set the flag in place (L2, after the barrier)
+                __hip_atomic_fetch_or(&gp[0].y, gf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        if (lane == 0) ngroups[J.visit] = ng;
         __syncthreads();
-        (void)first;
     }
 }
 
 // ---------------------------------------------------------------------------
-// storage <-> float4 for the 16-lane row layout
+// factor storage through buffer resources: 4 elements per lane
 // ---------------------------------------------------------------------------
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+constexpr int RSRC_WORD3 = 0x00020000;   // raw buffer, gfx9 family
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, RSRC_WORD3);
+}
+
 template <typename T>
 struct Vec4;
 template <>
 struct Vec4<float> {
+    static constexpr uint32_t BYTES = 4;
     static __device__ __forceinline__ float4 load(const float* p) { return *(const float4*)p; }
     static __device__ __forceinline__ void store(float* p, float4 v) { *(float4*)p = v; }
+    static __device__ __forceinline__ float4 bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+        return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float4 v) {
+        u32x4 u;
+        u.x = __float_as_uint(v.x); u.y = __float_as_uint(v.y); u.z = __float_as_uint(v.z); u.w = __float_as_uint(v.w);
+        __builtin_amdgcn_raw_buffer_store_b128(u, r, (int)off, 0, 0);
+    }
     static __device__ __forceinline__ float round(float v) { return v; }
 };
 template <>
 struct Vec4<bf16_t> {
+    static constexpr uint32_t BYTES = 2;
+    static __device__ __forceinline__ float4 unpack(uint32_t a, uint32_t b) {
+        return make_float4(__uint_as_float(a << 16), __uint_as_float(a & 0xffff0000u), __uint_as_float(b << 16),
+                           __uint_as_float(b & 0xffff0000u));
+    }
+    // round to nearest even (v_cvt_pk_bf16_f32)
+    static __device__ __forceinline__ uint32_t pack(float lo, float hi) {
+        const __bf16 a = (__bf16)lo, b = (__bf16)hi;
+        return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+    }
     static __device__ __forceinline__ float4 load(const bf16_t* p) {
         const uint2 u = *(const uint2*)p;
-        return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
-                           __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+        return unpack(u.x, u.y);
     }
     static __device__ __forceinline__ void store(bf16_t* p, float4 v) {
-        uint2 u;
-        u.x = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
-        u.y = (uint32_t)f32_to_bf16(v.z) | ((uint32_t)f32_to_bf16(v.w) << 16);
-        *(uint2*)p = u;
+        *(uint2*)p = make_uint2(pack(v.x, v.y), pack(v.z, v.w));
     }
-    // value the factor takes once stored (round to nearest even)
-    static __device__ __forceinline__ float round(float v) { return bf16_to_f32(f32_to_bf16(v)); }
+    static __device__ __forceinline__ float4 bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+        return unpack(u.x, u.y);
+    }
+    static __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float4 v) {
+        u32x2 u;
+        u.x = pack(v.x, v.y);
+        u.y = pack(v.z, v.w);
+        __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)off, 0, 0);
+    }
+    // value the factor takes once stored
+    static __device__ __forceinline__ float round(float v) { return (float)(__bf16)v; }
 };
 
 template <typename T>
 struct GArgs {
-    const Pt32* __restrict__ grec;
-    const uint2* __restrict__ ginfo;
+    const uint4* __restrict__ grec;
     const int32_t* __restrict__ ngroups;
     const int32_t* __restrict__ off2;
     const int64_t* __restrict__ blk;
     const uint32_t* __restrict__ sched;
-    const int64_t* __restrict__ voff;
+    const int64_t* __restrict__ goff;
     const int32_t* __restrict__ need;
     int32_t* ready;
     T* W;
     T* H;
-    const float* __restrict__ inv1;
     const float* __restrict__ inv2;
-    int d, r, ld, max_cols;
+    uint32_t w_bytes;          // W buffer size (rows m + FACTOR_TAIL_ROWS)
+    uint32_t ztail_b;          // byte offset of the zero rows after the last row
+    uint32_t dummy_b;          // byte offset of the dummy row
+    int d, ld, max_cols;
     float eps, lambda, tmin, tmax;
-    int kind, truncate;
+    unsigned long long* prof;   // optional: per task {wait, stage, loop, total} clocks + group count
 };
 
-struct GroupIn {      // stage A: descriptor + this slot's record
-    uint2 info;
-    Pt32 rec;
-};
-struct RowIn {        // stage B: this lane's four W elements + 1/nnz of the slot's point
-    float4 w;
-    float i1, i2;
-};
-
-// Loads are unconditional: indices past the last group are clamped to it.
-// (gi, gr point at this visit's descriptors / records; offsets fit in 32 bits)
-template <typename T>
-__device__ __forceinline__ void load_group(const uint2* __restrict__ gi, const Pt32* __restrict__ gr, int K,
-                                           int last, int slot, GroupIn& out) {
-    const int k = min(K, last);
-    out.info = gi[k];
-    out.rec = gr[(uint32_t)k * GROUP_SLOTS + slot];
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-template <typename T>
-__device__ __forceinline__ void load_row(const GArgs<T>& A, int lq, const GroupIn& g, RowIn& out) {
-    out.w = Vec4<T>::load(A.W + ((uint32_t)g.rec.i * (uint32_t)A.ld + 4 * lq));
-    out.i1 = A.inv1[(uint32_t)g.rec.i];
-    out.i2 = A.inv2[(uint32_t)g.rec.j];
+__device__ __forceinline__ uint4 load_rec(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, int K, int last) {
+    const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)lane_off, min(K, last) * 64, 0);
+    return make_uint4(u.x, u.y, u.z, u.w);
 }
 
-template <typename T>
-__device__ __forceinline__ void run_group(const GArgs<T>& A, int K, bool live, int slot, int l16, bool pad,
-                                          int col0, const GroupIn& g, const RowIn& wr, float* hblk, float* ring) {
-    const uint32_t size = g.info.x;
-    const uint32_t fw = (g.info.y >> (8 * slot)) & 0xffu;
-    const bool member = live && (uint32_t)slot < size;
-    float4 w = pad ? make_float4(0.f, 0.f, 0.f, 0.f) : wr.w;
-    if (member && (fw & 0x80u)) {   // updated by a recent group: take the LDS copy
-        const int src = (int)(((K - ((fw >> 2) & 15u)) & (GROUP_RING - 1)) * GROUP_SLOTS + (fw & 3u));
-        w = *(const float4*)(ring + src * 64 + 4 * l16);
-    }
-    const int jl = g.rec.j - col0;
-    float* hp = hblk + jl * 64 + 4 * l16;
-    const float4 h = *(const float4*)hp;
-    float part = w.x * h.x;
-    part = fmaf(w.y, h.y, part);
-    part = fmaf(w.z, h.z, part);
-    part = fmaf(w.w, h.w, part);
-    const float wh = row16_sum(part);
-    const float eps = A.eps;
-    const float f1 = (eps * -2.f) * (g.rec.x - wh);
-    const float f2 = (eps * 2.f) * A.lambda;
-    float a = 0.f, b = 0.f;
-    if (A.kind == UPD_NZSL_L2) { a = f2 * wr.i1; b = f2 * wr.i2; }
-    else if (A.kind == UPD_NZSL_NZL2) { a = f2; b = f2; }
-    float4 wn, hn;
-    wn.x = fmaf(-f1, h.x, fmaf(-a, w.x, w.x));
-    wn.y = fmaf(-f1, h.y, fmaf(-a, w.y, w.y));
-    wn.z = fmaf(-f1, h.z, fmaf(-a, w.z, w.z));
-    wn.w = fmaf(-f1, h.w, fmaf(-a, w.w, w.w));
-    hn.x = fmaf(-f1, w.x, fmaf(-b, h.x, h.x));
-    hn.y = fmaf(-f1, w.y, fmaf(-b, h.y, h.y));
-    hn.z = fmaf(-f1, w.z, fmaf(-b, h.z, h.z));
-    hn.w = fmaf(-f1, w.w, fmaf(-b, h.w, h.w));
-    if (A.truncate) {   // update-truncate.h:50-64
-#define TRUNC(v) v = (v <= A.tmin) ? A.tmin + eps : ((v >= A.tmax) ? A.tmax - eps : v)
-        TRUNC(wn.x); TRUNC(wn.y); TRUNC(wn.z); TRUNC(wn.w);
-        TRUNC(hn.x); TRUNC(hn.y); TRUNC(hn.z); TRUNC(hn.w);
-#undef TRUNC
-    }
-    wn.x = Vec4<T>::round(wn.x); wn.y = Vec4<T>::round(wn.y); wn.z = Vec4<T>::round(wn.z); wn.w = Vec4<T>::round(wn.w);
-    hn.x = Vec4<T>::round(hn.x); hn.y = Vec4<T>::round(hn.y); hn.z = Vec4<T>::round(hn.z); hn.w = Vec4<T>::round(hn.w);
-    if (member && !pad) {
-        Vec4<T>::store(A.W + ((uint32_t)g.rec.i * (uint32_t)A.ld + 4 * l16), wn);
-        *(float4*)(ring + (int)((K & (GROUP_RING - 1)) * GROUP_SLOTS + slot) * 64 + 4 * l16) = wn;
-        *(float4*)hp = hn;
-    }
-}
-
-template <typename T>
+template <typename T, int KIND, bool TRUNC>
 __global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
     extern __shared__ float lds[];
     float* ring = lds;                                  // GROUP_RING x 4 slots x 64
-    float* hblk = lds + GROUP_RING * GROUP_SLOTS * 64;  // max_cols x 64
+    float* hblk = lds + GROUP_RING * GROUP_SLOTS * 64;  // (max_cols + 1) x 64; the last row is the dummy column
+    float* hinv = hblk + (A.max_cols + 1) * 64;         // 1/nnz of the staged columns
     const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
-    const bool pad = 4 * l16 >= A.ld;                   // lanes past the row length
-    const int lq = pad ? 0 : l16;                       // their loads repeat element 0..3
+    constexpr uint32_t ES = Vec4<T>::BYTES;
+    // lanes past the row length load zeros from the tail and store into the dummy row
+    const bool pad = 4 * l16 >= A.ld;
+    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * ES;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(4 * l16 - A.ld) * ES : 4u * l16 * ES;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(4 * l16 - A.ld) * ES : 4u * l16 * ES;
+    const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
+    const uint32_t rec_off = 16u * slot;
+    float* const ring_lane = ring + slot * 64 + 4 * l16;
+    char* const hblk_lane = (char*)(hblk + 4 * l16);   // + column << 8 (64 floats per column)
+    const char* const hinv_b = (const char*)hinv;       // + column << 2
+    const float eps = A.eps;
+    const float c1 = eps * -2.f;
+    const float f2 = (eps * 2.f) * A.lambda;
     const int t = blockIdx.x, d = A.d;
     for (int e = lane; e < GROUP_RING * GROUP_SLOTS * 64; e += 64) ring[e] = 0.f;
+    unsigned long long c_wait = 0, c_stage = 0, c_loop = 0, n_groups = 0;
+    const unsigned long long c_begin = A.prof ? clock64() : 0;
+    float4 hprev = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int s = 0; s < 2 * d; ++s) {
+        // per-visit values are uniform: keep them in scalar registers
         const int visit = s * d + t;
-        const int b2 = (int)A.sched[visit];
+        const int b2 = uniform_i32((int)A.sched[visit]);
         const int b = t * 2 * d + b2;
         if (A.blk[b + 1] == A.blk[b]) continue;
-        const int64_t base = A.voff[visit];
-        const uint2* gi = A.ginfo + base;
-        const Pt32* gr = A.grec + base * GROUP_SLOTS;
-        const int ng = A.ngroups[visit];
-        const int last = ng - 1;
-        const int col0 = A.off2[b2];
-        const int ncol = A.off2[b2 + 1] - col0;
-        GroupIn qa[GQ];
-        RowIn qb[GPREF];
+        const int ng = uniform_i32(A.ngroups[visit]);
+        const int ngp = (ng + GQ - 1) / GQ * GQ;   // the prep kernel pads with empty groups
+        const int last = ngp - 1;
+        const int64_t gbase = uniform_i64(A.goff[visit]);
+        const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
+        const int col0 = uniform_i32(A.off2[b2]);
+        const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
+        uint4 qa[GQ];
+        float4 qb[GPREF];
         // prologue (W rows belong to this task: no need to wait for the column block)
 #pragma unroll
-        for (int u = 0; u < GQ; ++u) load_group<T>(gi, gr, u, last, slot, qa[u]);
+        for (int u = 0; u < GQ; ++u) qa[u] = load_rec(Gr, rec_off, u, last);
 #pragma unroll
-        for (int u = 0; u < GPREF; ++u) load_row(A, lq, qa[u], qb[u]);
+        for (int u = 0; u < GPREF; ++u) qb[u] = Vec4<T>::bload(Wr, __umul24(qa[u].x, ldb) + lbase);
+        const unsigned long long c0 = A.prof ? clock64() : 0;
         dataflow_wait(A.ready, b2, A.need[visit]);
+        const unsigned long long c1c = A.prof ? clock64() : 0;
         for (int e = lane; e < ncol * 16; e += 64) {
             const int c = e >> 4, q = e & 15;
             *(float4*)(hblk + c * 64 + 4 * q) = 4 * q < A.ld ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
                                                              : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        if (KIND == UPD_NZSL_L2)
+            for (int c = lane; c < ncol; c += 64) hinv[c] = A.inv2[col0 + c];
         __syncthreads();
-        for (int K0 = 0; K0 < ng; K0 += GQ) {
+        const unsigned long long c2 = A.prof ? clock64() : 0;
+        float4 hq = *(const float4*)(hblk_lane + (qa[0].y << 8));
+        float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
+        for (int K0 = 0; K0 < ngp; K0 += GQ) {
 #pragma unroll
             for (int u = 0; u < GQ; ++u) {
                 const int K = K0 + u;
-                run_group(A, K, K < ng, slot, l16, pad, col0, qa[u], qb[u % GPREF], hblk, ring);
-                load_row(A, lq, qa[(u + GPREF) % GQ], qb[u % GPREF]);
-                load_group<T>(gi, gr, K + GQ, last, slot, qa[u]);
+                {
+                    const uint4 rec = qa[u];
+                    float4 w = qb[u % GPREF];
+                    float4 h = hq;
+                    const float ih = ihq;
+                    // operands of group K+1: no member of group K writes them (see above)
+                    const uint32_t ncolx = qa[(u + 1) % GQ].y;
+                    hq = *(const float4*)(hblk_lane + (ncolx << 8));
+                    if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (ncolx << 2));
+                    if (uniform_i32((int)rec.y) < 0) {   // REC_GFWD of slot 0
+                        const float4 fwd = *(const float4*)(ring + (rec.x >> REC_RING_SHIFT) * 64 + 4 * l16);
+                        const bool wf = (rec.x & REC_WFWD) != 0;
+                        w.x = wf ? fwd.x : w.x; w.y = wf ? fwd.y : w.y;
+                        w.z = wf ? fwd.z : w.z; w.w = wf ? fwd.w : w.w;
+                    }
+                    const bool hr = (rec.x & REC_HREG) != 0;
+                    h.x = hr ? hprev.x : h.x; h.y = hr ? hprev.y : h.y;
+                    h.z = hr ? hprev.z : h.z; h.w = hr ? hprev.w : h.w;
+                    float part = w.x * h.x;
+                    part = fmaf(w.y, h.y, part);
+                    part = fmaf(w.z, h.z, part);
+                    part = fmaf(w.w, h.w, part);
+                    const float wh = row16_sum(part);
+                    const float f1 = c1 * (__uint_as_float(rec.z) - wh);
+                    float a = 0.f, bb = 0.f;
+                    if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = f2 * ih; }
+                    if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+                    float4 wn, hn;
+                    wn.x = fmaf(-f1, h.x, fmaf(-a, w.x, w.x));
+                    wn.y = fmaf(-f1, h.y, fmaf(-a, w.y, w.y));
+                    wn.z = fmaf(-f1, h.z, fmaf(-a, w.z, w.z));
+                    wn.w = fmaf(-f1, h.w, fmaf(-a, w.w, w.w));
+                    hn.x = fmaf(-f1, w.x, fmaf(-bb, h.x, h.x));
+                    hn.y = fmaf(-f1, w.y, fmaf(-bb, h.y, h.y));
+                    hn.z = fmaf(-f1, w.z, fmaf(-bb, h.z, h.z));
+                    hn.w = fmaf(-f1, w.w, fmaf(-bb, h.w, h.w));
+                    if (TRUNC) {   // update-truncate.h:50-64
+#define TRUNC_(v) v = (v <= A.tmin) ? A.tmin + eps : ((v >= A.tmax) ? A.tmax - eps : v)
+                        TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(wn.z); TRUNC_(wn.w);
+                        TRUNC_(hn.x); TRUNC_(hn.y); TRUNC_(hn.z); TRUNC_(hn.w);
+#undef TRUNC_
+                    }
+                    wn.x = Vec4<T>::round(wn.x); wn.y = Vec4<T>::round(wn.y);
+                    wn.z = Vec4<T>::round(wn.z); wn.w = Vec4<T>::round(wn.w);
+                    hn.x = Vec4<T>::round(hn.x); hn.y = Vec4<T>::round(hn.y);
+                    hn.z = Vec4<T>::round(hn.z); hn.w = Vec4<T>::round(hn.w);
+                    hprev = hn;
+                    Vec4<T>::bstore(Wr, __umul24(rec.x, ldb) + sbase, wn);
+                    *(float4*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * 64)) = wn;
+                    *(float4*)(hblk_lane + (rec.y << 8)) = hn;
+                    qb[u % GPREF] = Vec4<T>::bload(Wr, __umul24(qa[(u + GPREF) % GQ].x, ldb) + lbase);
+                    qa[u] = load_rec(Gr, rec_off, K + GQ, last);
+                }
             }
         }
         __syncthreads();
+        if (A.prof) {
+            const unsigned long long c3 = clock64();
+            c_wait += c1c - c0;
+            c_stage += c2 - c1c;
+            c_loop += c3 - c2;
+            n_groups += (unsigned long long)ng;
+        }
         for (int e = lane; e < ncol * 16; e += 64) {
             const int c = e >> 4, q = e & 15;
             if (4 * q < A.ld) Vec4<T>::store(A.H + (int64_t)(col0 + c) * A.ld + 4 * q, *(const float4*)(hblk + c * 64 + 4 * q));
         }
         dataflow_signal(A.ready, b2);
     }
+    if (A.prof && lane == 0) {
+        A.prof[5 * t + 0] = c_wait;
+        A.prof[5 * t + 1] = c_stage;
+        A.prof[5 * t + 2] = c_loop;
+        A.prof[5 * t + 3] = clock64() - c_begin;
+        A.prof[5 * t + 4] = n_groups;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static size_t prep_lds(int64_t cap, int rows) { return (size_t)(cap + rows) * sizeof(uint32_t); }
-static size_t epoch_lds(int max_cols) { return (size_t)(GROUP_RING * GROUP_SLOTS + max_cols) * 64 * sizeof(float); }
+static size_t epoch_lds(int max_cols) {
+    return ((size_t)(GROUP_RING * GROUP_SLOTS + max_cols + 1) * 64 + max_cols + 1) * sizeof(float);
+}
 
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols, int device,
                        std::string* why) {
@@ -350,19 +470,20 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     if (P.precision != PREC_FP32 && P.precision != PREC_BF16) return no("grouped kernel runs FP32/BF16 factors");
     if (P.ld > 64) return no("grouped kernel needs rank <= 64");
     if (max_rows >= 65536 || max_cols >= 65536) return no("block dimensions above 65535");
-    if (max_block >= (1 << 24)) return no("block above 2^24 points");
-    if ((P.m + 1) * P.ld >= (int64_t)1 << 31 || (P.n + 1) * P.ld >= (int64_t)1 << 31)
-        return no("factor offsets above 2^31 elements");
+    if (max_block >= (1 << 22)) return no("block above 2^22 points");
+    const int64_t es = P.precision == PREC_FP32 ? 4 : 2;
+    if ((P.m + FACTOR_TAIL_ROWS) * P.ld * es >= ((int64_t)1 << 32) || (P.n + 1) * P.ld >= ((int64_t)1 << 31))
+        return no("factor offsets above 2^32 bytes");
+    if (P.m + FACTOR_TAIL_ROWS >= (1 << 24)) return no("2^24 rows or more (packed records)");
     int lmax = 0;
     if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return no("no device");
-    if (prep_lds(max_block, max_rows) > (size_t)lmax) return no("prep LDS too large");
+    if (prep_lds(max_block, max_rows, max_cols) > (size_t)lmax) return no("prep LDS too large");
     if (epoch_lds(max_cols) > (size_t)lmax) return no("H block does not fit in LDS");
+    const void* fn = (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false>;
     int per_cu = 0, cus = 0;
-    if (hipFuncSetAttribute((const void*)k_epoch_grouped<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)epoch_lds(max_cols)) != hipSuccess)
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)epoch_lds(max_cols)) != hipSuccess)
         return no("LDS attribute");
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_epoch_grouped<float>, 64,
-                                                     epoch_lds(max_cols)) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, epoch_lds(max_cols)) != hipSuccess)
         return no("occupancy query");
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return no("no device");
     if ((int64_t)per_cu * cus < P.d) return no("not all tasks can be resident");
@@ -372,53 +493,74 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s) {
     if (nchains <= 0) return;
-    const size_t bytes = prep_lds(std::max<int64_t>(max_block, 1), B.max_rows);
+    const int64_t cap = std::max<int64_t>(max_block, 1);
+    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols);
     HIPK(hipFuncSetAttribute((const void*)k_prep_grouped, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_prep_grouped<<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
-                                              (Pt32*)B.grec, B.ginfo, B.ngroups, std::max<int64_t>(max_block, 1),
-                                              B.max_rows);
+                                              (uint4*)B.grec, B.ngroups, (const float*)P.inv1, cap, B.max_rows,
+                                              B.max_cols, (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2),
+                                              (uint32_t)B.max_cols);
     HIPK(hipGetLastError());
 }
 
-template <typename T>
+template <typename T, int KIND, bool TRUNC>
 static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                             const EpochLaunch& L, hipStream_t s) {
     GArgs<T> A;
-    A.grec = (const Pt32*)B.grec;
-    A.ginfo = B.ginfo;
+    A.grec = (const uint4*)B.grec;
     A.ngroups = B.ngroups;
     A.off2 = B.off2;
     A.blk = P.blk;
     A.sched = L.sched;
-    A.voff = L.voff;
+    A.goff = L.goff;
     A.need = L.need;
     A.ready = L.ready;
     A.W = (T*)P.W;
     A.H = (T*)P.H;
-    A.inv1 = (const float*)P.inv1;
     A.inv2 = (const float*)P.inv2;
+    const uint32_t es = Vec4<T>::BYTES;
+    A.w_bytes = (uint32_t)((P.m + FACTOR_TAIL_ROWS) * P.ld * es);
+    A.ztail_b = (uint32_t)(P.m * P.ld * es);
+    A.dummy_b = (uint32_t)((P.m + FACTOR_TAIL_ROWS / 2) * P.ld * es);
     A.d = P.d;
-    A.r = (int)P.r;
     A.ld = (int)P.ld;
     A.max_cols = B.max_cols;
     A.eps = (float)L.eps;
     A.lambda = (float)U.lambda;
     A.tmin = (float)U.tmin;
     A.tmax = (float)U.tmax;
-    A.kind = U.kind;
-    A.truncate = U.truncate ? 1 : 0;
+    A.prof = B.prof;
     const size_t bytes = epoch_lds(B.max_cols);
-    HIPK(hipFuncSetAttribute((const void*)k_epoch_grouped<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    const void* fn = (const void*)k_epoch_grouped<T, KIND, TRUNC>;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    HIPK(hipLaunchCooperativeKernel((const void*)k_epoch_grouped<T>, dim3(P.d), dim3(64), args, bytes, s));
+    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(64), args, bytes, s));
+}
+
+template <typename T, int KIND>
+static void epoch_grouped_k(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                            const EpochLaunch& L, hipStream_t s) {
+    if (U.truncate) epoch_grouped_t<T, KIND, true>(P, U, B, L, s);
+    else epoch_grouped_t<T, KIND, false>(P, U, B, L, s);
+}
+
+template <typename T>
+static void epoch_grouped_p(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                            const EpochLaunch& L, hipStream_t s) {
+    switch (U.kind) {
+        case UPD_NZSL: epoch_grouped_k<T, UPD_NZSL>(P, U, B, L, s); break;
+        case UPD_NZSL_L2: epoch_grouped_k<T, UPD_NZSL_L2>(P, U, B, L, s); break;
+        case UPD_NZSL_NZL2: epoch_grouped_k<T, UPD_NZSL_NZL2>(P, U, B, L, s); break;
+        default: throw std::invalid_argument("unknown update kind");
+    }
 }
 
 void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                           const EpochLaunch& L, hipStream_t s) {
-    if (P.precision == PREC_FP32) epoch_grouped_t<float>(P, U, B, L, s);
-    else if (P.precision == PREC_BF16) epoch_grouped_t<bf16_t>(P, U, B, L, s);
+    if (P.precision == PREC_FP32) epoch_grouped_p<float>(P, U, B, L, s);
+    else if (P.precision == PREC_BF16) epoch_grouped_p<bf16_t>(P, U, B, L, s);
     else throw std::invalid_argument("grouped kernel runs FP32/BF16 factors");
 }
 

@@ -25,7 +25,8 @@ def load():
 
 
 def _check(h, g, fx):
-    assert h.loss(0) == float.fromhex(g["loss0"])
+    l0, w0 = h.loss(0), float.fromhex(g["loss0"])
+    assert abs(l0 - w0) <= 1e-13 * abs(w0)   # GPU loss: tree reduction order
     for e, (eps, ep) in enumerate(zip(g["eps"], g["epochs"])):
         h.epoch(eps)
         assert h.schedule().tolist() == ep["schedule"]
