@@ -759,6 +759,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     HIPC(hipEventRecord(c->ev[2], c->stream));
     const int used = c->cur;
     c->lastPlan = used;
+    P.drawn = false;   // consumed: the next use of this slot draws a new plan
     c->cur ^= 1;
     // look ahead: the next epoch's plan, drawn now in master-stream order
     if (c->lookahead) {

@@ -67,84 +67,95 @@ __device__ __forceinline__ uint32_t writelane_u32(uint32_t v, int lane, uint32_t
 // ---------------------------------------------------------------------------
 // prep: visit order, groups and packed records
 // ---------------------------------------------------------------------------
-static size_t prep_lds(int64_t cap, int rows, int cols) {
-    return (size_t)(cap + rows + cols) * sizeof(uint32_t) + (size_t)((cap + GROUP_PAD + 3) & ~int64_t(3));
+// LDS of the prep kernel.  Narrow mode (blocks below PREP_NARROW points) keeps
+// the visit order and the row/column tags in 16 bits, which fits more
+// workgroups per CU.
+constexpr int64_t PREP_NARROW = 16000;   // tag ((group + 2) << 2 | slot) < 2^16
+static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow) {
+    const size_t es = narrow ? 2 : 4;
+    auto up4 = [](size_t b) { return (b + 3) & ~(size_t)3; };
+    return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + up4((size_t)cap + GROUP_PAD);
 }
 
+template <typename IX>
 __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
                                                      const int32_t* __restrict__ chain_start, int order,
                                                      const Pt32* __restrict__ pts, uint32_t* __restrict__ perm,
                                                      uint4* __restrict__ grec, int32_t* __restrict__ ngroups,
-                                                     const float* __restrict__ inv1, int64_t cap, int rows_cap,
+                                                     const float* __restrict__ inv1, int cap, int rows_cap,
                                                      int cols_cap, uint32_t dummy_row, uint32_t dummy_col) {
     extern __shared__ uint32_t sm[];
-    uint32_t* vis = sm;                          // visit order
-    uint32_t* lastR = vis + cap;                 // per local row: tag of its last update
-    uint32_t* lastC = lastR + rows_cap;          // per local column: the same
-    uint32_t* gmw = lastC + cols_cap;            // slot masks of the groups, 4 per word (+ padding groups)
+    auto up4 = [](int k) { return (k * (int)sizeof(IX) + 3) >> 2; };   // words
+    IX* vis = (IX*)sm;                               // visit order
+    IX* lastR = (IX*)(sm + up4(cap));                // per local row: tag of its last update
+    IX* lastC = (IX*)(sm + up4(cap) + up4(rows_cap));  // per local column: the same
+    uint32_t* gmw = sm + up4(cap) + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 per word
     const int lane = threadIdx.x;
     for (int jx = chain_start[blockIdx.x]; jx < chain_start[blockIdx.x + 1]; ++jx) {
         const PermJob J = jobs[jx];
-        const int64_t n = J.n;
+        const int n = (int)J.n;   // < 2^22
         // 1. visit order (sgd_impl.h:198-351)
         if (order == 0) {             // SEQ
-            for (int64_t k = lane; k < n; k += 64) vis[k] = (uint32_t)k;
+            for (int k = lane; k < n; k += 64) vis[k] = (IX)k;
             __syncthreads();
         } else if (order == 1) {      // WR
-            for (int64_t k = lane; k < n; k += 64)
-                vis[k] = (uint32_t)bounded(stream_draw(J.k0, J.k1, J.draw + k), (uint64_t)n);
+            for (int k = lane; k < n; k += 64)
+                vis[k] = (IX)bounded(stream_draw(J.k0, J.k1, J.draw + k), (uint64_t)n);
             __syncthreads();
         } else {                      // WOR: Fisher-Yates, 64 steps at a time in registers
             if (J.carry_base >= 0) {
-                for (int64_t k = lane; k < n; k += 64) vis[k] = perm[J.carry_base + k];
+                for (int k = lane; k < n; k += 64) vis[k] = (IX)perm[J.carry_base + k];
             } else {
-                for (int64_t k = lane; k < n; k += 64) vis[k] = (uint32_t)k;
+                for (int k = lane; k < n; k += 64) vis[k] = (IX)k;
             }
             __syncthreads();
-            for (int64_t base = 0; base < n - 1; base += 64) {
+            for (int base = 0; base < n - 1; base += 64) {
                 // lane l owns position base + l (cur) and, when step base + l swaps
                 // with a position past this chunk, the current value there (pre)
-                const int64_t s = base + lane, hi = base + 64;
-                int64_t t = s;
-                if (s < n - 1) t = s + (int64_t)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)(n - s));
+                const int s = base + lane, hi = base + 64;
+                int t = s;
+                if (s < n - 1) t = s + (int)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)(n - s));
                 const bool outside = t >= hi;
-                uint32_t cur = s < n ? vis[s] : 0u;
-                uint32_t pre = outside ? vis[t] : 0u;
-                const int cnt = (int)min<int64_t>(64, n - 1 - base);
+                uint32_t cur = s < n ? (uint32_t)vis[s] : 0u;
+                uint32_t pre = outside ? (uint32_t)vis[t] : 0u;
+                const int cnt = min(64, n - 1 - base);
                 for (int l = 0; l < cnt; ++l) {
-                    const uint32_t T = readlane_u32((uint32_t)t, l);   // t < 2^32: n < 2^24
+                    const int T = (int)readlane_u32((uint32_t)t, l);
                     const uint32_t a = readlane_u32(cur, l);
-                    if ((int64_t)T < hi) {
-                        const int tl = (int)((int64_t)T - base);
+                    if (T < hi) {
+                        const int tl = T - base;
                         const uint32_t b = readlane_u32(cur, tl);
                         cur = writelane_u32(b, l, cur);
                         cur = writelane_u32(a, tl, cur);
                     } else {
                         const uint32_t b = readlane_u32(pre, l);
                         cur = writelane_u32(b, l, cur);
-                        pre = (uint32_t)t == T ? a : pre;   // every lane aimed at T sees its new value
+                        pre = t == T ? a : pre;   // every lane aimed at T sees its new value
                     }
                 }
-                if (s < n) vis[s] = cur;
-                if (outside) vis[t] = pre;   // lanes aimed at the same position store the same value
+                if (s < n) vis[s] = (IX)cur;
+                if (outside) vis[t] = (IX)pre;   // lanes aimed at the same position store the same value
                 __syncthreads();
             }
         }
-        const int gwords = (int)((n + GROUP_PAD + 3) >> 2);
+        const int gwords = (n + GROUP_PAD + 3) >> 2;
         for (int k = lane; k < rows_cap; k += 64) lastR[k] = 0;
         for (int k = lane; k < cols_cap; k += 64) lastC[k] = 0;
         for (int k = lane; k < gwords; k += 64) gmw[k] = 0;
         __syncthreads();
         // 2. list scheduling, 64 visit positions at a time.  The scan runs on
-        //    uniform values.  Row and column states are tags ((group + 1) << 2 |
-        //    slot, 0 = none) held by the chunk's lanes; every placement updates
-        //    all lanes that share the row or column.  The slot masks of groups
-        //    maxg-62 .. maxg sit in lanes (group g in lane g % 64) and the lane of
-        //    maxg+1 is kept clear, so a point goes to a group in that range.
-        int maxg = -1;
+        //    uniform values.  Groups are numbered from 2 inside the scan (G = g + 2)
+        //    so that a tag (G << 2 | slot) is never 0, the "no update yet" state,
+        //    and "the group after the column's last one" is never a real group
+        //    for an untouched column.  Row and column tags sit in the chunk's
+        //    lanes; every placement updates all lanes that share the row or
+        //    column.  The slot masks of groups maxG-62 .. maxG sit in lanes (group
+        //    G in lane G % 64) and the lane of maxG+1 is kept clear, so the search
+        //    ends there at the latest.
+        int maxG = 1, lo = 2;
         uint32_t win = 0;
-        for (int64_t v0 = 0; v0 < n; v0 += 64) {
-            const int64_t v = v0 + lane;
+        for (int v0 = 0; v0 < n; v0 += 64) {
+            const int v = v0 + lane;
             Pt32 e = Pt32{J.row0, J.col0, 0.f, 0u};
             if (v < n) {
                 const uint32_t q = vis[v];
@@ -152,46 +163,49 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 e = pts[J.s0 + q];
             }
             const uint32_t r = (uint32_t)(e.i - J.row0), c = (uint32_t)(e.j - J.col0);
-            uint32_t myR = v < n ? lastR[r] : 0u, myC = v < n ? lastC[c] : 0u;
-            uint32_t res = 0;   // group << 10 | ring entry << 4 | wfwd << 3 | hreg << 2 | slot
-            const int cnt = (int)min<int64_t>(64, n - v0);
+            uint32_t myR = v < n ? (uint32_t)lastR[r] : 0u, myC = v < n ? (uint32_t)lastC[c] : 0u;
+            uint32_t res = 0;    // G << 3 | hreg << 2 | slot
+            uint32_t prevR = 0;  // the row's tag before this point
+            const int cnt = min(64, n - v0);
             for (int l = 0; l < cnt; ++l) {
                 const uint32_t R = readlane_u32(r, l), C = readlane_u32(c, l);
                 const uint32_t LR = readlane_u32(myR, l), LC = readlane_u32(myC, l);
-                const int gR = (int)(LR >> 2) - 1, gC = (int)(LC >> 2) - 1;
-                const uint32_t sC = LC & 3u;
-                int g = max(max(gR, gC) + 1, maxg - 62);
-                uint32_t mask;
-                for (;;) {   // ends at maxg + 1 at the latest (its lane is clear)
+                const int gH = (int)(LC >> 2) + 1;                 // H_j from registers in this group
+                const uint32_t only = 15u ^ (1u << (LC & 3u));     // the other slots
+                int g = max(max((int)(LR >> 2), (int)(LC >> 2)) + 1, lo);
+                uint32_t mask, em;
+                for (;;) {
                     mask = readlane_u32(win, g & 63);
-                    const uint32_t need = (LC != 0u && gC == g - 1) ? (1u << sC) : 0u;
-                    if ((mask & 15u) != 15u && !(mask & need)) break;
+                    em = g == gH ? (mask | only) : mask;
+                    if ((em & 15u) != 15u) break;
                     ++g;
                 }
-                // H_j from the previous group's result in the same slot (registers)
-                const uint32_t hreg = (LC != 0u && gC == g - 1) ? 1u : 0u;
-                const uint32_t fr = ~mask & 15u;
-                const uint32_t sl = (LC != 0u && ((fr >> sC) & 1u)) ? sC : (uint32_t)__builtin_ctz(fr);
-                if (g > maxg) {
-                    maxg = g;
+                const uint32_t hreg = g == gH ? 1u : 0u;
+                const uint32_t sl = (uint32_t)__builtin_ctz(~em & 15u);
+                if (g > maxG) {
+                    maxG = g;
+                    lo = max(g - 62, 2);
                     win = lane == ((g + 1) & 63) ? 0u : win;
                 }
                 win = writelane_u32(mask | (1u << sl), g & 63, win);
-                const uint32_t tag = ((uint32_t)(g + 1) << 2) | sl;
+                const uint32_t tag = ((uint32_t)g << 2) | sl;
                 myR = r == R ? tag : myR;
                 myC = c == C ? tag : myC;
-                uint32_t f = sl | (hreg << 2);
-                if (gR >= 0 && g - gR < GROUP_RING) f |= 8u | (((((uint32_t)gR & 15u) << 2) | (LR & 3u)) << 4);
-                res = writelane_u32(((uint32_t)g << 10) | f, l, res);
+                const bool me = lane == l;
+                res = me ? ((uint32_t)g << 3) | (hreg << 2) | sl : res;
+                prevR = me ? LR : prevR;
             }
             if (v < n) {
-                lastR[r] = myR;   // lanes of one row hold the same final tag
-                lastC[c] = myC;
-                const uint32_t g = res >> 10, sl = res & 3u;
-                atomicOr(gmw + (g >> 2), ((1u << sl) | ((res & 8u) ? GM_FWD : 0u)) << (8 * (g & 3u)));
+                lastR[r] = (IX)myR;   // lanes of one row hold the same final tag
+                lastC[c] = (IX)myC;
+                const uint32_t g = (res >> 3) - 2, sl = res & 3u;
+                // W_i from the LDS ring when the row was updated in the last GROUP_RING groups
+                const int gR = (int)(prevR >> 2) - 2;
+                const bool wf = prevR != 0u && (int)g - gR < GROUP_RING;
+                atomicOr(gmw + (g >> 2), ((1u << sl) | (wf ? GM_FWD : 0u)) << (8 * (g & 3u)));
                 uint4 o;
-                o.x = (uint32_t)e.i | (((res >> 2) & 1u) << 24) | (((res >> 3) & 1u) << 25) |
-                      (((res >> 4) & 63u) << REC_RING_SHIFT);
+                o.x = (uint32_t)e.i | (((res >> 2) & 1u) << 24) |
+                      (wf ? (REC_WFWD | (((((uint32_t)gR & 15u) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
                 o.y = c;
                 o.z = __float_as_uint(e.x);
                 o.w = __float_as_uint(inv1[e.i]);
@@ -199,6 +213,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
             }
             __syncthreads();
         }
+        const int maxg = maxG - 2;
         const int ng = maxg + 1;
         const int ngp = (ng + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;
         // 3. empty slots (and the padding groups) get the dummy record; slot 0
@@ -477,7 +492,7 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     if (P.m + FACTOR_TAIL_ROWS >= (1 << 24)) return no("2^24 rows or more (packed records)");
     int lmax = 0;
     if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return no("no device");
-    if (prep_lds(max_block, max_rows, max_cols) > (size_t)lmax) return no("prep LDS too large");
+    if (prep_lds(max_block, max_rows, max_cols, max_block < PREP_NARROW) > (size_t)lmax) return no("prep LDS too large");
     if (epoch_lds(max_cols) > (size_t)lmax) return no("H block does not fit in LDS");
     const void* fn = (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false>;
     int per_cu = 0, cus = 0;
@@ -493,13 +508,20 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s) {
     if (nchains <= 0) return;
-    const int64_t cap = std::max<int64_t>(max_block, 1);
-    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols);
-    HIPK(hipFuncSetAttribute((const void*)k_prep_grouped, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    k_prep_grouped<<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
-                                              (uint4*)B.grec, B.ngroups, (const float*)P.inv1, cap, B.max_rows,
-                                              B.max_cols, (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2),
-                                              (uint32_t)B.max_cols);
+    const int cap = (int)std::max<int64_t>(max_block, 1);
+    const bool narrow = cap < PREP_NARROW;
+    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow);
+    const void* fn = narrow ? (const void*)k_prep_grouped<uint16_t> : (const void*)k_prep_grouped<uint32_t>;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
+    if (narrow)
+        k_prep_grouped<uint16_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
+                                                            (uint4*)B.grec, B.ngroups, (const float*)P.inv1, cap,
+                                                            B.max_rows, B.max_cols, drow, dcol);
+    else
+        k_prep_grouped<uint32_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
+                                                            (uint4*)B.grec, B.ngroups, (const float*)P.inv1, cap,
+                                                            B.max_rows, B.max_cols, drow, dcol);
     HIPK(hipGetLastError());
 }
 

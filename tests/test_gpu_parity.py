@@ -316,3 +316,24 @@ def test_grouped_carry_over_and_many_tasks():
     Wo, Ho = o.get_factors()
     np.testing.assert_array_equal(Wg, Wo)
     np.testing.assert_array_equal(Hg, Ho)
+
+
+@pytest.mark.parametrize("precision,kernel", [("FP64_STRICT", 0), ("FP32", 3)])
+def test_lookahead_off_matches_oracle(monkeypatch, precision, kernel):
+    """Without the look-ahead plan, every epoch still draws a fresh plan in
+    master-stream order (a consumed plan is never run twice)."""
+    print(f"case: lookahead off {precision} kernel {kernel}", flush=True)
+    monkeypatch.setenv("DSGDPP_LOOKAHEAD", "0")
+    p = skewed_problem(200, 160, 5000, 16, seed=23)
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=4, seed=29, precision=precision, kernel=kernel)
+    if precision == "FP32":
+        o.set_arith("FP32")
+    for e in range(4):
+        g.epoch(0.01)
+        o.epoch(0.01)
+        np.testing.assert_array_equal(g.schedule(), o.schedule(), err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(g.task_seeds(), o.task_seeds(), err_msg=f"epoch {e}")
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
