@@ -17,6 +17,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/dsgdpp.h"
@@ -166,6 +167,8 @@ struct EpochPlan {
     std::vector<int32_t> need;        // 2d x d
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
+    int nSmall = 0;                   // grouped prep: leading chains whose blocks are <= PREP_SPLIT points
+    int64_t capSmall = 0;
     uint32_t* dSched = nullptr;
     int64_t* dVoff = nullptr;
     int64_t* dGoff = nullptr;
@@ -392,6 +395,17 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
     HostBlocks& hb = c->blocks[which];
     buildBlocks<Pt>(c, nnz, I, J, X, pts, hb.start);
     hb.nnz = (int64_t)pts.size();
+    if constexpr (std::is_same<Pt, Pt32>::value) {
+        // training points carry 1/nnz of their row (FP32 bits) for the grouped prep kernel
+        if (which == DSGDPP_TRAIN) {
+            std::vector<int64_t> rc(c->m, 0);
+            for (const Pt& e : pts) rc[e.i]++;
+            for (Pt& e : pts) {
+                const float f = (float)(1. / (double)rc[e.i]);
+                memcpy(&e.aux, &f, 4);
+            }
+        }
+    }
     dalloc(c->dPts[which], pts.size() * sizeof(Pt));
     HIPC(hipMemcpy(c->dPts[which], pts.data(), pts.size() * sizeof(Pt), hipMemcpyHostToDevice));
     dalloc(c->dBlk[which], hb.start.size() * sizeof(int64_t));
@@ -652,6 +666,31 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
                 maxBlock = std::max(maxBlock, j.n);
             }
         chains.push_back((int32_t)jobs.size());
+        P.nSmall = 0;
+        P.capSmall = 0;
+        if (P.grouped && maxBlock > PREP_SPLIT) {
+            // chains of small blocks first: their prep launch needs less LDS per
+            // workgroup, so more of them run at once
+            std::vector<PermJob> j2;
+            std::vector<int32_t> c2;
+            j2.reserve(jobs.size());
+            c2.reserve(chains.size());
+            for (int pass = 0; pass < 2; ++pass)
+                for (size_t k = 0; k + 1 < chains.size(); ++k) {
+                    int64_t mx = 0;
+                    for (int32_t q = chains[k]; q < chains[k + 1]; ++q) mx = std::max(mx, jobs[q].n);
+                    if ((mx <= PREP_SPLIT) != (pass == 0)) continue;
+                    c2.push_back((int32_t)j2.size());
+                    for (int32_t q = chains[k]; q < chains[k + 1]; ++q) j2.push_back(jobs[q]);
+                    if (pass == 0) {
+                        ++P.nSmall;
+                        P.capSmall = std::max(P.capSmall, mx);
+                    }
+                }
+            c2.push_back((int32_t)j2.size());
+            jobs.swap(j2);
+            chains.swap(c2);
+        }
         if (total > P.permCap) {
             P.permCap = total;
             dalloc(P.perm, (size_t)total * sizeof(uint32_t));
@@ -705,10 +744,23 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
     GroupedBuffers G = c->G;
     G.grec = P.grec;
     G.ngroups = P.ngroups;
+    G.pprof = nullptr;
+    if (c->kprof && P.grouped) {   // prep clock sums, one set per plan slot
+        if (!c->G.pprof) {
+            unsigned long long* q = nullptr;
+            dalloc(q, 8 * sizeof(unsigned long long));
+            c->G.pprof = q;
+        }
+        G.pprof = c->G.pprof + (&P == &c->plan[0] ? 0 : 4);
+        HIPC(hipMemsetAsync(G.pprof, 0, 4 * sizeof(unsigned long long), s));
+    }
     const int nchains = (int)P.chains.size() - 1;
     HIPC(hipEventRecord(P.evStart, s));
-    if (P.grouped)
-        launch_prep_grouped(Dp, G, P.dJobs, P.dChains, nchains, c->sgdOrder, maxBlock, s);
+    if (P.grouped) {
+        if (P.nSmall > 0)
+            launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s);
+        launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.nSmall, nchains - P.nSmall, c->sgdOrder, maxBlock, s);
+    }
     else if (c->sgdOrder == DSGDPP_SGD_WOR)
         launch_permute_wor(P.dJobs, P.dChains, nchains, P.perm, maxBlock, s);
     else if (c->sgdOrder == DSGDPP_SGD_WR)
@@ -769,6 +821,13 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     }
     HIPC(hipEventSynchronize(c->ev[2]));
     const auto t1 = std::chrono::steady_clock::now();
+    if (c->kprof && P.grouped && c->G.pprof) {
+        unsigned long long h[4];
+        HIPC(hipMemcpy(h, c->G.pprof + (used == 0 ? 0 : 4), sizeof(h), hipMemcpyDeviceToHost));
+        const double v = (double)std::max<unsigned long long>(1, h[3]);
+        fprintf(stderr, "[kprof] prep clocks/visit: visit order %.3g scheduling %.3g fill %.3g (%llu visits)\n",
+                h[0] / v, h[1] / v, h[2] / v, h[3]);
+    }
     if (c->kprof && P.grouped && c->G.prof) {
         // clock counters of the grouped epoch kernel: the slowest task and the mean
         std::vector<unsigned long long> h((size_t)d * 5);
@@ -1158,6 +1217,7 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     v = c->dPartials; dfree(v);
     v = const_cast<int32_t*>(c->G.off2); dfree(v);
     v = c->G.prof; dfree(v);
+    v = c->G.pprof; dfree(v);
     v = c->dReady; dfree(v);
     if (c->hPinned) (void)hipHostFree(c->hPinned);
     for (auto& e : c->ev)

@@ -92,7 +92,8 @@ bool dataflow_fits(const DevProblem& P, int device);
 // group per wave step (16 lanes per point, H block staged in LDS).
 constexpr int GROUP_SLOTS = 4;
 constexpr int GROUP_RING = 16;    // W forwarding window, in groups
-constexpr int GROUP_PAD = 24;     // every visit's group list is padded to a multiple of this
+constexpr int GROUP_PAD = 32;     // every visit's group list is padded to a multiple of this
+constexpr int64_t PREP_SPLIT = 4096;   // grouped prep: blocks up to this size run in a launch of their own
 // Factor buffers carry FACTOR_TAIL_ROWS spare rows after the last one: the
 // first half stays zero (read by lanes past the row length), the second half
 // takes the stores of empty group slots.
@@ -104,6 +105,7 @@ struct GroupedBuffers {
     int max_cols = 0;             // widest column block
     int max_rows = 0;             // tallest row block
     unsigned long long* prof = nullptr;   // optional kernel clock counters (5 per task)
+    unsigned long long* pprof = nullptr;  // optional prep clock sums {visit order, scheduling, fill, visits}
 };
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols,
                        int device, std::string* why);

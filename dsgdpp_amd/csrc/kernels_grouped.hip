@@ -38,7 +38,7 @@
 
 namespace dsgdpp {
 
-constexpr int GPREF = 12;           // W prefetch distance (groups), <= GROUP_RING
+constexpr int GPREF = 16;           // W prefetch distance (groups), <= GROUP_RING
 constexpr int GQ = 2 * GPREF;       // record prefetch distance = unroll of the group loop
 static_assert(GPREF <= GROUP_RING, "a prefetched W row may be at most GROUP_RING groups old");
 static_assert(GQ == GROUP_PAD, "group lists are padded to whole unrolled iterations");
@@ -81,9 +81,10 @@ template <typename IX>
 __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
                                                      const int32_t* __restrict__ chain_start, int order,
                                                      const Pt32* __restrict__ pts, uint32_t* __restrict__ perm,
-                                                     uint4* __restrict__ grec, int32_t* __restrict__ ngroups,
-                                                     const float* __restrict__ inv1, int cap, int rows_cap,
-                                                     int cols_cap, uint32_t dummy_row, uint32_t dummy_col) {
+                                                     uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
+                                                     int rows_cap,
+                                                     int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
+                                                     unsigned long long* __restrict__ pprof) {
     extern __shared__ uint32_t sm[];
     auto up4 = [](int k) { return (k * (int)sizeof(IX) + 3) >> 2; };   // words
     IX* vis = (IX*)sm;                               // visit order
@@ -91,9 +92,11 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
     IX* lastC = (IX*)(sm + up4(cap) + up4(rows_cap));  // per local column: the same
     uint32_t* gmw = sm + up4(cap) + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 per word
     const int lane = threadIdx.x;
+    unsigned long long t_order = 0, t_sched = 0, t_fill = 0;
     for (int jx = chain_start[blockIdx.x]; jx < chain_start[blockIdx.x + 1]; ++jx) {
         const PermJob J = jobs[jx];
         const int n = (int)J.n;   // < 2^22
+        const unsigned long long t0 = pprof ? clock64() : 0;
         // 1. visit order (sgd_impl.h:198-351)
         if (order == 0) {             // SEQ
             for (int k = lane; k < n; k += 64) vis[k] = (IX)k;
@@ -138,6 +141,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 __syncthreads();
             }
         }
+        const unsigned long long t1 = pprof ? clock64() : 0;
         const int gwords = (n + GROUP_PAD + 3) >> 2;
         for (int k = lane; k < rows_cap; k += 64) lastR[k] = 0;
         for (int k = lane; k < cols_cap; k += 64) lastC[k] = 0;
@@ -154,16 +158,24 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
         //    ends there at the latest.
         int maxG = 1, lo = 2;
         uint32_t win = 0;
-        for (int v0 = 0; v0 < n; v0 += 64) {
-            const int v = v0 + lane;
+        // the next chunk's points are fetched while the current chunk is scanned
+        // (a training point carries 1/nnz of its row in .aux)
+        auto fetch = [&](int v) {
             Pt32 e = Pt32{J.row0, J.col0, 0.f, 0u};
             if (v < n) {
                 const uint32_t q = vis[v];
                 perm[J.base + v] = q;
                 e = pts[J.s0 + q];
             }
+            return e;
+        };
+        Pt32 en = fetch(lane);
+        for (int v0 = 0; v0 < n; v0 += 64) {
+            const int v = v0 + lane;
+            const Pt32 e = en;
             const uint32_t r = (uint32_t)(e.i - J.row0), c = (uint32_t)(e.j - J.col0);
             uint32_t myR = v < n ? (uint32_t)lastR[r] : 0u, myC = v < n ? (uint32_t)lastC[c] : 0u;
+            en = fetch(v + 64);
             uint32_t res = 0;    // G << 3 | hreg << 2 | slot
             uint32_t prevR = 0;  // the row's tag before this point
             const int cnt = min(64, n - v0);
@@ -208,12 +220,13 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                       (wf ? (REC_WFWD | (((((uint32_t)gR & 15u) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
                 o.y = c;
                 o.z = __float_as_uint(e.x);
-                o.w = __float_as_uint(inv1[e.i]);
+                o.w = e.aux;
                 grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
             }
             __syncthreads();
         }
         const int maxg = maxG - 2;
+        const unsigned long long t2 = pprof ? clock64() : 0;
         const int ng = maxg + 1;
         const int ngp = (ng + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;
         // 3. empty slots (and the padding groups) get the dummy record; slot 0
@@ -229,6 +242,18 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
         }
         if (lane == 0) ngroups[J.visit] = ng;
         __syncthreads();
+        if (pprof) {
+            const unsigned long long t3 = clock64();
+            t_order += t1 - t0;
+            t_sched += t2 - t1;
+            t_fill += t3 - t2;
+        }
+    }
+    if (pprof && lane == 0) {
+        atomicAdd(pprof + 0, t_order);
+        atomicAdd(pprof + 1, t_sched);
+        atomicAdd(pprof + 2, t_fill);
+        atomicAdd(pprof + 3, (unsigned long long)(chain_start[blockIdx.x + 1] - chain_start[blockIdx.x]));
     }
 }
 
@@ -516,12 +541,12 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
     if (narrow)
         k_prep_grouped<uint16_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
-                                                            (uint4*)B.grec, B.ngroups, (const float*)P.inv1, cap,
-                                                            B.max_rows, B.max_cols, drow, dcol);
+                                                            (uint4*)B.grec, B.ngroups, cap,
+                                                            B.max_rows, B.max_cols, drow, dcol, B.pprof);
     else
         k_prep_grouped<uint32_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
-                                                            (uint4*)B.grec, B.ngroups, (const float*)P.inv1, cap,
-                                                            B.max_rows, B.max_cols, drow, dcol);
+                                                            (uint4*)B.grec, B.ngroups, cap,
+                                                            B.max_rows, B.max_cols, drow, dcol, B.pprof);
     HIPK(hipGetLastError());
 }
 
