@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--workload", default="zipf100m", choices=["zipf100m", "quickstart1m"])
     ap.add_argument("--shrink", type=int, default=1)
     ap.add_argument("--precision", default=None)
     ap.add_argument("--kernel", type=int, default=0)
@@ -64,16 +64,15 @@ def main():
         sys.exit(3)
 
     t0 = time.time()
-    if args.config == 2:
-        p = data.config2(device="cuda", shrink=args.shrink)
+    name = args.workload
+    if name == "zipf100m":
+        p = data.zipf_100m(device="cuda", shrink=args.shrink)
         update, decay0, tasks = "Nzsl_L2(0.05)", 0.01, args.tasks or 128
         precision = args.precision or "FP32"
-        name = "config2"
     else:
-        p = data.config1(device="cuda", shrink=args.shrink)
+        p = data.quickstart_1m(device="cuda", shrink=args.shrink)
         update, decay0, tasks = "Nzsl_L2(1)", 0.001, args.tasks or 4
         precision = args.precision or "FP64_STRICT"
-        name = "config1"
     torch.cuda.synchronize()
     m, n, r = p["m"], p["n"], p["rank"]
     nnz, nnz_test = len(p["train"][2]), len(p["test"][2])
@@ -137,7 +136,7 @@ def main():
         cpu = cpu_baseline(p, update, tasks, args.cpu_threads)
 
     out = {
-        "metric": "nnz updates/s (DSGD++ epoch, 100M-nonzero Zipf workload)" if name == "config2"
+        "metric": "nnz updates/s (DSGD++ epoch, 100M-nonzero Zipf workload)" if name == "zipf100m"
                   else "nnz updates/s (DSGD++ epoch, quick-start workload)",
         "value": value,
         "unit": "nnz updates/s",

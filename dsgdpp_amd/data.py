@@ -153,22 +153,23 @@ def zipf_problem(m, n, nnz_train, nnz_test, rank, truth_rank, s=1.0, truth_std=0
                 rule=f"zipf({s})", seed=seed)
 
 
-# Named workloads.  "config1" is the reference quick-start shape, "config2" the
-# realistic 100M-nonzero shape; both take a `shrink` factor for small test runs.
-def config1(seed=1, device="cpu", shrink=1):
+# Named workloads: the reference quick-start shape (100k x 100k, 1M uniform
+# nonzeros, rank 10), the same at rank 128, and the 100M-nonzero Zipf shape
+# (480,189 x 17,770, rank 50); all take a `shrink` factor for small test runs.
+def quickstart_1m(seed=1, device="cpu", shrink=1):
     m = n = 100_000 // shrink
     return uniform_problem(m, n, 1_000_000 // shrink, 100_000 // shrink, rank=10, noise=0.1,
                            seed=seed, device=device)
 
 
-def config2(seed=2, device="cpu", shrink=1, rank=50):
+def zipf_100m(seed=2, device="cpu", shrink=1, rank=50):
     return zipf_problem(480_189 // shrink, 17_770 // max(1, int(math.sqrt(shrink))),
                         100_000_000 // shrink, 1_400_000 // shrink, rank=rank, truth_rank=20,
                         s=1.0, truth_std=0.3, noise=0.8, seed=seed, device=device)
 
 
-def config3(seed=1, device="cpu", shrink=1):
-    p = config1(seed=seed, device=device, shrink=shrink)
+def quickstart_1m_r128(seed=1, device="cpu", shrink=1):
+    p = quickstart_1m(seed=seed, device=device, shrink=shrink)
     rng = np.random.default_rng(seed + 100)
     p["W0"] = rng.uniform(-0.5, 0.5, (p["m"], 128))
     p["H0"] = rng.uniform(-0.5, 0.5, (128, p["n"]))

@@ -164,12 +164,12 @@ def test_run_loop_trace_and_test_loss_strict(tmp_path):
     assert txt.count("list(it=") == 7
 
 
-def test_config1_shape_strict_per_epoch_loss():
-    """The quick-start shape (config 1 family, reduced 4x) for 8 epochs:
+def test_quickstart_shape_strict_per_epoch_loss():
+    """The quick-start shape (reduced 4x) for 8 epochs:
     bit-identical factors and per-epoch loss within 1e-9 relative."""
-    print("case: config1/4 strict, 8 epochs", flush=True)
+    print("case: quick-start shape / 4 strict, 8 epochs", flush=True)
     from dsgdpp_amd import data
-    p = data.config1(shrink=4, seed=3)
+    p = data.quickstart_1m(shrink=4, seed=3)
     g, o = make_pair(p, tasks=4, seed=1234, test=p["test"])
     rg = g.run_native(8, "BoldDriver(0.001)")
     ro = o.run(8, "BoldDriver(0.001)")

@@ -336,10 +336,10 @@ def test_const_decay():
     assert list(r["eps"][1:]) == [0.02] * 4
 
 
-def test_config1_statistics():
+def test_quickstart_statistics():
     """E[NZSL(W0,H0)] = nnz (r + s^2 + r/144); lambda L2(W0,H0) = lambda (m+n) r / 12."""
     from dsgdpp_amd import data
-    p = data.config1(shrink=10, seed=5)
+    p = data.quickstart_1m(shrink=10, seed=5)
     o = _oracle(p, "Nzsl_L2(1)", tasks=4)
     nnz = len(p["train"][0])
     l2 = ((p["W0"] ** 2).sum() + (p["H0"] ** 2).sum())
