@@ -28,6 +28,9 @@ EXPORTS = (
     "dsgdpp_load_factors_mma", "dsgdpp_write_factors_mma", "dsgdpp_epoch", "dsgdpp_loss",
     "dsgdpp_balance", "dsgdpp_run", "dsgdpp_get_schedule", "dsgdpp_get_task_seeds",
     "dsgdpp_get_permutation", "dsgdpp_block_nnz", "dsgdpp_dims",
+    "dsgdpp_mlp_num_params", "dsgdpp_mlp_create", "dsgdpp_mlp_create_error", "dsgdpp_mlp_last_error",
+    "dsgdpp_mlp_destroy", "dsgdpp_mlp_set_params", "dsgdpp_mlp_get_params", "dsgdpp_mlp_forward",
+    "dsgdpp_mlp_gradient", "dsgdpp_mlp_train",
 )
 
 
@@ -100,6 +103,7 @@ def load():
     pu64 = ctypes.POINTER(ctypes.c_uint64)
     pi64 = ctypes.POINTER(ctypes.c_int64)
     pd = ctypes.POINTER(ctypes.c_double)
+    pf = ctypes.POINTER(ctypes.c_float)
     sig = {
         "dsgdpp_create": (i32, [ctypes.POINTER(Cfg), ctypes.POINTER(P)]),
         "dsgdpp_destroy": (None, [P]),
@@ -120,6 +124,16 @@ def load():
         "dsgdpp_get_permutation": (i32, [P, i32, i32, pu32, i64, pi64]),
         "dsgdpp_block_nnz": (i32, [P, i32, pi64]),
         "dsgdpp_dims": (i32, [P, pi64]),
+        "dsgdpp_mlp_num_params": (i32, []),
+        "dsgdpp_mlp_create": (i32, [i32, ctypes.POINTER(P)]),
+        "dsgdpp_mlp_create_error": (ctypes.c_char_p, []),
+        "dsgdpp_mlp_last_error": (ctypes.c_char_p, [P]),
+        "dsgdpp_mlp_destroy": (None, [P]),
+        "dsgdpp_mlp_set_params": (i32, [P, pf]),
+        "dsgdpp_mlp_get_params": (i32, [P, pf]),
+        "dsgdpp_mlp_forward": (i32, [P, pf, i64, pf]),
+        "dsgdpp_mlp_gradient": (i32, [P, pf, pf, i64, pf, pd]),
+        "dsgdpp_mlp_train": (i32, [P, pf, pf, i64, i64, ctypes.c_float, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -163,6 +163,32 @@ int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint
 int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);
 int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
 
+/* ---------------------------------------------------------------------------
+ * Step-size controller network (no counterpart in the reference: it extends
+ * the decay choices of mf/sgd/decay/ with a learned per-epoch step size).
+ * A 9 -> 64 -> 64 -> 1 perceptron with GELU hidden units in FP32, evaluated and
+ * trained on the GPU.  Parameters: W1[64x9] (row = hidden unit), b1[64],
+ * W2[64x64], b2[64], W3[64], b3[1], dsgdpp_mlp_num_params() floats in all.
+ * Training minimises the mean squared error with Adam (0.9, 0.999, 1e-8);
+ * gradients are summed over samples in sample order (reproducible).
+ * --------------------------------------------------------------------------- */
+typedef struct dsgdpp_mlp dsgdpp_mlp;
+int dsgdpp_mlp_num_params(void);
+int dsgdpp_mlp_create(int32_t device, dsgdpp_mlp** out);   /* parameters start at 0 */
+const char* dsgdpp_mlp_create_error(void);
+const char* dsgdpp_mlp_last_error(dsgdpp_mlp* m);
+void dsgdpp_mlp_destroy(dsgdpp_mlp* m);
+int dsgdpp_mlp_set_params(dsgdpp_mlp* m, const float* params);   /* also resets Adam */
+int dsgdpp_mlp_get_params(dsgdpp_mlp* m, float* params);
+/* X: n x 9 row-major, out: n */
+int dsgdpp_mlp_forward(dsgdpp_mlp* m, const float* X, int64_t n, float* out);
+/* gradient of mean((f(X) - Y)^2) with respect to the parameters, and that loss */
+int dsgdpp_mlp_gradient(dsgdpp_mlp* m, const float* X, const float* Y, int64_t n, float* grad,
+                        double* loss);
+/* `steps` full-batch Adam steps; losses (optional, steps entries): loss before each step */
+int dsgdpp_mlp_train(dsgdpp_mlp* m, const float* X, const float* Y, int64_t n, int64_t steps,
+                     float lr, double* losses);
+
 #ifdef __cplusplus
 }
 #endif
