@@ -31,6 +31,8 @@ EXPORTS = (
     "dsgdpp_mlp_num_params", "dsgdpp_mlp_create", "dsgdpp_mlp_create_error", "dsgdpp_mlp_last_error",
     "dsgdpp_mlp_destroy", "dsgdpp_mlp_set_params", "dsgdpp_mlp_get_params", "dsgdpp_mlp_forward",
     "dsgdpp_mlp_gradient", "dsgdpp_mlp_train",
+    "dsgdpp_batch_create", "dsgdpp_batch_create_error", "dsgdpp_batch_last_error", "dsgdpp_batch_destroy",
+    "dsgdpp_batch_epoch",
 )
 
 
@@ -134,6 +136,11 @@ def load():
         "dsgdpp_mlp_forward": (i32, [P, pf, i64, pf]),
         "dsgdpp_mlp_gradient": (i32, [P, pf, pf, i64, pf, pd]),
         "dsgdpp_mlp_train": (i32, [P, pf, pf, i64, i64, ctypes.c_float, pd]),
+        "dsgdpp_batch_create": (i32, [ctypes.POINTER(P), i32, i32, ctypes.POINTER(P)]),
+        "dsgdpp_batch_create_error": (ctypes.c_char_p, []),
+        "dsgdpp_batch_last_error": (ctypes.c_char_p, [P]),
+        "dsgdpp_batch_destroy": (None, [P]),
+        "dsgdpp_batch_epoch": (i32, [P, pd, ctypes.POINTER(EpochStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,171 @@
+"""Batched DSGD++: many independent factorizations advance one epoch each in
+one persistent launch (C ABI ``dsgdpp_batch_*``), plus the autotuner that
+sizes each problem's share of the launch.
+
+Each problem is an ordinary ``api.DsgdPp`` context (its own shape, rank up to
+128, lambda, seed and step size); ``DsgdPpBatch.epoch`` runs all of them in
+one launch of the grouped kernel, bit-identical to running each context's
+``epoch`` alone.  The launch is captured into a HIP graph once and replayed.
+
+Autotuner.  Per problem the free choice is the number of tasks d (d x 2d
+blocking: d workgroups of one wavefront each, and an H tile of n / 2d
+columns x 64 or 128 floats in LDS, by rank).  All tasks of a batch must be
+resident at once, so the tuner splits the device's task budget: it measures,
+once per rank class on a probe problem, the update rate per task for each
+candidate d (cached on disk), predicts every problem's epoch time for each
+candidate, and repeatedly doubles d of the problem with the largest
+predicted time while the budget and the problem's shape allow.
+"""
+import ctypes
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import TRAIN
+
+
+class DsgdPpBatch:
+    def __init__(self, contexts, use_graph=True):
+        self._lib = _lib.load()
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        h = ctypes.c_void_p()
+        rc = self._lib.dsgdpp_batch_create(arr, len(self.contexts), 1 if use_graph else 0, ctypes.byref(h))
+        if rc != _lib.OK:
+            raise RuntimeError(f"dsgdpp_batch_create failed ({rc}): {self._lib.dsgdpp_batch_create_error().decode()}")
+        self._h = h
+
+    def epoch(self, eps):
+        eps = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float64), (len(self.contexts),)))
+        st = _lib.EpochStats()
+        rc = self._lib.dsgdpp_batch_epoch(self._h, _lib.ptr(eps, ctypes.c_double), ctypes.byref(st))
+        if rc != _lib.OK:
+            raise RuntimeError(f"dsgdpp_batch_epoch failed ({rc}): {self._lib.dsgdpp_batch_last_error(self._h).decode()}")
+        return {"seconds": st.seconds, "sgd_seconds": st.sgd_seconds, "updates": st.updates, "launches": st.launches}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dsgdpp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:   # noqa: BLE001
+            pass
+
+
+# ---------------------------------------------------------------------------
+# autotuner
+# ---------------------------------------------------------------------------
+CANDIDATE_TASKS = (1, 2, 4, 8, 16, 32)
+RANK_CLASSES = (16, 32, 64, 128)
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "dsgdpp_amd", "autotune.json")
+
+
+def rank_class(r):
+    for c in RANK_CLASSES:
+        if r <= c:
+            return c
+    raise ValueError("rank above 128")
+
+
+def _probe_problem(rank, seed=0):
+    """100,000 x 20,000 with 2M uniform nonzeros, the given rank."""
+    from . import data
+    return data.uniform_problem(100_000, 20_000, 2_000_000, 0, rank=rank, noise=0.1, seed=seed)
+
+
+def measure_rates(precision="FP32", update="Nzsl_L2(0.05)", device=0, cache=CACHE, log=None):
+    """Updates/s per task for every (rank class, d), measured once per device
+    and cached: {rank: {d: rate}}."""
+    from . import api
+    key = f"{precision}|{update}"
+    table = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            table = json.load(f)
+    if key in table:
+        return {int(r): {int(d): v for d, v in m.items()} for r, m in table[key].items()}
+    rates = {}
+    for r in RANK_CLASSES:
+        p = _probe_problem(r)
+        rates[r] = {}
+        for d in CANDIDATE_TASKS:
+            g = api.DsgdPp(update=update, loss=update, rank=r, tasks_per_rank=d, seed=1, precision=precision,
+                           device=device)
+            g.load_coo(TRAIN, p["m"], p["n"], *p["train"])
+            g.set_factors(p["W0"], p["H0"])
+            g.epoch(1e-4)   # plans and caches warm
+            st = g.epoch(1e-4)
+            g.close()
+            rates[r][d] = len(p["train"][2]) / st["sgd_seconds"] / d
+            if log:
+                log(f"autotune probe rank {r} d {d}: {rates[r][d] * d:.3g} updates/s")
+    if cache:
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        table[key] = {str(r): {str(d): v for d, v in m.items()} for r, m in rates.items()}
+        with open(cache, "w") as f:
+            json.dump(table, f, indent=1)
+    return rates
+
+
+def plan_tasks(problems, budget, rates):
+    """Tasks per problem.  problems: list of dicts with m, n, nnz, rank.
+    Greedy: start at d = 1; double d of the problem with the largest predicted
+    epoch time nnz / (d * rate(d)) while the total stays within `budget` and
+    the problem has at least d rows and 2d columns per block."""
+    d = [1] * len(problems)
+
+    def pred(k, dk):
+        r = rates[rank_class(problems[k]["rank"])]
+        rate = r.get(dk) or r[max(x for x in r if x <= dk)]
+        return problems[k]["nnz"] / (dk * rate)
+
+    while True:
+        order = sorted(range(len(problems)), key=lambda k: -pred(k, d[k]))
+        grown = False
+        for k in order:
+            nd = 2 * d[k]
+            p = problems[k]
+            if nd > max(CANDIDATE_TASKS) or sum(d) + d[k] > budget or p["m"] < nd or p["n"] < 2 * nd:
+                continue
+            if pred(k, nd) >= pred(k, d[k]):
+                continue
+            d[k] = nd
+            grown = True
+            break
+        if not grown:
+            return d
+
+
+def batch_problems(count=64, seed=0, shrink=1, max_nnz=20_000_000):
+    """Independent problems: rows log-uniform in [1e4, 1e6], columns
+    log-uniform in [1e3, 1e5], nnz = U(1e-4, 1e-2) * rows * cols capped at
+    max_nnz, rank-10 N(0,1) ground truth plus N(0, 0.1^2) noise, rank drawn
+    from {16, 32, 64, 128}, lambda log-uniform in [1e-3, 1e-1].  `shrink`
+    divides rows, columns and the cap (quick runs)."""
+    from . import data
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        m = int(math.exp(rng.uniform(math.log(1e4), math.log(1e6))) / shrink)
+        n = int(math.exp(rng.uniform(math.log(1e3), math.log(1e5))) / shrink)
+        nnz = int(min(rng.uniform(1e-4, 1e-2) * m * n, max_nnz / shrink))
+        nnz = max(nnz, 4 * (m + n) // 10)
+        nnz = min(nnz, m * n // 2)
+        rank = int(rng.choice([16, 32, 64, 128]))
+        lam = float(math.exp(rng.uniform(math.log(1e-3), math.log(1e-1))))
+        p = data.uniform_problem(m, n, nnz, 0, rank=rank, truth_rank=10, noise=0.1, seed=seed * 1000 + k,
+                                 init=0.1)
+        p["lambda"] = lam
+        out.append(p)
+    return out
+
+
+def now():
+    return time.perf_counter()

@@ -498,7 +498,10 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
         c->P.r = c->r;
         c->P.d = (int)c->d;
         c->P.precision = c->precision;
-        const int64_t vec = 16 / (int64_t)c->factorBytes();
+        // rows padded to 16 bytes; above rank 64 to 8 elements (the grouped
+        // kernel's lanes then hold 8 elements of a row, never part of the next)
+        const int64_t vec = c->r > 64 ? std::max<int64_t>(8, 16 / (int64_t)c->factorBytes())
+                                      : 16 / (int64_t)c->factorBytes();
         c->P.ld = (c->r + vec - 1) / vec * vec;
         c->dataflowOk = dataflow_fits(c->P, c->device);
     }
@@ -760,8 +763,7 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
         if (P.nSmall > 0)
             launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s);
         launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.nSmall, nchains - P.nSmall, c->sgdOrder, maxBlock, s);
-    }
-    else if (c->sgdOrder == DSGDPP_SGD_WOR)
+    } else if (c->sgdOrder == DSGDPP_SGD_WOR)
         launch_permute_wor(P.dJobs, P.dChains, nchains, P.perm, maxBlock, s);
     else if (c->sgdOrder == DSGDPP_SGD_WR)
         launch_visit_wr(P.dJobs, (int)P.jobs.size(), P.perm, s);
@@ -1377,6 +1379,219 @@ int dsgdpp_dims(dsgdpp_ctx* c, int64_t* out) {
         out[4] = c->blocks[0].nnz;
         out[5] = c->blocks[1].nnz;
     });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched epochs: several contexts (independent factorizations of any shape,
+// rank <= 128, own lambda and step size) advance one epoch each in a single
+// persistent launch.  Each context keeps its data, factors, random streams
+// and look-ahead plans; the batch gathers their kernel arguments into one
+// device table and replays a captured graph of the launch every epoch.
+// ---------------------------------------------------------------------------
+struct dsgdpp_batch {
+    std::vector<dsgdpp_ctx*> ctx;
+    int device = 0, precision = 0, kind = 0;
+    bool trunc = false;
+    size_t argBytes = 0, lds = 0;
+    int nblocks = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint8_t* hArgs = nullptr;    // pinned host staging of the argument table
+    void* dArgs = nullptr;
+    int2* dTmap = nullptr;
+    std::vector<int32_t*> ready;
+    std::vector<int> dsz;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool useGraph = true;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batchCreateError;
+
+void batchDestroy(dsgdpp_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->hArgs) (void)hipHostFree(b->hArgs);
+    void* q = b->dArgs;
+    dfree(q);
+    q = b->dTmap;
+    dfree(q);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t K = b->ctx.size();
+    int64_t updates = 0;
+    for (size_t k = 0; k < K; ++k) {
+        dsgdpp_ctx* c = b->ctx[k];
+        requireReady(c);
+        EpochPlan& P = c->plan[c->cur];
+        if (!P.drawn) drawPlan(c, P);
+        if (!P.built || P.dataVersion != c->dataVersion) {
+            HIPC(hipStreamSynchronize(c->prepStream));
+            buildPlan(c, P, c->prepStream);
+        }
+        if (!P.grouped) throw Error(DSGDPP_E_UNSUPPORTED, "batched epochs need the grouped kernel");
+        HIPC(hipStreamWaitEvent(b->stream, P.evEnd, 0));
+        HIPC(hipEventRecord(c->ev[3], c->stream));   // the context's own earlier work comes first
+        HIPC(hipStreamWaitEvent(b->stream, c->ev[3], 0));
+        EpochLaunch L;
+        L.sched = P.dSched;
+        L.voff = P.dVoff;
+        L.goff = P.dGoff;
+        L.need = P.dNeed;
+        L.ready = c->dReady;
+        L.dataflow = 1;
+        L.eps = eps[k];
+        DevProblem Dp = c->P;
+        Dp.perm = P.perm;
+        GroupedBuffers G = c->G;
+        G.grec = P.grec;
+        G.ngroups = P.ngroups;
+        fill_grouped_args(Dp, c->upd, G, L, b->hArgs + k * b->argBytes);
+        updates += c->blocks[0].nnz;
+    }
+    HIPC(hipMemcpyAsync(b->dArgs, b->hArgs, K * b->argBytes, hipMemcpyHostToDevice, b->stream));
+    HIPC(hipEventRecord(b->ev0, b->stream));
+    if (b->useGraph && !b->exec) {
+        // one graph: the counters' reset and the launch; the table it reads is refreshed above
+        HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
+                           b->ready.data(), b->dsz.data(), (int)K, b->stream);
+        HIPC(hipStreamEndCapture(b->stream, &b->graph));
+        HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    if (b->useGraph) HIPC(hipGraphLaunch(b->exec, b->stream));
+    else launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
+                            b->ready.data(), b->dsz.data(), (int)K, b->stream);
+    HIPC(hipEventRecord(b->ev1, b->stream));
+    for (size_t k = 0; k < K; ++k) {
+        dsgdpp_ctx* c = b->ctx[k];
+        c->lastPlan = c->cur;
+        c->plan[c->cur].drawn = false;
+        c->cur ^= 1;
+        // every context's next plan is prepared while the batch runs
+        HIPC(hipStreamWaitEvent(c->prepStream, b->ev0, 0));
+        if (c->lookahead) {
+            EpochPlan& N = c->plan[c->cur];
+            drawPlan(c, N);
+            buildPlan(c, N, c->prepStream);
+        }
+    }
+    HIPC(hipEventSynchronize(b->ev1));
+    // the contexts' own streams see the batch's writes before their next work
+    for (dsgdpp_ctx* c : b->ctx) HIPC(hipStreamWaitEvent(c->stream, b->ev1, 0));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        memset(st, 0, sizeof(*st));
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->sgd_seconds = ms / 1e3;
+        st->device_seconds = ms / 1e3;
+        st->updates = updates;
+        st->launches = 1;
+        st->kernel = 3;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_graph, dsgdpp_batch** out) {
+    if (!out || !ctxs || count <= 0) {
+        g_batchCreateError = "invalid arguments";
+        return DSGDPP_E_ARG;
+    }
+    *out = nullptr;
+    dsgdpp_batch* b = new dsgdpp_batch();
+    try {
+        for (int k = 0; k < count; ++k) {
+            dsgdpp_ctx* c = ctxs[k];
+            if (!c) throw Error(DSGDPP_E_ARG, "null context");
+            for (int q = 0; q < k; ++q)
+                if (ctxs[q] == c) throw Error(DSGDPP_E_ARG, "a context appears twice");
+            requireReady(c);
+            if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3))
+                throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " cannot use the grouped kernel: " +
+                                                      (c->groupedOk ? std::string("kernel forced") : c->groupedWhy));
+            if (k == 0) {
+                b->device = c->device;
+                b->precision = c->precision;
+                b->kind = c->upd.kind;
+                b->trunc = c->upd.truncate;
+            } else if (c->device != b->device || c->precision != b->precision || c->upd.kind != b->kind ||
+                       c->upd.truncate != b->trunc) {
+                throw Error(DSGDPP_E_UNSUPPORTED, "batched contexts need the same device, precision, update and truncation");
+            }
+            b->ctx.push_back(c);
+        }
+        HIPC(hipSetDevice(b->device));
+        b->useGraph = use_graph != 0;
+        b->argBytes = grouped_args_bytes(b->precision);   // the kernel indexes the table by sizeof
+        std::vector<int2> tmap;
+        for (int k = 0; k < count; ++k) {
+            dsgdpp_ctx* c = b->ctx[k];
+            for (int t = 0; t < (int)c->d; ++t) tmap.push_back(make_int2(k, t));
+            b->lds = std::max(b->lds, batch_epoch_lds(c->G.max_cols, c->P.ld));
+            if (!c->dReady) dalloc(c->dReady, (size_t)2 * c->d * sizeof(int32_t));
+            b->ready.push_back(c->dReady);
+            b->dsz.push_back((int)c->d);
+            HIPC(hipStreamSynchronize(c->stream));
+        }
+        b->nblocks = (int)tmap.size();
+        const int cap = batch_capacity(b->precision, b->kind, b->trunc, b->lds, b->device);
+        if (b->nblocks > cap)
+            throw Error(DSGDPP_E_UNSUPPORTED, "the batch has " + std::to_string(b->nblocks) + " tasks; at most " +
+                                                  std::to_string(cap) + " can be resident at once");
+        HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIPC(hipEventCreate(&b->ev0));
+        HIPC(hipEventCreate(&b->ev1));
+        HIPC(hipHostMalloc((void**)&b->hArgs, (size_t)count * b->argBytes, hipHostMallocDefault));
+        dalloc(b->dArgs, (size_t)count * b->argBytes);
+        dalloc(b->dTmap, tmap.size() * sizeof(int2));
+        HIPC(hipMemcpy(b->dTmap, tmap.data(), tmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+    } catch (const std::exception& e) {
+        g_batchCreateError = e.what();
+        const Error* ee = dynamic_cast<const Error*>(&e);
+        batchDestroy(b);
+        return ee ? ee->code : DSGDPP_E_DEVICE;
+    }
+    *out = b;
+    return DSGDPP_OK;
+}
+
+const char* dsgdpp_batch_create_error(void) { return g_batchCreateError.c_str(); }
+const char* dsgdpp_batch_last_error(dsgdpp_batch* b) { return b ? b->err.c_str() : ""; }
+
+void dsgdpp_batch_destroy(dsgdpp_batch* b) { batchDestroy(b); }
+
+int dsgdpp_batch_epoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* stats) {
+    if (!b || !eps) return DSGDPP_E_ARG;
+    try {
+        HIPC(hipSetDevice(b->device));
+        batchEpoch(b, eps, stats);
+        return DSGDPP_OK;
+    } catch (const Error& e) {
+        b->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        b->err = e.what();
+        return DSGDPP_E_DEVICE;
+    }
 }
 
 }  // extern "C"

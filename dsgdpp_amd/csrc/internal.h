@@ -114,6 +114,15 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
                          hipStream_t s);
 void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                           const EpochLaunch& L, hipStream_t s);
+// batched grouped epoch: one launch over several problems (same precision,
+// update kind and truncation); per-problem arguments live in device memory
+size_t grouped_args_bytes(int precision);
+void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
+                       void* out);
+size_t batch_epoch_lds(int max_cols, int64_t max_ld);
+int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device);
+void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
+                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, hipStream_t s);
 double device_nzsl(const DevProblem& P, int which, double* d_partials, double* h_pinned,
                    hipStream_t s);
 double device_sumsq(const DevProblem& P, int which_factor, const double* d_weights,

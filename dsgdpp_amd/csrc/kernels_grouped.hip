@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <string>
 
 #include "device.h"
@@ -38,10 +39,6 @@
 
 namespace dsgdpp {
 
-constexpr int GPREF = 16;           // W prefetch distance (groups), <= GROUP_RING
-constexpr int GQ = 2 * GPREF;       // record prefetch distance = unroll of the group loop
-static_assert(GPREF <= GROUP_RING, "a prefetched W row may be at most GROUP_RING groups old");
-static_assert(GQ == GROUP_PAD, "group lists are padded to whole unrolled iterations");
 
 // packed record of one slot (uint4):
 //   .x  global row (bits 0-23) | REC_HREG | REC_WFWD | ring entry << 26
@@ -332,12 +329,12 @@ struct GArgs {
     T* W;
     T* H;
     const float* __restrict__ inv2;
+    unsigned long long* prof;   // optional: per task {wait, stage, loop, total} clocks + group count
     uint32_t w_bytes;          // W buffer size (rows m + FACTOR_TAIL_ROWS)
     uint32_t ztail_b;          // byte offset of the zero rows after the last row
     uint32_t dummy_b;          // byte offset of the dummy row
     int d, ld, max_cols;
     float eps, lambda, tmin, tmax;
-    unsigned long long* prof;   // optional: per task {wait, stage, loop, total} clocks + group count
 };
 
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -352,32 +349,53 @@ __device__ __forceinline__ uint4 load_rec(__amdgpu_buffer_rsrc_t r, uint32_t lan
     return make_uint4(u.x, u.y, u.z, u.w);
 }
 
-template <typename T, int KIND, bool TRUNC>
-__global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
-    extern __shared__ float lds[];
-    float* ring = lds;                                  // GROUP_RING x 4 slots x 64
-    float* hblk = lds + GROUP_RING * GROUP_SLOTS * 64;  // (max_cols + 1) x 64; the last row is the dummy column
-    float* hinv = hblk + (A.max_cols + 1) * 64;         // 1/nnz of the staged columns
+// Lane layout: VEC float4s per lane, lane l16 of a slot holds elements
+// [4 VEC l16, 4 VEC (l16 + 1)); VEC = 1 up to rank 64, 2 up to rank 128.
+template <int VEC>
+struct Prefetch {
+    static constexpr int GPREF = VEC == 1 ? 16 : 8;   // W prefetch distance (groups), <= GROUP_RING
+};
+constexpr int GQ = GROUP_PAD;                         // record prefetch distance = unroll of the group loop
+static_assert(Prefetch<1>::GPREF <= GROUP_RING && Prefetch<2>::GPREF <= GROUP_RING,
+              "a prefetched W row may be at most GROUP_RING groups old");
+static_assert(GQ % Prefetch<1>::GPREF == 0 && GQ % Prefetch<2>::GPREF == 0, "W queue cycles with the unroll");
+
+static size_t epoch_lds(int max_cols, int vec) {
+    return ((size_t)(GROUP_RING * GROUP_SLOTS + max_cols + 1) * 64 * vec + max_cols + 1) * sizeof(float);
+}
+
+// One task's visits of one epoch: the body of the grouped epoch kernel.
+template <typename T, int KIND, bool TRUNC, int VEC>
+__device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, float* lds) {
+    constexpr int GPREF = Prefetch<VEC>::GPREF;
+    constexpr int ROW = 64 * VEC;                       // floats per staged row
+    float* ring = lds;                                  // GROUP_RING x 4 slots x ROW
+    float* hblk = lds + GROUP_RING * GROUP_SLOTS * ROW;  // (max_cols + 1) x ROW; the last row is the dummy column
+    float* hinv = hblk + (A.max_cols + 1) * ROW;        // 1/nnz of the staged columns
     const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
     constexpr uint32_t ES = Vec4<T>::BYTES;
+    const int e0 = 4 * VEC * l16;                       // this lane's first element
     // lanes past the row length load zeros from the tail and store into the dummy row
-    const bool pad = 4 * l16 >= A.ld;
+    const bool pad = e0 >= A.ld;
     const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * ES;
-    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(4 * l16 - A.ld) * ES : 4u * l16 * ES;
-    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(4 * l16 - A.ld) * ES : 4u * l16 * ES;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * ES : (uint32_t)e0 * ES;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * ES : (uint32_t)e0 * ES;
     const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
     const uint32_t rec_off = 16u * slot;
-    float* const ring_lane = ring + slot * 64 + 4 * l16;
-    char* const hblk_lane = (char*)(hblk + 4 * l16);   // + column << 8 (64 floats per column)
+    float* const ring_lane = ring + slot * ROW + e0;
+    char* const hblk_lane = (char*)(hblk + e0);         // + column * ROW floats
     const char* const hinv_b = (const char*)hinv;       // + column << 2
+    constexpr int RSH = VEC == 1 ? 8 : 9;               // log2(ROW * 4 bytes)
     const float eps = A.eps;
     const float c1 = eps * -2.f;
     const float f2 = (eps * 2.f) * A.lambda;
-    const int t = blockIdx.x, d = A.d;
-    for (int e = lane; e < GROUP_RING * GROUP_SLOTS * 64; e += 64) ring[e] = 0.f;
+    const int d = A.d;
+    for (int e = lane; e < GROUP_RING * GROUP_SLOTS * ROW; e += 64) ring[e] = 0.f;
     unsigned long long c_wait = 0, c_stage = 0, c_loop = 0, n_groups = 0;
     const unsigned long long c_begin = A.prof ? clock64() : 0;
-    float4 hprev = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 hprev[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) hprev[v] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int s = 0; s < 2 * d; ++s) {
         // per-visit values are uniform: keep them in scalar registers
         const int visit = s * d + t;
@@ -392,83 +410,114 @@ __global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
         const int col0 = uniform_i32(A.off2[b2]);
         const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
         uint4 qa[GQ];
-        float4 qb[GPREF];
+        float4 qb[GPREF][VEC];
+        auto load_w = [&](const uint4& rec, float4* out) {
+            const uint32_t o = __umul24(rec.x, ldb) + lbase;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) out[v] = Vec4<T>::bload(Wr, o + 4 * v * ES);
+        };
         // prologue (W rows belong to this task: no need to wait for the column block)
 #pragma unroll
         for (int u = 0; u < GQ; ++u) qa[u] = load_rec(Gr, rec_off, u, last);
 #pragma unroll
-        for (int u = 0; u < GPREF; ++u) qb[u] = Vec4<T>::bload(Wr, __umul24(qa[u].x, ldb) + lbase);
+        for (int u = 0; u < GPREF; ++u) load_w(qa[u], qb[u]);
         const unsigned long long c0 = A.prof ? clock64() : 0;
         dataflow_wait(A.ready, b2, A.need[visit]);
         const unsigned long long c1c = A.prof ? clock64() : 0;
-        for (int e = lane; e < ncol * 16; e += 64) {
-            const int c = e >> 4, q = e & 15;
-            *(float4*)(hblk + c * 64 + 4 * q) = 4 * q < A.ld ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
-                                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int e = lane; e < ncol * 16 * VEC; e += 64) {
+            const int c = e / (16 * VEC), q = e % (16 * VEC);
+            *(float4*)(hblk + c * ROW + 4 * q) = 4 * q < A.ld ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
+                                                               : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         if (KIND == UPD_NZSL_L2)
             for (int c = lane; c < ncol; c += 64) hinv[c] = A.inv2[col0 + c];
         __syncthreads();
         const unsigned long long c2 = A.prof ? clock64() : 0;
-        float4 hq = *(const float4*)(hblk_lane + (qa[0].y << 8));
+        float4 hq[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) hq[v] = *(const float4*)(hblk_lane + (qa[0].y << RSH) + 16 * v);
         float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
         for (int K0 = 0; K0 < ngp; K0 += GQ) {
 #pragma unroll
             for (int u = 0; u < GQ; ++u) {
                 const int K = K0 + u;
-                {
-                    const uint4 rec = qa[u];
-                    float4 w = qb[u % GPREF];
-                    float4 h = hq;
-                    const float ih = ihq;
-                    // operands of group K+1: no member of group K writes them (see above)
-                    const uint32_t ncolx = qa[(u + 1) % GQ].y;
-                    hq = *(const float4*)(hblk_lane + (ncolx << 8));
-                    if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (ncolx << 2));
-                    if (uniform_i32((int)rec.y) < 0) {   // REC_GFWD of slot 0
-                        const float4 fwd = *(const float4*)(ring + (rec.x >> REC_RING_SHIFT) * 64 + 4 * l16);
-                        const bool wf = (rec.x & REC_WFWD) != 0;
-                        w.x = wf ? fwd.x : w.x; w.y = wf ? fwd.y : w.y;
-                        w.z = wf ? fwd.z : w.z; w.w = wf ? fwd.w : w.w;
+                const uint4 rec = qa[u];
+                float4 w[VEC], h[VEC];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    w[v] = qb[u % GPREF][v];
+                    h[v] = hq[v];
+                }
+                const float ih = ihq;
+                // operands of group K+1: no member of group K writes them (see above)
+                const uint32_t ncolx = qa[(u + 1) % GQ].y;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) hq[v] = *(const float4*)(hblk_lane + (ncolx << RSH) + 16 * v);
+                if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (ncolx << 2));
+                if (uniform_i32((int)rec.y) < 0) {   // REC_GFWD of slot 0
+                    const bool wf = (rec.x & REC_WFWD) != 0;
+                    const float* src = ring + (rec.x >> REC_RING_SHIFT) * ROW + e0;
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) {
+                        const float4 fwd = *(const float4*)(src + 4 * v);
+                        w[v].x = wf ? fwd.x : w[v].x; w[v].y = wf ? fwd.y : w[v].y;
+                        w[v].z = wf ? fwd.z : w[v].z; w[v].w = wf ? fwd.w : w[v].w;
                     }
-                    const bool hr = (rec.x & REC_HREG) != 0;
-                    h.x = hr ? hprev.x : h.x; h.y = hr ? hprev.y : h.y;
-                    h.z = hr ? hprev.z : h.z; h.w = hr ? hprev.w : h.w;
-                    float part = w.x * h.x;
-                    part = fmaf(w.y, h.y, part);
-                    part = fmaf(w.z, h.z, part);
-                    part = fmaf(w.w, h.w, part);
-                    const float wh = row16_sum(part);
-                    const float f1 = c1 * (__uint_as_float(rec.z) - wh);
-                    float a = 0.f, bb = 0.f;
-                    if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = f2 * ih; }
-                    if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
-                    float4 wn, hn;
-                    wn.x = fmaf(-f1, h.x, fmaf(-a, w.x, w.x));
-                    wn.y = fmaf(-f1, h.y, fmaf(-a, w.y, w.y));
-                    wn.z = fmaf(-f1, h.z, fmaf(-a, w.z, w.z));
-                    wn.w = fmaf(-f1, h.w, fmaf(-a, w.w, w.w));
-                    hn.x = fmaf(-f1, w.x, fmaf(-bb, h.x, h.x));
-                    hn.y = fmaf(-f1, w.y, fmaf(-bb, h.y, h.y));
-                    hn.z = fmaf(-f1, w.z, fmaf(-bb, h.z, h.z));
-                    hn.w = fmaf(-f1, w.w, fmaf(-bb, h.w, h.w));
+                }
+                const bool hr = (rec.x & REC_HREG) != 0;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    h[v].x = hr ? hprev[v].x : h[v].x; h[v].y = hr ? hprev[v].y : h[v].y;
+                    h[v].z = hr ? hprev[v].z : h[v].z; h[v].w = hr ? hprev[v].w : h[v].w;
+                }
+                float part = w[0].x * h[0].x;
+                part = fmaf(w[0].y, h[0].y, part);
+                part = fmaf(w[0].z, h[0].z, part);
+                part = fmaf(w[0].w, h[0].w, part);
+#pragma unroll
+                for (int v = 1; v < VEC; ++v) {
+                    part = fmaf(w[v].x, h[v].x, part);
+                    part = fmaf(w[v].y, h[v].y, part);
+                    part = fmaf(w[v].z, h[v].z, part);
+                    part = fmaf(w[v].w, h[v].w, part);
+                }
+                const float wh = row16_sum(part);
+                const float f1 = c1 * (__uint_as_float(rec.z) - wh);
+                float a = 0.f, bb = 0.f;
+                if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = f2 * ih; }
+                if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+                float4 wn[VEC], hn[VEC];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    wn[v].x = fmaf(-f1, h[v].x, fmaf(-a, w[v].x, w[v].x));
+                    wn[v].y = fmaf(-f1, h[v].y, fmaf(-a, w[v].y, w[v].y));
+                    wn[v].z = fmaf(-f1, h[v].z, fmaf(-a, w[v].z, w[v].z));
+                    wn[v].w = fmaf(-f1, h[v].w, fmaf(-a, w[v].w, w[v].w));
+                    hn[v].x = fmaf(-f1, w[v].x, fmaf(-bb, h[v].x, h[v].x));
+                    hn[v].y = fmaf(-f1, w[v].y, fmaf(-bb, h[v].y, h[v].y));
+                    hn[v].z = fmaf(-f1, w[v].z, fmaf(-bb, h[v].z, h[v].z));
+                    hn[v].w = fmaf(-f1, w[v].w, fmaf(-bb, h[v].w, h[v].w));
                     if (TRUNC) {   // update-truncate.h:50-64
-#define TRUNC_(v) v = (v <= A.tmin) ? A.tmin + eps : ((v >= A.tmax) ? A.tmax - eps : v)
-                        TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(wn.z); TRUNC_(wn.w);
-                        TRUNC_(hn.x); TRUNC_(hn.y); TRUNC_(hn.z); TRUNC_(hn.w);
+#define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
+                        TRUNC_(wn[v].x); TRUNC_(wn[v].y); TRUNC_(wn[v].z); TRUNC_(wn[v].w);
+                        TRUNC_(hn[v].x); TRUNC_(hn[v].y); TRUNC_(hn[v].z); TRUNC_(hn[v].w);
 #undef TRUNC_
                     }
-                    wn.x = Vec4<T>::round(wn.x); wn.y = Vec4<T>::round(wn.y);
-                    wn.z = Vec4<T>::round(wn.z); wn.w = Vec4<T>::round(wn.w);
-                    hn.x = Vec4<T>::round(hn.x); hn.y = Vec4<T>::round(hn.y);
-                    hn.z = Vec4<T>::round(hn.z); hn.w = Vec4<T>::round(hn.w);
-                    hprev = hn;
-                    Vec4<T>::bstore(Wr, __umul24(rec.x, ldb) + sbase, wn);
-                    *(float4*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * 64)) = wn;
-                    *(float4*)(hblk_lane + (rec.y << 8)) = hn;
-                    qb[u % GPREF] = Vec4<T>::bload(Wr, __umul24(qa[(u + GPREF) % GQ].x, ldb) + lbase);
-                    qa[u] = load_rec(Gr, rec_off, K + GQ, last);
+                    wn[v].x = Vec4<T>::round(wn[v].x); wn[v].y = Vec4<T>::round(wn[v].y);
+                    wn[v].z = Vec4<T>::round(wn[v].z); wn[v].w = Vec4<T>::round(wn[v].w);
+                    hn[v].x = Vec4<T>::round(hn[v].x); hn[v].y = Vec4<T>::round(hn[v].y);
+                    hn[v].z = Vec4<T>::round(hn[v].z); hn[v].w = Vec4<T>::round(hn[v].w);
+                    hprev[v] = hn[v];
                 }
+                const uint32_t so = __umul24(rec.x, ldb) + sbase;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    Vec4<T>::bstore(Wr, so + 4 * v * ES, wn[v]);
+                    *(float4*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = wn[v];
+                    *(float4*)(hblk_lane + (rec.y << RSH) + 16 * v) = hn[v];
+                }
+                load_w(qa[(u + GPREF) % GQ], qb[u % GPREF]);
+                qa[u] = load_rec(Gr, rec_off, K + GQ, last);
             }
         }
         __syncthreads();
@@ -479,9 +528,9 @@ __global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
             c_loop += c3 - c2;
             n_groups += (unsigned long long)ng;
         }
-        for (int e = lane; e < ncol * 16; e += 64) {
-            const int c = e >> 4, q = e & 15;
-            if (4 * q < A.ld) Vec4<T>::store(A.H + (int64_t)(col0 + c) * A.ld + 4 * q, *(const float4*)(hblk + c * 64 + 4 * q));
+        for (int e = lane; e < ncol * 16 * VEC; e += 64) {
+            const int c = e / (16 * VEC), q = e % (16 * VEC);
+            if (4 * q < A.ld) Vec4<T>::store(A.H + (int64_t)(col0 + c) * A.ld + 4 * q, *(const float4*)(hblk + c * ROW + 4 * q));
         }
         dataflow_signal(A.ready, b2);
     }
@@ -494,12 +543,33 @@ __global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
     }
 }
 
+template <typename T, int KIND, bool TRUNC, int VEC>
+__global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
+    extern __shared__ float lds[];
+    grouped_task<T, KIND, TRUNC, VEC>(A, blockIdx.x, lds);
+}
+
+// Many problems in one persistent launch: workgroup -> (problem, task) by
+// `tmap`; each problem's arguments are read from device memory, so a captured
+// launch replays with new step sizes and plans.
+template <typename T, int KIND, bool TRUNC>
+__global__ __launch_bounds__(64) void k_epoch_grouped_batch(const GArgs<T>* __restrict__ probs,
+                                                            const int2* __restrict__ tmap) {
+    extern __shared__ float lds[];
+    const int p = uniform_i32(tmap[blockIdx.x].x), t = uniform_i32(tmap[blockIdx.x].y);
+    GArgs<T> A;
+    const uint32_t* src = (const uint32_t*)(probs + p);
+    uint32_t* dst = (uint32_t*)&A;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(GArgs<T>) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
+    if (A.ld <= 64) grouped_task<T, KIND, TRUNC, 1>(A, t, lds);
+    else grouped_task<T, KIND, TRUNC, 2>(A, t, lds);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static size_t epoch_lds(int max_cols) {
-    return ((size_t)(GROUP_RING * GROUP_SLOTS + max_cols + 1) * 64 + max_cols + 1) * sizeof(float);
-}
+static int vec_of(int64_t ld) { return ld <= 64 ? 1 : 2; }
 
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols, int device,
                        std::string* why) {
@@ -508,7 +578,8 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
         return false;
     };
     if (P.precision != PREC_FP32 && P.precision != PREC_BF16) return no("grouped kernel runs FP32/BF16 factors");
-    if (P.ld > 64) return no("grouped kernel needs rank <= 64");
+    if (P.ld > 128) return no("grouped kernel needs rank <= 128");
+    if (P.ld > 64 && P.ld % 8 != 0) return no("rows above rank 64 must be padded to 8 elements");
     if (max_rows >= 65536 || max_cols >= 65536) return no("block dimensions above 65535");
     if (max_block >= (1 << 22)) return no("block above 2^22 points");
     const int64_t es = P.precision == PREC_FP32 ? 4 : 2;
@@ -518,13 +589,14 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     int lmax = 0;
     if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return no("no device");
     if (prep_lds(max_block, max_rows, max_cols, max_block < PREP_NARROW) > (size_t)lmax) return no("prep LDS too large");
-    if (epoch_lds(max_cols) > (size_t)lmax) return no("H block does not fit in LDS");
-    const void* fn = (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false>;
+    const size_t lds = epoch_lds(max_cols, vec_of(P.ld));
+    if (lds > (size_t)lmax) return no("H block does not fit in LDS");
+    const void* fn = vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
+                                       : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
     int per_cu = 0, cus = 0;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)epoch_lds(max_cols)) != hipSuccess)
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return no("LDS attribute");
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, epoch_lds(max_cols)) != hipSuccess)
-        return no("occupancy query");
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, lds) != hipSuccess) return no("occupancy query");
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return no("no device");
     if ((int64_t)per_cu * cus < P.d) return no("not all tasks can be resident");
     return true;
@@ -550,10 +622,10 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     HIPK(hipGetLastError());
 }
 
-template <typename T, int KIND, bool TRUNC>
-static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
-                            const EpochLaunch& L, hipStream_t s) {
+template <typename T>
+static GArgs<T> make_gargs(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L) {
     GArgs<T> A;
+    memset(&A, 0, sizeof(A));
     A.grec = (const uint4*)B.grec;
     A.ngroups = B.ngroups;
     A.off2 = B.off2;
@@ -577,8 +649,15 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     A.tmin = (float)U.tmin;
     A.tmax = (float)U.tmax;
     A.prof = B.prof;
-    const size_t bytes = epoch_lds(B.max_cols);
-    const void* fn = (const void*)k_epoch_grouped<T, KIND, TRUNC>;
+    return A;
+}
+
+template <typename T, int KIND, bool TRUNC, int VEC>
+static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                            const EpochLaunch& L, hipStream_t s) {
+    GArgs<T> A = make_gargs<T>(P, U, B, L);
+    const size_t bytes = epoch_lds(B.max_cols, VEC);
+    const void* fn = (const void*)k_epoch_grouped<T, KIND, TRUNC, VEC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
@@ -586,11 +665,18 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(64), args, bytes, s));
 }
 
+template <typename T, int KIND, bool TRUNC>
+static void epoch_grouped_v(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                            const EpochLaunch& L, hipStream_t s) {
+    if (vec_of(P.ld) == 1) epoch_grouped_t<T, KIND, TRUNC, 1>(P, U, B, L, s);
+    else epoch_grouped_t<T, KIND, TRUNC, 2>(P, U, B, L, s);
+}
+
 template <typename T, int KIND>
 static void epoch_grouped_k(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                             const EpochLaunch& L, hipStream_t s) {
-    if (U.truncate) epoch_grouped_t<T, KIND, true>(P, U, B, L, s);
-    else epoch_grouped_t<T, KIND, false>(P, U, B, L, s);
+    if (U.truncate) epoch_grouped_v<T, KIND, true>(P, U, B, L, s);
+    else epoch_grouped_v<T, KIND, false>(P, U, B, L, s);
 }
 
 template <typename T>
@@ -609,6 +695,64 @@ void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const Grouped
     if (P.precision == PREC_FP32) epoch_grouped_p<float>(P, U, B, L, s);
     else if (P.precision == PREC_BF16) epoch_grouped_p<bf16_t>(P, U, B, L, s);
     else throw std::invalid_argument("grouped kernel runs FP32/BF16 factors");
+}
+
+// ---------------------------------------------------------------------------
+// batched epoch
+// ---------------------------------------------------------------------------
+static_assert(sizeof(GArgs<float>) % 8 == 0 && sizeof(GArgs<bf16_t>) % 8 == 0, "argument table stride");
+
+size_t grouped_args_bytes(int precision) {
+    return precision == PREC_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<bf16_t>);
+}
+
+void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
+                       void* out) {
+    if (P.precision == PREC_FP32) {
+        GArgs<float> A = make_gargs<float>(P, U, B, L);
+        A.prof = nullptr;
+        memcpy(out, &A, sizeof(A));
+    } else {
+        GArgs<bf16_t> A = make_gargs<bf16_t>(P, U, B, L);
+        A.prof = nullptr;
+        memcpy(out, &A, sizeof(A));
+    }
+}
+
+size_t batch_epoch_lds(int max_cols, int64_t max_ld) { return epoch_lds(max_cols, vec_of(max_ld)); }
+
+template <typename T, int KIND, bool TRUNC>
+static const void* batch_fn_t() {
+    return (const void*)k_epoch_grouped_batch<T, KIND, TRUNC>;
+}
+
+template <typename T>
+static const void* batch_fn_p(int kind, bool trunc) {
+    switch (kind) {
+        case UPD_NZSL: return trunc ? batch_fn_t<T, UPD_NZSL, true>() : batch_fn_t<T, UPD_NZSL, false>();
+        case UPD_NZSL_L2: return trunc ? batch_fn_t<T, UPD_NZSL_L2, true>() : batch_fn_t<T, UPD_NZSL_L2, false>();
+        case UPD_NZSL_NZL2: return trunc ? batch_fn_t<T, UPD_NZSL_NZL2, true>() : batch_fn_t<T, UPD_NZSL_NZL2, false>();
+        default: throw std::invalid_argument("unknown update kind");
+    }
+}
+
+int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device) {
+    const void* fn = precision == PREC_FP32 ? batch_fn_p<float>(kind, trunc) : batch_fn_p<bf16_t>(kind, trunc);
+    int per_cu = 0, cus = 0;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIPK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, lds));
+    HIPK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    return per_cu * cus;
+}
+
+void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
+                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, hipStream_t s) {
+    // (batch_capacity has set the function's dynamic LDS limit; nothing here but
+    // stream work, so the sequence can be captured into a graph)
+    for (int k = 0; k < nprob; ++k) k_zero_i32<<<1, 256, 0, s>>>(ready[k], 2 * dsz[k]);
+    const void* fn = precision == PREC_FP32 ? batch_fn_p<float>(kind, trunc) : batch_fn_p<bf16_t>(kind, trunc);
+    void* args[] = {const_cast<void**>(&d_args), const_cast<int2**>(&d_tmap)};
+    HIPK(hipLaunchKernel(fn, dim3(nblocks), dim3(64), args, lds, s));
 }
 
 }  // namespace dsgdpp

@@ -98,7 +98,8 @@ def _host(*ts):
 
 def uniform_problem(m, n, nnz_train, nnz_test, rank, truth_rank=None, noise=0.1, init=0.5,
                     seed=1, device="cpu"):
-    """Config-1 family: cells uniform without replacement, x = W*_i.H*_j + N(0, noise^2),
+    """Uniform family (the reference quick-start shape): cells uniform without
+    replacement, x = W*_i.H*_j + N(0, noise^2),
     W*, H* iid N(0,1), W0/H0 iid U(-init, init)."""
     torch = _torch()
     g = _gen(device, seed)
@@ -119,7 +120,7 @@ def uniform_problem(m, n, nnz_train, nnz_test, rank, truth_rank=None, noise=0.1,
 def zipf_problem(m, n, nnz_train, nnz_test, rank, truth_rank, s=1.0, truth_std=0.3, noise=0.8,
                  offset=3.6, scale=1.0, lo=1.0, hi=5.0, round_values=True, init=0.5, seed=2,
                  device="cpu"):
-    """Config-2 family: Zipf(s) degrees capped at n (rows) / m (columns);
+    """Zipf family: Zipf(s) degrees capped at n (rows) / m (columns);
     x = clip(round(offset + scale * W*_i.H*_j + N(0, noise^2)), lo, hi) with
     W*, H* entries N(0, truth_std^2)."""
     torch = _torch()

@@ -164,6 +164,25 @@ int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);
 int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
 
 /* ---------------------------------------------------------------------------
+ * Batched epochs (no counterpart in the reference, whose runner advances one
+ * factorization at a time: dsgdpp_impl.h:282-349).  `count` contexts --
+ * independent problems of any shape, rank <= 128, own lambda -- sharing the
+ * device, precision (FP32 or BF16), update kind and truncation advance one
+ * epoch each in ONE persistent launch of the grouped kernel; every context's
+ * result is bit-identical to dsgdpp_epoch on it alone.  With use_graph the
+ * launch is captured once and replayed each epoch.  The sum of the contexts'
+ * tasks must fit on the device at once (E_UNSUPPORTED otherwise).  The
+ * contexts stay owned by the caller and usable on their own.
+ * --------------------------------------------------------------------------- */
+typedef struct dsgdpp_batch dsgdpp_batch;
+int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_graph, dsgdpp_batch** out);
+const char* dsgdpp_batch_create_error(void);
+const char* dsgdpp_batch_last_error(dsgdpp_batch* b);
+void dsgdpp_batch_destroy(dsgdpp_batch* b);
+/* eps: one step size per context; stats (optional) for the whole batch */
+int dsgdpp_batch_epoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* stats);
+
+/* ---------------------------------------------------------------------------
  * Step-size controller network (no counterpart in the reference: it extends
  * the decay choices of mf/sgd/decay/ with a learned per-epoch step size).
  * A 9 -> 64 -> 64 -> 1 perceptron with GELU hidden units in FP32, evaluated and

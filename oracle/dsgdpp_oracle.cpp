@@ -465,20 +465,19 @@ static inline float bf16Round(float f) {
 
 static inline void updatePointF(const UpdateArgs& a, int mode, double* w, double* h, double xd, double epsd,
                                 double n1, double n2) {
+    // 16 lanes per point; lane l holds elements [E l, E l + E), E = 4 up to
+    // rank 64 and 8 up to rank 128, and sums its products in element order
     const int64_t r = a.r;
-    const int64_t ld = (r + 3) / 4 * 4;
-    float wf[64], hf[64];
-    for (int64_t z = 0; z < 64; ++z) {
+    const int E = r > 64 ? 8 : 4;
+    float wf[128], hf[128];
+    for (int64_t z = 0; z < 16 * E; ++z) {
         wf[z] = z < r ? (float)w[z] : 0.f;
         hf[z] = z < r ? (float)h[z] : 0.f;
     }
-    (void)ld;
     float dl[16];
     for (int l = 0; l < 16; ++l) {
-        float p = wf[4 * l] * hf[4 * l];
-        p = fmaf(wf[4 * l + 1], hf[4 * l + 1], p);
-        p = fmaf(wf[4 * l + 2], hf[4 * l + 2], p);
-        p = fmaf(wf[4 * l + 3], hf[4 * l + 3], p);
+        float p = wf[E * l] * hf[E * l];
+        for (int e = 1; e < E; ++e) p = fmaf(wf[E * l + e], hf[E * l + e], p);
         dl[l] = p;
     }
     float q[4];
@@ -991,7 +990,7 @@ int orc_set_record(orc_ctx* c, int on) { c->record = on != 0; return 0; }
 
 int orc_set_arith(orc_ctx* c, int mode) {
     if (mode < 0 || mode > 2) { setErr(c, "invalid arithmetic mode"); return -1; }
-    if (mode != 0 && c->r > 64) { setErr(c, "float emulation needs rank <= 64"); return -1; }
+    if (mode != 0 && c->r > 128) { setErr(c, "float emulation needs rank <= 128"); return -1; }
     c->arith = mode;
     // values as the GPU stores them: x as float, factors rounded to the storage type
     if (mode != 0) {

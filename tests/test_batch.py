@@ -1,0 +1,113 @@
+"""Batched epochs (one persistent launch over several independent problems)
+and the task-count autotuner.  GPU tests: every problem of a batch ends each
+epoch bit-identical to the same problem run alone, with and without the
+captured graph; the rank-128 lane layout of the grouped kernel matches the
+oracle's float emulation bit for bit."""
+import numpy as np
+import pytest
+
+from dsgdpp_amd import batch as B
+
+
+def test_plan_tasks_balances_predicted_time():
+    rates = {r: {1: 1e7, 2: 1e7, 4: 9e6, 8: 8e6, 16: 6e6, 32: 4e6} for r in B.RANK_CLASSES}
+    probs = [dict(m=10_000, n=5_000, nnz=8_000_000, rank=16),
+             dict(m=10_000, n=5_000, nnz=1_000_000, rank=64),
+             dict(m=40, n=30, nnz=600, rank=128)]
+    d = B.plan_tasks(probs, budget=24, rates=rates)
+    assert sum(d) <= 24
+    assert d[0] > d[1] >= d[2]            # the big problem gets the most tasks
+    assert probs[2]["m"] >= d[2] and probs[2]["n"] >= 2 * d[2]
+    pred = [p["nnz"] / (dk * rates[16][dk]) for p, dk in zip(probs, d)]
+    assert max(pred) == pred[0]
+    assert B.plan_tasks(probs, budget=3, rates=rates) == [1, 1, 1]
+
+
+def test_rank_class():
+    assert [B.rank_class(r) for r in (1, 16, 17, 64, 65, 128)] == [16, 16, 32, 64, 128, 128]
+    with pytest.raises(ValueError):
+        B.rank_class(129)
+
+
+def _ctx(p, tasks, seed, lam, precision="FP32"):
+    from dsgdpp_amd import TRAIN, api
+    upd = f"Nzsl_L2({lam})"
+    g = api.DsgdPp(update=upd, loss=upd, rank=p["rank"], tasks_per_rank=tasks, seed=seed, precision=precision)
+    g.load_coo(TRAIN, p["m"], p["n"], *p["train"])
+    g.set_factors(p["W0"], p["H0"])
+    return g
+
+
+def _problems():
+    from test_gpu_parity import skewed_problem
+    from dsgdpp_amd import data
+    out = [(data.uniform_problem(300, 200, 6000, 0, rank=16, seed=3), 2, 0.05),
+           (skewed_problem(400, 300, 9000, 64, seed=4), 3, 0.1),
+           (data.uniform_problem(250, 250, 5000, 0, rank=96, seed=5), 4, 0.02),
+           (skewed_problem(300, 200, 7000, 128, seed=6), 2, 0.07)]
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_bitexact_vs_individual(use_graph):
+    print(f"case: batch of 4 (ranks 16/64/96/128), graph={use_graph}", flush=True)
+    probs = _problems()
+    batch_ctx = [_ctx(p, d, seed=10 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    solo_ctx = [_ctx(p, d, seed=10 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    bt = B.DsgdPpBatch(batch_ctx, use_graph=use_graph)
+    eps = np.array([0.01, 0.02, 0.005, 0.015])
+    for e in range(3):
+        st = bt.epoch(eps * (1 + 0.1 * e))
+        assert st["launches"] == 1 and st["updates"] == sum(len(p[0]["train"][2]) for p in probs)
+        for k, g in enumerate(solo_ctx):
+            g.epoch(eps[k] * (1 + 0.1 * e))
+        for k in range(len(probs)):
+            Wb, Hb = batch_ctx[k].get_factors()
+            Ws, Hs = solo_ctx[k].get_factors()
+            np.testing.assert_array_equal(Wb, Ws, err_msg=f"problem {k} epoch {e}")
+            np.testing.assert_array_equal(Hb, Hs, err_msg=f"problem {k} epoch {e}")
+    # the contexts stay usable on their own after batched epochs
+    batch_ctx[0].epoch(0.01)
+    solo_ctx[0].epoch(0.01)
+    np.testing.assert_array_equal(batch_ctx[0].get_factors()[0], solo_ctx[0].get_factors()[0])
+    bt.close()
+    for g in batch_ctx + solo_ctx:
+        g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rank,precision", [(96, "FP32"), (128, "FP32"), (128, "BF16"), (65, "FP32")])
+def test_grouped_wide_rank_vs_emulation(rank, precision):
+    """Ranks 65..128 use two float4s per lane (8-element lane partials)."""
+    from test_gpu_parity import make_pair, skewed_problem
+    print(f"case: grouped rank {rank} {precision} vs emulation", flush=True)
+    p = skewed_problem(200, 160, 5000, rank, seed=rank)
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=3, seed=rank + 1, precision=precision, kernel=3)
+    o.set_arith(precision)
+    for e in range(3):
+        st = g.epoch(0.01)
+        assert st["kernel"] == 3
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+
+
+@pytest.mark.gpu
+def test_batch_rejects_mixed_contexts():
+    print("case: batch rejects mixed precision / FP64", flush=True)
+    from dsgdpp_amd import data
+    p = data.uniform_problem(100, 80, 1000, 0, rank=8, seed=1)
+    a = _ctx(p, 2, 1, 0.1, "FP32")
+    b = _ctx(p, 2, 1, 0.1, "BF16")
+    c = _ctx(p, 2, 1, 0.1, "FP64_STRICT")
+    with pytest.raises(RuntimeError, match="same device, precision"):
+        B.DsgdPpBatch([a, b])
+    with pytest.raises(RuntimeError, match="grouped kernel"):
+        B.DsgdPpBatch([c])
+    with pytest.raises(RuntimeError, match="twice"):
+        B.DsgdPpBatch([a, a])
+    for g in (a, b, c):
+        g.close()
