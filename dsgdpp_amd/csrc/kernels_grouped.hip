@@ -47,7 +47,7 @@ namespace dsgdpp {
 //   .z  value x (float bits)
 //   .w  1/nnz of the row (float bits)
 constexpr uint32_t REC_HREG = 1u << 24;   // H_j: the previous group's result in this slot
-constexpr uint32_t REC_WFWD = 1u << 25;   // W_i: ring entry (group & 15) * 4 + slot of its last update
+constexpr uint32_t REC_WFWD = 1u << 25;   // W_i: ring entry (group & 15) * 4 + slot of its last update (bit 25: the blend mask relies on it)
 constexpr int REC_RING_SHIFT = 26;
 constexpr uint32_t REC_GFWD = 1u << 31;   // in .y: LDS offsets (.y << 8, .y << 2) drop this bit
 constexpr uint32_t GM_FWD = 16u;          // group mask bit: some member forwards W_i
@@ -455,13 +455,17 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
                 for (int v = 0; v < VEC; ++v) hq[v] = *(const float4*)(hblk_lane + (ncolx << RSH) + 16 * v);
                 if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (ncolx << 2));
                 if (uniform_i32((int)rec.y) < 0) {   // REC_GFWD of slot 0
-                    const bool wf = (rec.x & REC_WFWD) != 0;
+                    // every lane reads; a bitwise blend keeps the read (and the
+                    // branch) uniform: the forwarding lanes take the ring copy
+                    const uint32_t msk = (uint32_t)((int32_t)(rec.x << (31 - 25)) >> 31);   // REC_WFWD
                     const float* src = ring + (rec.x >> REC_RING_SHIFT) * ROW + e0;
 #pragma unroll
                     for (int v = 0; v < VEC; ++v) {
                         const float4 fwd = *(const float4*)(src + 4 * v);
-                        w[v].x = wf ? fwd.x : w[v].x; w[v].y = wf ? fwd.y : w[v].y;
-                        w[v].z = wf ? fwd.z : w[v].z; w[v].w = wf ? fwd.w : w[v].w;
+                        w[v].x = __uint_as_float((__float_as_uint(fwd.x) & msk) | (__float_as_uint(w[v].x) & ~msk));
+                        w[v].y = __uint_as_float((__float_as_uint(fwd.y) & msk) | (__float_as_uint(w[v].y) & ~msk));
+                        w[v].z = __uint_as_float((__float_as_uint(fwd.z) & msk) | (__float_as_uint(w[v].z) & ~msk));
+                        w[v].w = __uint_as_float((__float_as_uint(fwd.w) & msk) | (__float_as_uint(w[v].w) & ~msk));
                     }
                 }
                 const bool hr = (rec.x & REC_HREG) != 0;
