@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="zipf100m", choices=["zipf100m", "quickstart1m"])
+    ap.add_argument("--workload", default="zipf100m", choices=["zipf100m", "quickstart1m", "batch64"])
     ap.add_argument("--shrink", type=int, default=1)
     ap.add_argument("--precision", default=None)
     ap.add_argument("--kernel", type=int, default=0)
@@ -44,9 +44,22 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--report", choices=["precision", "time-to-rmse"], default=None,
+                    help="instead of the throughput line: FP32/BF16 vs FP64 on identical sequences, or wall "
+                         "time to the plain FP64 run's final test RMSE")
+    ap.add_argument("--epochs", type=int, default=None)
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.report == "precision":
+        report_precision(args)
+        return
+    if args.report == "time-to-rmse":
+        report_time_to_rmse(args)
+        return
+    if args.workload == "batch64":
+        bench_batch(args)
         return
 
     if args.gpus != 1:
@@ -227,6 +240,202 @@ def cpu_baseline(p, update, tasks, threads):
     return {"value": nnz / secs, "unit": "nnz updates/s", "cores": cores, "kind": "port",
             "sample": f"one full FP64 epoch of the same workload ({nnz} updates, d={tasks}), "
                       f"{cores} threads, epoch time {secs:.3f}s"}
+
+
+def _ctx(D, p, update, tasks, precision, seed, kernel=0):
+    g = D.DsgdPp(update=update, loss=update, rank=p["rank"], tasks_per_rank=tasks, sgd_order="WOR",
+                 stratum_order="COWOR", seed=seed, precision=precision, kernel=kernel)
+    g.load_coo(D.TRAIN, p["m"], p["n"], *p["train"])
+    if len(p["test"][2]):
+        g.load_coo(D.TEST, p["m"], p["n"], *p["test"])
+    g.set_factors(p["W0"], p["H0"])
+    return g
+
+
+def report_precision(args):
+    """FP32 and BF16 factors against FP64 on identical stratum and point
+    sequences (same seed) and identical step sizes (the FP64 run's BoldDriver
+    sequence): per-epoch training loss and final test RMSE, relative error."""
+    import torch
+    import dsgdpp_amd as D
+    from dsgdpp_amd import data
+    epochs = args.epochs or 20
+    p = data.quickstart_1m_r128(device="cuda", shrink=args.shrink)
+    torch.cuda.synchronize()
+    update, decay0, tasks = "Nzsl_L2(1)", 0.001, args.tasks or 4
+    runs = {}
+    eps_seq = None
+    for prec in ("FP64_STRICT", "FP32", "BF16"):
+        t0 = time.time()
+        g = _ctx(D, p, update, tasks, prec, seed=12345)
+        losses = [g.loss(D.TRAIN)]
+        if eps_seq is None:
+            bold, prev, eps_seq = D.BoldDriver(decay0), None, []
+            for _ in range(epochs):
+                eps_seq.append(bold(prev, losses[-1]))
+                g.epoch(eps_seq[-1])
+                prev = losses[-1]
+                losses.append(g.loss(D.TRAIN))
+        else:
+            for e in eps_seq:
+                g.epoch(e)
+                losses.append(g.loss(D.TRAIN))
+        rmse = math.sqrt(g.loss(D.TEST) / len(p["test"][2]))
+        runs[prec] = dict(loss=losses, test_rmse=rmse)
+        g.close()
+        log(f"[precision] {prec}: final loss {losses[-1]:.8g}, test rmse {rmse:.6f} ({time.time() - t0:.1f}s)")
+    ref = runs["FP64_STRICT"]
+    out = {"report": "precision", "workload": f"quickstart_1m_r128: {p['m']}x{p['n']}, {len(p['train'][2])} train / "
+                                              f"{len(p['test'][2])} test nonzeros, rank 128, {update}, "
+                                              f"BoldDriver({decay0}) step sequence of the FP64 run, {tasks}x{2 * tasks} "
+                                              f"blocks, {epochs} epochs, same seed",
+           "fp64": {"final_train_loss": ref["loss"][-1], "test_rmse": ref["test_rmse"]}}
+    for prec in ("FP32", "BF16"):
+        r = runs[prec]
+        rel = [abs(a - b) / abs(b) for a, b in zip(r["loss"], ref["loss"])]
+        out[prec.lower()] = {"final_train_loss": r["loss"][-1], "test_rmse": r["test_rmse"],
+                             "max_rel_err_train_loss": max(rel), "rel_err_train_loss_per_epoch": rel,
+                             "rel_err_test_rmse": abs(r["test_rmse"] - ref["test_rmse"]) / ref["test_rmse"]}
+    print(json.dumps(out))
+
+
+def report_time_to_rmse(args):
+    """Wall time to reach the plain FP64 GPU run's final test RMSE (+0.5%):
+    plain DSGD++ (FP64 reference arithmetic, BoldDriver) for the full run,
+    then FP32 factors with BoldDriver and FP32 factors with the learned step
+    controller, each stopping at the target.  Every epoch evaluates the
+    training and test loss, as the reference runner does."""
+    import torch
+    import dsgdpp_amd as D
+    from dsgdpp_amd import controller, data
+    epochs = args.epochs or 30
+    p = data.zipf_100m(device="cuda", shrink=args.shrink)
+    torch.cuda.synchronize()
+    update, decay0, tasks = "Nzsl_L2(0.05)", 0.01, args.tasks or 128
+    nt = len(p["test"][2])
+
+    def run(prec, rule, target=None, max_epochs=epochs):
+        g = _ctx(D, p, update, tasks, prec, seed=12345)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prev, cur = None, g.loss(D.TRAIN)
+        hist = []
+        for e in range(max_epochs):
+            g.epoch(rule(prev, cur))
+            prev, cur = cur, g.loss(D.TRAIN)
+            rmse = math.sqrt(g.loss(D.TEST) / nt)
+            hist.append((time.perf_counter() - t0, rmse))
+            if target is not None and rmse <= target:
+                break
+        g.close()
+        return hist
+
+    h64 = run("FP64_STRICT", D.BoldDriver(decay0))
+    target = h64[-1][1] * 1.005
+    log(f"[time-to-rmse] FP64 plain: {len(h64)} epochs, {h64[-1][0]:.2f}s, test rmse {h64[-1][1]:.5f}; "
+        f"target {target:.5f}")
+    t64 = next(t for t, r in h64 if r <= target)
+    h32 = run("FP32", D.BoldDriver(decay0), target)
+    hl = run("FP32", controller.LearnedDecay(decay0), target)
+    res = {}
+    for name, h in (("fp32_bolddriver", h32), ("fp32_learned", hl)):
+        hit = [t for t, r in h if r <= target]
+        res[name] = {"epochs": len(h), "seconds": hit[0] if hit else None, "final_test_rmse": h[-1][1],
+                     "reached": bool(hit), "speedup_vs_fp64": (t64 / hit[0]) if hit else None}
+        log(f"[time-to-rmse] {name}: {res[name]}")
+    print(json.dumps({"report": "time-to-rmse", "workload": f"zipf100m: {p['m']}x{p['n']}, "
+                      f"{len(p['train'][2])} train / {nt} test nonzeros, rank 50, {update}, start "
+                      f"BoldDriver({decay0}), {tasks}x{2 * tasks} blocks",
+                      "fp64_plain": {"epochs": len(h64), "seconds_total": h64[-1][0], "final_test_rmse": h64[-1][1],
+                                     "seconds_to_target": t64},
+                      "target_test_rmse": target, **res}))
+
+
+def bench_batch(args):
+    """64 independent factorizations (shapes, ranks, lambdas drawn as in
+    dsgdpp_amd.batch.batch_problems), FP32, advanced together by batched
+    epochs (one persistent launch per epoch, graph replay), task counts from
+    the autotuner.  value = all problems' nnz updates per second."""
+    import torch
+    import dsgdpp_amd as D
+    from dsgdpp_amd import batch as B
+    t0 = time.time()
+    probs = B.batch_problems(64, seed=3, shrink=args.shrink)
+    nnz = [len(p["train"][2]) for p in probs]
+    log(f"[bench] generated 64 problems, {sum(nnz)} nonzeros in {time.time() - t0:.1f}s")
+    rates = B.measure_rates(log=log)
+    meta = [dict(m=p["m"], n=p["n"], nnz=len(p["train"][2]), rank=p["rank"]) for p in probs]
+    budget = torch.cuda.get_device_properties(0).multi_processor_count   # one resident task per CU
+    d = B.plan_tasks(meta, budget, rates)
+    parts = B.partition(d, budget)
+    log(f"[bench] tasks per problem: {d} (sum {sum(d)}), {len(parts)} launch(es) per epoch")
+    t0 = time.time()
+    ctxs = []
+    for k, p in enumerate(probs):
+        upd = f"Nzsl_L2({p['lambda']:.6g})"
+        while True:   # the library has the last word on the grouped kernel's limits
+            g = _ctx(D, dict(p, test=((), (), ())), upd, d[k], "FP32", seed=100 + k)
+            ok, why = g.grouped_ok()
+            if ok or 2 * d[k] > p["m"] or 4 * d[k] > p["n"]:
+                break
+            log(f"[bench] problem {k}: d={d[k]} ({why}); doubling")
+            g.close()
+            d[k] *= 2
+        ctxs.append(g)
+    parts = B.partition(d, budget)
+    batches = [B.DsgdPpBatch([ctxs[k] for k in part], use_graph=True) for part in parts]
+    log(f"[bench] contexts loaded + batches created in {time.time() - t0:.1f}s")
+
+    class _All:
+        def epoch(self, eps):
+            out = {"seconds": 0.0, "sgd_seconds": 0.0}
+            for part, bt in zip(parts, batches):
+                st = bt.epoch([eps[k] for k in part])
+                out["seconds"] += st["seconds"]
+                out["sgd_seconds"] += st["sgd_seconds"]
+            return out
+
+        def close(self):
+            for bt in batches:
+                bt.close()
+
+    bt = _All()
+    rules = [D.BoldDriver(0.01) for _ in ctxs]
+    prev = [None] * len(ctxs)
+    cur = [g.loss(D.TRAIN) for g in ctxs]
+
+    def step():
+        eps = [r(a, b) for r, a, b in zip(rules, prev, cur)]
+        st = bt.epoch(eps)
+        for k, g in enumerate(ctxs):
+            prev[k], cur[k] = cur[k], g.loss(D.TRAIN)
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    wall, kern = 0.0, 0.0
+    for k in range(args.steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        st = bt.epoch([r(x, y) for r, x, y in zip(rules, prev, cur)])
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - a
+        kern += st["sgd_seconds"]
+        for q, g in enumerate(ctxs):
+            prev[q], cur[q] = cur[q], g.loss(D.TRAIN)
+        log(f"[bench] batch step {k + 1}/{args.steps}: {st['seconds'] * 1e3:.1f} ms (launch {st['sgd_seconds'] * 1e3:.1f} ms)")
+    total = sum(nnz)
+    print(json.dumps({
+        "metric": "nnz updates/s (64 batched DSGD++ solves, one launch per epoch)",
+        "value": total * args.steps / wall, "unit": "nnz updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic (uniform cells, seed 3)",
+        "config": {"workload": f"batch64: 64 problems, {total} nonzeros in all, ranks "
+                               f"{sorted(set(p['rank'] for p in probs))}, shrink {args.shrink}",
+                   "tasks": d, "launches_per_epoch": len(parts), "launch_ms": kern / args.steps * 1e3}}))
+    bt.close()
+    for g in ctxs:
+        g.close()
 
 
 if __name__ == "__main__":

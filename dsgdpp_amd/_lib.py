@@ -27,7 +27,7 @@ EXPORTS = (
     "dsgdpp_load_coo", "dsgdpp_load_mmc", "dsgdpp_set_factors", "dsgdpp_get_factors",
     "dsgdpp_load_factors_mma", "dsgdpp_write_factors_mma", "dsgdpp_epoch", "dsgdpp_loss",
     "dsgdpp_balance", "dsgdpp_run", "dsgdpp_get_schedule", "dsgdpp_get_task_seeds",
-    "dsgdpp_get_permutation", "dsgdpp_block_nnz", "dsgdpp_dims",
+    "dsgdpp_get_permutation", "dsgdpp_block_nnz", "dsgdpp_dims", "dsgdpp_grouped_ok",
     "dsgdpp_mlp_num_params", "dsgdpp_mlp_create", "dsgdpp_mlp_create_error", "dsgdpp_mlp_last_error",
     "dsgdpp_mlp_destroy", "dsgdpp_mlp_set_params", "dsgdpp_mlp_get_params", "dsgdpp_mlp_forward",
     "dsgdpp_mlp_gradient", "dsgdpp_mlp_train",
@@ -126,6 +126,7 @@ def load():
         "dsgdpp_get_permutation": (i32, [P, i32, i32, pu32, i64, pi64]),
         "dsgdpp_block_nnz": (i32, [P, i32, pi64]),
         "dsgdpp_dims": (i32, [P, pi64]),
+        "dsgdpp_grouped_ok": (i32, [P, pi32, ctypes.c_char_p, i64]),
         "dsgdpp_mlp_num_params": (i32, []),
         "dsgdpp_mlp_create": (i32, [i32, ctypes.POINTER(P)]),
         "dsgdpp_mlp_create_error": (ctypes.c_char_p, []),

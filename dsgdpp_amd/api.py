@@ -191,6 +191,13 @@ class DsgdPp:
         if rc != 0:
             raise DsgdPpError(rc, self._lib.dsgdpp_last_error(self._h).decode())
 
+    def grouped_ok(self):
+        """(True, "") when epochs use the grouped kernel, else (False, reason)."""
+        ok = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(256)
+        self._check(self._lib.dsgdpp_grouped_ok(self._h, ctypes.byref(ok), buf, 256))
+        return bool(ok.value), buf.value.decode()
+
     # data ------------------------------------------------------------------
     def load_coo(self, which, m, n, i, j, x):
         i, j, x = L.as_i32(i), L.as_i32(j), L.as_f64(x)

@@ -75,9 +75,10 @@ def rank_class(r):
 
 
 def _probe_problem(rank, seed=0):
-    """100,000 x 20,000 with 2M uniform nonzeros, the given rank."""
+    """200,000 x 500 with 2M uniform nonzeros, the given rank (narrow enough
+    that every candidate d keeps its H tile in LDS)."""
     from . import data
-    return data.uniform_problem(100_000, 20_000, 2_000_000, 0, rank=rank, noise=0.1, seed=seed)
+    return data.uniform_problem(200_000, 500, 2_000_000, 0, rank=rank, noise=0.1, seed=seed)
 
 
 def measure_rates(precision="FP32", update="Nzsl_L2(0.05)", device=0, cache=CACHE, log=None):
@@ -99,10 +100,15 @@ def measure_rates(precision="FP32", update="Nzsl_L2(0.05)", device=0, cache=CACH
             g = api.DsgdPp(update=update, loss=update, rank=r, tasks_per_rank=d, seed=1, precision=precision,
                            device=device)
             g.load_coo(TRAIN, p["m"], p["n"], *p["train"])
+            if not g.grouped_ok()[0]:   # blocks too large for this d: the planner extrapolates
+                g.close()
+                continue
             g.set_factors(p["W0"], p["H0"])
             g.epoch(1e-4)   # plans and caches warm
             st = g.epoch(1e-4)
             g.close()
+            if st["kernel"] != 3:
+                raise RuntimeError(f"autotune probe rank {r} d {d} did not run the grouped kernel")
             rates[r][d] = len(p["train"][2]) / st["sgd_seconds"] / d
             if log:
                 log(f"autotune probe rank {r} d {d}: {rates[r][d] * d:.3g} updates/s")
@@ -114,16 +120,50 @@ def measure_rates(precision="FP32", update="Nzsl_L2(0.05)", device=0, cache=CACH
     return rates
 
 
+LDS_BYTES = 160 * 1024
+RING_ROWS = 16 * 4   # the kernel's W ring: 16 groups x 4 slots
+
+
+def row_floats(rank):
+    return 64 if rank <= 64 else 128
+
+
+def lds_need(n, d, rank):
+    """LDS of one task (bytes): W ring, the H tile of ceil(n / 2d) columns
+    plus the spare column, and the columns' 1/nnz."""
+    cols = -(-n // (2 * d))
+    return ((RING_ROWS + cols + 1) * row_floats(rank) + cols + 1) * 4
+
+
+def prep_lds_estimate(m, n, nnz, d):
+    """LDS of the prep kernel for blocks of about 1.5 nnz / 2d^2 points (the
+    kernel keeps the visit order and row/column tags, 16-bit below 16,000
+    points, plus one byte per group)."""
+    cap = int(1.5 * nnz / (2 * d * d)) + 1
+    es = 2 if cap < 16000 else 4
+    return (cap + -(-m // d) + -(-n // (2 * d))) * es + cap + 64
+
+
+def min_tasks(p):
+    """Smallest power-of-two d whose H tile and prep state fit in LDS."""
+    d = 1
+    while (lds_need(p["n"], d, p["rank"]) > LDS_BYTES
+           or prep_lds_estimate(p["m"], p["n"], p["nnz"], d) > LDS_BYTES):
+        d *= 2
+    return d
+
+
 def plan_tasks(problems, budget, rates):
     """Tasks per problem.  problems: list of dicts with m, n, nnz, rank.
-    Greedy: start at d = 1; double d of the problem with the largest predicted
-    epoch time nnz / (d * rate(d)) while the total stays within `budget` and
-    the problem has at least d rows and 2d columns per block."""
-    d = [1] * len(problems)
+    Greedy: start at the smallest d whose H tile fits in LDS; double d of the
+    problem with the largest predicted epoch time nnz / (d * rate(d)) while
+    the total stays within `budget` and the problem has at least d rows and
+    2d columns per block."""
+    d = [min_tasks(p) for p in problems]
 
     def pred(k, dk):
         r = rates[rank_class(problems[k]["rank"])]
-        rate = r.get(dk) or r[max(x for x in r if x <= dk)]
+        rate = (r.get(dk) or r[max(x for x in r if x <= dk)]) if dk >= min(r) else r[min(r)]
         return problems[k]["nnz"] / (dk * rate)
 
     while True:
@@ -132,7 +172,7 @@ def plan_tasks(problems, budget, rates):
         for k in order:
             nd = 2 * d[k]
             p = problems[k]
-            if nd > max(CANDIDATE_TASKS) or sum(d) + d[k] > budget or p["m"] < nd or p["n"] < 2 * nd:
+            if nd > max(max(CANDIDATE_TASKS), d[k]) or sum(d) + d[k] > budget or p["m"] < nd or p["n"] < 2 * nd:
                 continue
             if pred(k, nd) >= pred(k, d[k]):
                 continue
@@ -141,6 +181,23 @@ def plan_tasks(problems, budget, rates):
             break
         if not grown:
             return d
+
+
+def partition(tasks, budget):
+    """Problem indices split into launches of at most `budget` tasks each
+    (first fit, largest first); a problem larger than the budget runs alone."""
+    order = sorted(range(len(tasks)), key=lambda k: -tasks[k])
+    parts, load = [], []
+    for k in order:
+        for q in range(len(parts)):
+            if load[q] + tasks[k] <= budget:
+                parts[q].append(k)
+                load[q] += tasks[k]
+                break
+        else:
+            parts.append([k])
+            load.append(tasks[k])
+    return [sorted(p) for p in parts]
 
 
 def batch_problems(count=64, seed=0, shrink=1, max_nnz=20_000_000):

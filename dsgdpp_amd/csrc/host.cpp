@@ -1300,6 +1300,20 @@ int dsgdpp_write_factors_mma(dsgdpp_ctx* c, const char* wpath, const char* hpath
     });
 }
 
+int dsgdpp_grouped_ok(dsgdpp_ctx* c, int32_t* ok, char* why, int64_t cap) {
+    return guarded(c, [&] {
+        if (!ok) throw Error(DSGDPP_E_ARG, "null output");
+        if (!c->blocks[0].loaded) throw Error(DSGDPP_E_STATE, "load the training matrix first");
+        *ok = c->groupedOk ? 1 : 0;
+        if (why && cap > 0) {
+            const std::string w = c->groupedOk ? std::string() : c->groupedWhy;
+            const size_t k = std::min<size_t>(w.size(), (size_t)cap - 1);
+            memcpy(why, w.data(), k);
+            why[k] = 0;
+        }
+    });
+}
+
 int dsgdpp_epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* stats) {
     return guarded(c, [&] { epoch(c, eps, stats); });
 }
@@ -1463,6 +1477,7 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         fill_grouped_args(Dp, c->upd, G, L, b->hArgs + k * b->argBytes);
         updates += c->blocks[0].nnz;
     }
+    const auto t_prep = std::chrono::steady_clock::now();
     HIPC(hipMemcpyAsync(b->dArgs, b->hArgs, K * b->argBytes, hipMemcpyHostToDevice, b->stream));
     HIPC(hipEventRecord(b->ev0, b->stream));
     if (b->useGraph && !b->exec) {
@@ -1477,6 +1492,7 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     else launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
                             b->ready.data(), b->dsz.data(), (int)K, b->stream);
     HIPC(hipEventRecord(b->ev1, b->stream));
+    const auto t_launch = std::chrono::steady_clock::now();
     for (size_t k = 0; k < K; ++k) {
         dsgdpp_ctx* c = b->ctx[k];
         c->lastPlan = c->cur;
@@ -1490,10 +1506,16 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
             buildPlan(c, N, c->prepStream);
         }
     }
+    const auto t_ahead = std::chrono::steady_clock::now();
     HIPC(hipEventSynchronize(b->ev1));
     // the contexts' own streams see the batch's writes before their next work
     for (dsgdpp_ctx* c : b->ctx) HIPC(hipStreamWaitEvent(c->stream, b->ev1, 0));
     const auto t1 = std::chrono::steady_clock::now();
+    if (b->ctx[0]->kprof) {
+        auto ms = [](auto a, auto z) { return std::chrono::duration<double, std::milli>(z - a).count(); };
+        fprintf(stderr, "[kprof] batch host ms: arguments %.2f launch %.2f look-ahead plans %.2f wait %.2f\n",
+                ms(t0, t_prep), ms(t_prep, t_launch), ms(t_launch, t_ahead), ms(t_ahead, t1));
+    }
     if (st) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, b->ev0, b->ev1));

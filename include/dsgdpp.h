@@ -159,6 +159,10 @@ int dsgdpp_get_schedule(dsgdpp_ctx* ctx, uint32_t* out);
 int dsgdpp_get_task_seeds(dsgdpp_ctx* ctx, uint64_t* out);
 int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint32_t* out,
                            int64_t cap, int64_t* n);
+/* Whether epochs of this context can use the grouped kernel (FP32/BF16,
+ * rank <= 128, H tile and block within LDS); `why` (optional) gets the reason
+ * when not.  No counterpart in the reference (a kernel-choice query). */
+int dsgdpp_grouped_ok(dsgdpp_ctx* ctx, int32_t* ok, char* why, int64_t cap);
 /* nnz per block (d x 2d, row-major) and dims {m, n, r, d, nnz_train, nnz_test} */
 int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);
 int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);

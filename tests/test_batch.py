@@ -14,13 +14,21 @@ def test_plan_tasks_balances_predicted_time():
     probs = [dict(m=10_000, n=5_000, nnz=8_000_000, rank=16),
              dict(m=10_000, n=5_000, nnz=1_000_000, rank=64),
              dict(m=40, n=30, nnz=600, rank=128)]
-    d = B.plan_tasks(probs, budget=24, rates=rates)
-    assert sum(d) <= 24
-    assert d[0] > d[1] >= d[2]            # the big problem gets the most tasks
-    assert probs[2]["m"] >= d[2] and probs[2]["n"] >= 2 * d[2]
+    mins = [B.min_tasks(p) for p in probs]
+    # 5000 columns need 8 tasks for the H tile; 8M points need 16 for the prep state
+    assert mins == [16, 8, 1]
+    assert all(B.lds_need(p["n"], m, p["rank"]) <= B.LDS_BYTES for p, m in zip(probs, mins))
+    assert B.prep_lds_estimate(10_000, 5_000, 8_000_000, 8) > B.LDS_BYTES
+    d = B.plan_tasks(probs, budget=60, rates=rates)
+    assert sum(d) <= 60 and all(a >= b for a, b in zip(d, mins))
+    assert d[0] >= d[1] >= d[2] and d[0] > mins[0]   # the big problem grows first
     pred = [p["nnz"] / (dk * rates[16][dk]) for p, dk in zip(probs, d)]
     assert max(pred) == pred[0]
-    assert B.plan_tasks(probs, budget=3, rates=rates) == [1, 1, 1]
+    assert probs[2]["m"] >= d[2] and probs[2]["n"] >= 2 * d[2]
+    assert B.plan_tasks(probs, budget=3, rates=rates) == mins   # never below the LDS minimum
+    parts = B.partition([8, 8, 1, 20, 5], budget=21)
+    assert sorted(sum(parts, [])) == [0, 1, 2, 3, 4]
+    assert all(sum([8, 8, 1, 20, 5][k] for k in q) <= 21 for q in parts)
 
 
 def test_rank_class():
