@@ -92,8 +92,12 @@ __device__ __forceinline__ C wave_sum(C v) {
 
 __device__ __forceinline__ void dataflow_wait(int32_t* ready, int b2, int32_t need) {
     if (need <= 0) return;
-    while (__hip_atomic_load(ready + b2, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+    // Poll relaxed, then acquire once: an agent-scope acquire invalidates the
+    // XCD's L2, and doing that on every poll would evict the rows that the
+    // XCD's other tasks are reusing.
+    while (__hip_atomic_load(ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
         __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 __device__ __forceinline__ void dataflow_signal(int32_t* ready, int b2) {

@@ -191,12 +191,18 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 }
                 const uint32_t hreg = g == gH ? 1u : 0u;
                 const uint32_t sl = (uint32_t)__builtin_ctz(~em & 15u);
-                if (g > maxG) {
+                if (g > maxG) {   // g <= maxG + 1: the lane of maxG + 1 is clear
                     maxG = g;
-                    lo = max(g - 62, 2);
+                    lo = max(lo, g - 62);
                     win = lane == ((g + 1) & 63) ? 0u : win;
                 }
                 win = writelane_u32(mask | (1u << sl), g & 63, win);
+                // lo: the first group of the window with a free slot.  Every group
+                // below it is full, so starting the search there places each
+                // point where the plain scan from maxG - 62 would.  The clear lane
+                // of maxG + 1 ends the advance.
+                if (g == lo)
+                    while ((readlane_u32(win, lo & 63) & 15u) == 15u) ++lo;
                 const uint32_t tag = ((uint32_t)g << 2) | sl;
                 myR = r == R ? tag : myR;
                 myC = c == C ? tag : myC;
@@ -344,8 +350,12 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ uint4 load_rec(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, int K, int last) {
-    const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)lane_off, min(K, last) * 64, 0);
+// Records of group K sit at K * 64 bytes (+ 16 per slot).  The offset goes in
+// the vector offset, which the buffer range check covers: the loop's reads of
+// up to GQ groups past the visit return zeros (row 0, column 0, no flags),
+// used only as prefetch operands and never processed.
+__device__ __forceinline__ uint4 load_rec(__amdgpu_buffer_rsrc_t r, int off) {
+    const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     return make_uint4(u.x, u.y, u.z, u.w);
 }
 
@@ -404,7 +414,6 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
         if (A.blk[b + 1] == A.blk[b]) continue;
         const int ng = uniform_i32(A.ngroups[visit]);
         const int ngp = (ng + GQ - 1) / GQ * GQ;   // the prep kernel pads with empty groups
-        const int last = ngp - 1;
         const int64_t gbase = uniform_i64(A.goff[visit]);
         const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
         const int col0 = uniform_i32(A.off2[b2]);
@@ -418,7 +427,7 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
         };
         // prologue (W rows belong to this task: no need to wait for the column block)
 #pragma unroll
-        for (int u = 0; u < GQ; ++u) qa[u] = load_rec(Gr, rec_off, u, last);
+        for (int u = 0; u < GQ; ++u) qa[u] = load_rec(Gr, (int)rec_off + 64 * u);
 #pragma unroll
         for (int u = 0; u < GPREF; ++u) load_w(qa[u], qb[u]);
         const unsigned long long c0 = A.prof ? clock64() : 0;
@@ -429,8 +438,8 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
             *(float4*)(hblk + c * ROW + 4 * q) = 4 * q < A.ld ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
                                                                : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        if (KIND == UPD_NZSL_L2)
-            for (int c = lane; c < ncol; c += 64) hinv[c] = A.inv2[col0 + c];
+        if (KIND == UPD_NZSL_L2)   // the column's factor f2 / nnz_j, the same product the update forms
+            for (int c = lane; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
         __syncthreads();
         const unsigned long long c2 = A.prof ? clock64() : 0;
         float4 hq[VEC];
@@ -438,6 +447,7 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
         for (int v = 0; v < VEC; ++v) hq[v] = *(const float4*)(hblk_lane + (qa[0].y << RSH) + 16 * v);
         float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
         for (int K0 = 0; K0 < ngp; K0 += GQ) {
+            const int rbase = (int)rec_off + 64 * (K0 + GQ);   // < 2^31: a visit has < 2^22 groups
 #pragma unroll
             for (int u = 0; u < GQ; ++u) {
                 const int K = K0 + u;
@@ -488,7 +498,7 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
                 const float wh = row16_sum(part);
                 const float f1 = c1 * (__uint_as_float(rec.z) - wh);
                 float a = 0.f, bb = 0.f;
-                if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = f2 * ih; }
+                if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = ih; }
                 if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
                 float4 wn[VEC], hn[VEC];
 #pragma unroll
@@ -521,7 +531,7 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
                     *(float4*)(hblk_lane + (rec.y << RSH) + 16 * v) = hn[v];
                 }
                 load_w(qa[(u + GPREF) % GQ], qb[u % GPREF]);
-                qa[u] = load_rec(Gr, rec_off, K + GQ, last);
+                qa[u] = load_rec(Gr, rbase + 64 * u);
             }
         }
         __syncthreads();
@@ -660,7 +670,8 @@ template <typename T, int KIND, bool TRUNC, int VEC>
 static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                             const EpochLaunch& L, hipStream_t s) {
     GArgs<T> A = make_gargs<T>(P, U, B, L);
-    const size_t bytes = epoch_lds(B.max_cols, VEC);
+    size_t bytes = epoch_lds(B.max_cols, VEC);
+    if (const char* e = getenv("DSGDPP_EPOCH_LDS")) bytes = std::max(bytes, (size_t)atol(e));
     const void* fn = (const void*)k_epoch_grouped<T, KIND, TRUNC, VEC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
