@@ -137,11 +137,11 @@ def lds_need(n, d, rank):
 
 def prep_lds_estimate(m, n, nnz, d):
     """LDS of the prep kernel for blocks of about 1.5 nnz / 2d^2 points (the
-    kernel keeps the visit order and row/column tags, 16-bit below 16,000
-    points, plus one byte per group)."""
+    kernel keeps the visit order and row/column tags, 16-bit below 8,000
+    points, plus one byte per group, fewer than two groups per point)."""
     cap = int(1.5 * nnz / (2 * d * d)) + 1
-    es = 2 if cap < 16000 else 4
-    return (cap + -(-m // d) + -(-n // (2 * d))) * es + cap + 64
+    es = 2 if cap < 8000 else 4
+    return (cap + -(-m // d) + -(-n // (2 * d))) * es + 2 * cap + 64
 
 
 def min_tasks(p):

@@ -621,7 +621,7 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
             P.voff[(size_t)sub * d + id] = total;
             P.goff[(size_t)sub * d + id] = gtotal;
             total += nb;
-            gtotal += (nb + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;   // groups <= points, padded
+            gtotal += (2 * nb + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;   // groups < 2 points, padded
         }
     P.grouped = c->groupedOk && (c->kernel == 0 || c->kernel == 3);
     P.dataflow = P.grouped || (c->kernel != 1 && c->dataflowOk);

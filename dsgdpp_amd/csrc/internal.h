@@ -91,7 +91,7 @@ bool dataflow_fits(const DevProblem& P, int device);
 // (packed 16-byte records in (group, slot) order); the epoch kernel runs one
 // group per wave step (16 lanes per point, H block staged in LDS).
 constexpr int GROUP_SLOTS = 4;
-constexpr int GROUP_RING = 16;    // W forwarding window, in groups
+constexpr int GROUP_RING = 16;    // W forwarding window, in groups (power of two)
 constexpr int GROUP_PAD = 32;     // every visit's group list is padded to a multiple of this
 constexpr int64_t PREP_SPLIT = 4096;   // grouped prep: blocks up to this size run in a launch of their own
 // Factor buffers carry FACTOR_TAIL_ROWS spare rows after the last one: the

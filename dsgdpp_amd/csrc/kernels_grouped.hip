@@ -41,16 +41,14 @@ namespace dsgdpp {
 
 
 // packed record of one slot (uint4):
-//   .x  global row (bits 0-23) | REC_HREG | REC_WFWD | ring entry << 26
-//   .y  column index inside the column block; bit 31 of slot 0's record: some
-//       member of the group takes W_i from the ring (REC_GFWD)
+//   .x  global row (bits 0-23) | REC_WFWD | ring entry << REC_RING_SHIFT
+//   .y  column index inside the column block | REC_HREG
 //   .z  value x (float bits)
 //   .w  1/nnz of the row (float bits)
-constexpr uint32_t REC_HREG = 1u << 24;   // H_j: the previous group's result in this slot
-constexpr uint32_t REC_WFWD = 1u << 25;   // W_i: ring entry (group & 15) * 4 + slot of its last update (bit 25: the blend mask relies on it)
-constexpr int REC_RING_SHIFT = 26;
-constexpr uint32_t REC_GFWD = 1u << 31;   // in .y: LDS offsets (.y << 8, .y << 2) drop this bit
-constexpr uint32_t GM_FWD = 16u;          // group mask bit: some member forwards W_i
+constexpr uint32_t REC_WFWD = 1u << 24;   // W_i: ring entry (group % GROUP_RING) * 4 + slot of its last update (bit 24: the blend mask relies on it)
+constexpr int REC_RING_SHIFT = 25;        // 7 bits: GROUP_RING * GROUP_SLOTS <= 128 entries
+constexpr uint32_t REC_HREG = 1u << 30;   // in .y: H_j is the previous group's result in this slot (LDS offsets .y << 8, .y << 2 drop it)
+static_assert(GROUP_RING * GROUP_SLOTS <= (1 << (32 - REC_RING_SHIFT)), "ring entry field");
 
 __global__ void k_zero_i32(int32_t* p, int n) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
@@ -67,11 +65,11 @@ __device__ __forceinline__ uint32_t writelane_u32(uint32_t v, int lane, uint32_t
 // LDS of the prep kernel.  Narrow mode (blocks below PREP_NARROW points) keeps
 // the visit order and the row/column tags in 16 bits, which fits more
 // workgroups per CU.
-constexpr int64_t PREP_NARROW = 16000;   // tag ((group + 2) << 2 | slot) < 2^16
+constexpr int64_t PREP_NARROW = 8000;    // tag ((group + 2) << 2 | slot) < 2^16 with group < 2 n
 static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow) {
     const size_t es = narrow ? 2 : 4;
     auto up4 = [](size_t b) { return (b + 3) & ~(size_t)3; };
-    return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + up4((size_t)cap + GROUP_PAD);
+    return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + up4(2 * (size_t)cap + GROUP_PAD);
 }
 
 template <typename IX>
@@ -139,7 +137,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
             }
         }
         const unsigned long long t1 = pprof ? clock64() : 0;
-        const int gwords = (n + GROUP_PAD + 3) >> 2;
+        const int gwords = (2 * n + GROUP_PAD + 3) >> 2;   // a visit has fewer than 2 n groups
         for (int k = lane; k < rows_cap; k += 64) lastR[k] = 0;
         for (int k = lane; k < cols_cap; k += 64) lastC[k] = 0;
         for (int k = lane; k < gwords; k += 64) gmw[k] = 0;
@@ -150,9 +148,10 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
         //    and "the group after the column's last one" is never a real group
         //    for an untouched column.  Row and column tags sit in the chunk's
         //    lanes; every placement updates all lanes that share the row or
-        //    column.  The slot masks of groups maxG-62 .. maxG sit in lanes (group
-        //    G in lane G % 64) and the lane of maxG+1 is kept clear, so the search
-        //    ends there at the latest.
+        //    column.  The slot masks of groups maxG-61 .. maxG sit in lanes (group
+        //    G in lane G % 64) and the lanes of maxG+1 and maxG+2 are kept clear:
+        //    a point starts at most two groups past maxG, and its search ends at
+        //    the first clear lane.
         int maxG = 1, lo = 2;
         uint32_t win = 0;
         // the next chunk's points are fetched while the current chunk is scanned
@@ -181,7 +180,9 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 const uint32_t LR = readlane_u32(myR, l), LC = readlane_u32(myC, l);
                 const int gH = (int)(LC >> 2) + 1;                 // H_j from registers in this group
                 const uint32_t only = 15u ^ (1u << (LC & 3u));     // the other slots
-                int g = max(max((int)(LR >> 2), (int)(LC >> 2)) + 1, lo);
+                // two groups after the row's last update: the epoch kernel reads a
+                // forwarded W row from the ring one group ahead
+                int g = max(max((int)(LR >> 2) + 2, (int)(LC >> 2) + 1), lo);
                 uint32_t mask, em;
                 for (;;) {
                     mask = readlane_u32(win, g & 63);
@@ -191,15 +192,15 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 }
                 const uint32_t hreg = g == gH ? 1u : 0u;
                 const uint32_t sl = (uint32_t)__builtin_ctz(~em & 15u);
-                if (g > maxG) {   // g <= maxG + 1: the lane of maxG + 1 is clear
+                if (g > maxG) {   // g <= maxG + 2
                     maxG = g;
-                    lo = max(lo, g - 62);
-                    win = lane == ((g + 1) & 63) ? 0u : win;
+                    lo = max(lo, g - 61);
+                    win = (lane == ((g + 1) & 63) || lane == ((g + 2) & 63)) ? 0u : win;
                 }
                 win = writelane_u32(mask | (1u << sl), g & 63, win);
                 // lo: the first group of the window with a free slot.  Every group
                 // below it is full, so starting the search there places each
-                // point where the plain scan from maxG - 62 would.  The clear lane
+                // point where the plain scan from maxG - 61 would.  The clear lane
                 // of maxG + 1 ends the advance.
                 if (g == lo)
                     while ((readlane_u32(win, lo & 63) & 15u) == 15u) ++lo;
@@ -217,11 +218,11 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                 // W_i from the LDS ring when the row was updated in the last GROUP_RING groups
                 const int gR = (int)(prevR >> 2) - 2;
                 const bool wf = prevR != 0u && (int)g - gR < GROUP_RING;
-                atomicOr(gmw + (g >> 2), ((1u << sl) | (wf ? GM_FWD : 0u)) << (8 * (g & 3u)));
+                atomicOr(gmw + (g >> 2), (1u << sl) << (8 * (g & 3u)));
                 uint4 o;
-                o.x = (uint32_t)e.i | (((res >> 2) & 1u) << 24) |
-                      (wf ? (REC_WFWD | (((((uint32_t)gR & 15u) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
-                o.y = c;
+                o.x = (uint32_t)e.i |
+                      (wf ? (REC_WFWD | (((((uint32_t)gR & (GROUP_RING - 1)) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
+                o.y = c | (((res >> 2) & 1u) ? REC_HREG : 0u);
                 o.z = __float_as_uint(e.x);
                 o.w = e.aux;
                 grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
@@ -232,16 +233,12 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
         const unsigned long long t2 = pprof ? clock64() : 0;
         const int ng = maxg + 1;
         const int ngp = (ng + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;
-        // 3. empty slots (and the padding groups) get the dummy record; slot 0
-        //    carries the group's W forwarding flag
+        // 3. empty slots (and the padding groups) get the dummy record
         for (int g = lane; g < ngp; g += 64) {
             const uint32_t mk = (gmw[g >> 2] >> (8 * (g & 3))) & 0xffu;
             uint4* gp = grec + ((uint64_t)J.gbase + g) * GROUP_SLOTS;
-            const uint32_t gf = (mk & GM_FWD) ? REC_GFWD : 0u;
             for (int sl = 0; sl < GROUP_SLOTS; ++sl)
-                if (!((mk >> sl) & 1u)) gp[sl] = make_uint4(dummy_row, dummy_col | (sl == 0 ? gf : 0u), 0u, 0u);
-            if ((mk & 1u) && gf)   // slot 0 holds a point: set the flag in place (L2, after the barrier)
-                __hip_atomic_fetch_or(&gp[0].y, gf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (!((mk >> sl) & 1u)) gp[sl] = make_uint4(dummy_row, dummy_col, 0u, 0u);
         }
         if (lane == 0) ngroups[J.visit] = ng;
         __syncthreads();
@@ -442,9 +439,12 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
             for (int c = lane; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
         __syncthreads();
         const unsigned long long c2 = A.prof ? clock64() : 0;
-        float4 hq[VEC];
+        float4 hq[VEC], wq[VEC];
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) hq[v] = *(const float4*)(hblk_lane + (qa[0].y << RSH) + 16 * v);
+        for (int v = 0; v < VEC; ++v) {
+            hq[v] = *(const float4*)(hblk_lane + (qa[0].y << RSH) + 16 * v);
+            wq[v] = *(const float4*)(ring + (qa[0].x >> REC_RING_SHIFT) * ROW + e0 + 4 * v);
+        }
         float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
         for (int K0 = 0; K0 < ngp; K0 += GQ) {
             const int rbase = (int)rec_off + 64 * (K0 + GQ);   // < 2^31: a visit has < 2^22 groups
@@ -452,33 +452,37 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
             for (int u = 0; u < GQ; ++u) {
                 const int K = K0 + u;
                 const uint4 rec = qa[u];
-                float4 w[VEC], h[VEC];
+                float4 w[VEC], h[VEC], fwd[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     w[v] = qb[u % GPREF][v];
                     h[v] = hq[v];
+                    fwd[v] = wq[v];
                 }
                 const float ih = ihq;
-                // operands of group K+1: no member of group K writes them (see above)
-                const uint32_t ncolx = qa[(u + 1) % GQ].y;
+                // operands of group K+1, read before group K writes: its H columns
+                // are not written by group K (same-slot columns come from
+                // registers), and its ring entries were written by groups <= K-1
+                // (a row's points are at least two groups apart)
+                const uint4 nrec = qa[(u + 1) % GQ];
+                const float* const nsrc = ring + (nrec.x >> REC_RING_SHIFT) * ROW + e0;
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) hq[v] = *(const float4*)(hblk_lane + (ncolx << RSH) + 16 * v);
-                if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (ncolx << 2));
-                if (uniform_i32((int)rec.y) < 0) {   // REC_GFWD of slot 0
-                    // every lane reads; a bitwise blend keeps the read (and the
-                    // branch) uniform: the forwarding lanes take the ring copy
-                    const uint32_t msk = (uint32_t)((int32_t)(rec.x << (31 - 25)) >> 31);   // REC_WFWD
-                    const float* src = ring + (rec.x >> REC_RING_SHIFT) * ROW + e0;
+                for (int v = 0; v < VEC; ++v) {
+                    hq[v] = *(const float4*)(hblk_lane + (nrec.y << RSH) + 16 * v);
+                    wq[v] = *(const float4*)(nsrc + 4 * v);
+                }
+                if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (nrec.y << 2));
+                {   // W_i from the ring where the prefetch may predate its last update
+                    const uint32_t msk = (uint32_t)((int32_t)(rec.x << (31 - 24)) >> 31);   // REC_WFWD
 #pragma unroll
                     for (int v = 0; v < VEC; ++v) {
-                        const float4 fwd = *(const float4*)(src + 4 * v);
-                        w[v].x = __uint_as_float((__float_as_uint(fwd.x) & msk) | (__float_as_uint(w[v].x) & ~msk));
-                        w[v].y = __uint_as_float((__float_as_uint(fwd.y) & msk) | (__float_as_uint(w[v].y) & ~msk));
-                        w[v].z = __uint_as_float((__float_as_uint(fwd.z) & msk) | (__float_as_uint(w[v].z) & ~msk));
-                        w[v].w = __uint_as_float((__float_as_uint(fwd.w) & msk) | (__float_as_uint(w[v].w) & ~msk));
+                        w[v].x = __uint_as_float((__float_as_uint(fwd[v].x) & msk) | (__float_as_uint(w[v].x) & ~msk));
+                        w[v].y = __uint_as_float((__float_as_uint(fwd[v].y) & msk) | (__float_as_uint(w[v].y) & ~msk));
+                        w[v].z = __uint_as_float((__float_as_uint(fwd[v].z) & msk) | (__float_as_uint(w[v].z) & ~msk));
+                        w[v].w = __uint_as_float((__float_as_uint(fwd[v].w) & msk) | (__float_as_uint(w[v].w) & ~msk));
                     }
                 }
-                const bool hr = (rec.x & REC_HREG) != 0;
+                const bool hr = (rec.y & REC_HREG) != 0;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     h[v].x = hr ? hprev[v].x : h[v].x; h[v].y = hr ? hprev[v].y : h[v].y;
