@@ -129,10 +129,11 @@ def row_floats(rank):
 
 
 def lds_need(n, d, rank):
-    """LDS of one task (bytes): W ring, the H tile of ceil(n / 2d) columns
-    plus the spare column, and the columns' 1/nnz."""
+    """LDS of one task (bytes): W ring, two H tiles of ceil(n / 2d) columns
+    plus the spare column and the columns' factors, two flags."""
     cols = -(-n // (2 * d))
-    return ((RING_ROWS + cols + 1) * row_floats(rank) + cols + 1) * 4
+    tile = ((cols + 1) * row_floats(rank) + cols + 1 + 3) // 4 * 4
+    return (RING_ROWS * row_floats(rank) + 2 * tile) * 4 + 8
 
 
 def prep_lds_estimate(m, n, nnz, d):

@@ -100,6 +100,12 @@ __device__ __forceinline__ void dataflow_wait(int32_t* ready, int b2, int32_t ne
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// The same from a single wave (its own stores; no workgroup barrier).
+__device__ __forceinline__ void dataflow_signal_wave(int32_t* ready, int b2) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(ready + b2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void dataflow_signal(int32_t* ready, int b2) {
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_fetch_add(ready + b2, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

@@ -840,7 +840,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
             mw += h[5 * t]; ms += h[5 * t + 1]; ml += h[5 * t + 2]; mt += h[5 * t + 3]; mg += h[5 * t + 4];
             if (h[5 * t + 2] > h[5 * worst + 2]) worst = t;
         }
-        fprintf(stderr, "[kprof] mean clocks/task: wait %.3g stage %.3g groups %.3g total %.3g; groups/task %.0f; "
+        fprintf(stderr, "[kprof] mean clocks/task: wait %.3g (mover's dataflow wait %.3g) groups %.3g total %.3g; groups/task %.0f; "
                         "busiest task %zu: loop %.3g clocks, %llu groups (%.1f clocks/group)\n",
                 mw / d, ms / d, ml / d, mt / d, mg / d, worst, (double)h[5 * worst + 2], h[5 * worst + 4],
                 (double)h[5 * worst + 2] / std::max<unsigned long long>(1, h[5 * worst + 4]));

@@ -367,18 +367,135 @@ static_assert(Prefetch<1>::GPREF <= GROUP_RING && Prefetch<2>::GPREF <= GROUP_RI
               "a prefetched W row may be at most GROUP_RING groups old");
 static_assert(GQ % Prefetch<1>::GPREF == 0 && GQ % Prefetch<2>::GPREF == 0, "W queue cycles with the unroll");
 
+// LDS of the grouped epoch kernel (floats): the W ring (GROUP_RING x 4 slots x
+// ROW), then two H buffers, each (max_cols + 1) x ROW (the last row is the
+// dummy column) followed by the columns' factors, then two flags.
+__host__ __device__ inline int hbuf_floats(int max_cols, int vec) {
+    return ((max_cols + 1) * 64 * vec + max_cols + 1 + 3) & ~3;   // float4-aligned
+}
 static size_t epoch_lds(int max_cols, int vec) {
-    return ((size_t)(GROUP_RING * GROUP_SLOTS + max_cols + 1) * 64 * vec + max_cols + 1) * sizeof(float);
+    return ((size_t)GROUP_RING * GROUP_SLOTS * 64 * vec + 2 * (size_t)hbuf_floats(max_cols, vec)) * sizeof(float) +
+           2 * sizeof(int32_t);
 }
 
-// One task's visits of one epoch: the body of the grouped epoch kernel.
+// The two waves of a task's workgroup hand H buffers over through two LDS
+// counters: `staged` (the mover has filled the buffer of the task's k-th
+// non-empty visit) and `done` (the compute wave has finished visit k).  Both
+// only order LDS accesses.
+__device__ __forceinline__ void lds_publish(int32_t* flag, int k) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __hip_atomic_store(flag, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ bool lds_reached(int32_t* flag, int k) {
+    return __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= k;
+}
+__device__ __forceinline__ void lds_await(int32_t* flag, int k) {
+    while (!lds_reached(flag, k)) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The mover wave of a task: for each non-empty visit k it waits for the column
+// block (dataflow), stages its H columns and their factors into buffer k & 1,
+// and, as soon as the compute wave is done with visit k - 1, writes that
+// buffer's columns back and signals the block.  It polls both conditions, so
+// a finished block is released while the next one is still awaited.
+template <typename T, int KIND, int VEC>
+__device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, float* lds) {
+    constexpr int ROW = 64 * VEC;
+    constexpr int CH = 8;                                  // float4s in flight per lane
+    const int lane = threadIdx.x & 63;
+    const int hb = hbuf_floats(A.max_cols, VEC);
+    float* const hbuf0 = lds + GROUP_RING * GROUP_SLOTS * ROW;
+    int32_t* const flags = (int32_t*)(hbuf0 + 2 * hb);     // [0] staged, [1] done
+    const float f2 = (A.eps * 2.f) * A.lambda;
+    const int d = A.d;
+    unsigned long long c_df = 0;
+    int k = 0, pend_b2 = -1, pend_col0 = 0, pend_ncol = 0;
+    auto write_back = [&]() {   // visit k - 1, buffer (k - 1) & 1
+        const float* hbp = hbuf0 + ((k - 1) & 1) * hb;
+        const int total = pend_ncol * 16 * VEC;
+        for (int base = 0; base < total; base += 64 * CH) {
+            float4 v[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e / (16 * VEC), q = e % (16 * VEC);
+                v[u] = e < total ? *(const float4*)(hbp + c * ROW + 4 * q) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e / (16 * VEC), q = e % (16 * VEC);
+                if (e < total && 4 * q < A.ld) Vec4<T>::store(A.H + (int64_t)(pend_col0 + c) * A.ld + 4 * q, v[u]);
+            }
+        }
+        dataflow_signal_wave(A.ready, pend_b2);
+        pend_b2 = -1;
+    };
+    for (int s = 0; s < 2 * d; ++s) {
+        const int visit = s * d + t;
+        const int b2 = uniform_i32((int)A.sched[visit]);
+        const int b = t * 2 * d + b2;
+        if (A.blk[b + 1] == A.blk[b]) continue;
+        const int col0 = uniform_i32(A.off2[b2]);
+        const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
+        const int32_t need = uniform_i32(A.need[visit]);
+        // buffer k & 1 held visit k - 2, written back before this loop began
+        const unsigned long long c0 = A.prof ? clock64() : 0;
+        while (need > 0 && __hip_atomic_load(A.ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            if (pend_b2 >= 0 && lds_reached(flags + 1, k - 1)) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                write_back();
+            } else {
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (A.prof) c_df += clock64() - c0;
+        float* const hbp = hbuf0 + (k & 1) * hb;
+        const int total = ncol * 16 * VEC;
+        for (int base = 0; base < total; base += 64 * CH) {
+            float4 v[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e / (16 * VEC), q = e % (16 * VEC);
+                v[u] = (e < total && 4 * q < A.ld) ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
+                                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e / (16 * VEC), q = e % (16 * VEC);
+                if (e < total) *(float4*)(hbp + c * ROW + 4 * q) = v[u];
+            }
+        }
+        if (KIND == UPD_NZSL_L2) {   // the column's factor f2 / nnz_j, the same product the update forms
+            float* const hinv = hbp + (A.max_cols + 1) * ROW;
+            for (int c = lane; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
+        }
+        lds_publish(flags + 0, k);
+        if (pend_b2 >= 0) {   // the compute wave is on visit k - 1 (or done with it)
+            lds_await(flags + 1, k - 1);
+            write_back();
+        }
+        pend_b2 = b2;
+        pend_col0 = col0;
+        pend_ncol = ncol;
+        ++k;
+    }
+    if (pend_b2 >= 0) {
+        lds_await(flags + 1, k - 1);
+        write_back();
+    }
+    if (A.prof && lane == 0) A.prof[5 * t + 1] = c_df;
+}
+
+// The compute wave of a task: its visits' groups, in order.
 template <typename T, int KIND, bool TRUNC, int VEC>
-__device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, float* lds) {
+__device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, float* lds) {
     constexpr int GPREF = Prefetch<VEC>::GPREF;
     constexpr int ROW = 64 * VEC;                       // floats per staged row
     float* ring = lds;                                  // GROUP_RING x 4 slots x ROW
-    float* hblk = lds + GROUP_RING * GROUP_SLOTS * ROW;  // (max_cols + 1) x ROW; the last row is the dummy column
-    float* hinv = hblk + (A.max_cols + 1) * ROW;        // 1/nnz of the staged columns
+    const int hb = hbuf_floats(A.max_cols, VEC);
+    float* const hbuf0 = lds + GROUP_RING * GROUP_SLOTS * ROW;
+    int32_t* const flags = (int32_t*)(hbuf0 + 2 * hb);
     const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
     constexpr uint32_t ES = Vec4<T>::BYTES;
     const int e0 = 4 * VEC * l16;                       // this lane's first element
@@ -390,15 +507,14 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
     const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
     const uint32_t rec_off = 16u * slot;
     float* const ring_lane = ring + slot * ROW + e0;
-    char* const hblk_lane = (char*)(hblk + e0);         // + column * ROW floats
-    const char* const hinv_b = (const char*)hinv;       // + column << 2
     constexpr int RSH = VEC == 1 ? 8 : 9;               // log2(ROW * 4 bytes)
     const float eps = A.eps;
     const float c1 = eps * -2.f;
     const float f2 = (eps * 2.f) * A.lambda;
     const int d = A.d;
     for (int e = lane; e < GROUP_RING * GROUP_SLOTS * ROW; e += 64) ring[e] = 0.f;
-    unsigned long long c_wait = 0, c_stage = 0, c_loop = 0, n_groups = 0;
+    unsigned long long c_wait = 0, c_loop = 0, n_groups = 0;
+    int k = 0;   // non-empty visits done
     const unsigned long long c_begin = A.prof ? clock64() : 0;
     float4 hprev[VEC];
 #pragma unroll
@@ -413,8 +529,9 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
         const int ngp = (ng + GQ - 1) / GQ * GQ;   // the prep kernel pads with empty groups
         const int64_t gbase = uniform_i64(A.goff[visit]);
         const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
-        const int col0 = uniform_i32(A.off2[b2]);
-        const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
+        const float* const hbp = hbuf0 + (k & 1) * hb;
+        char* const hblk_lane = (char*)(hbp + e0);                     // + column * ROW floats
+        const char* const hinv_b = (const char*)(hbp + (A.max_cols + 1) * ROW);   // + column << 2
         uint4 qa[GQ];
         float4 qb[GPREF][VEC];
         auto load_w = [&](const uint4& rec, float4* out) {
@@ -428,16 +545,7 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
 #pragma unroll
         for (int u = 0; u < GPREF; ++u) load_w(qa[u], qb[u]);
         const unsigned long long c0 = A.prof ? clock64() : 0;
-        dataflow_wait(A.ready, b2, A.need[visit]);
-        const unsigned long long c1c = A.prof ? clock64() : 0;
-        for (int e = lane; e < ncol * 16 * VEC; e += 64) {
-            const int c = e / (16 * VEC), q = e % (16 * VEC);
-            *(float4*)(hblk + c * ROW + 4 * q) = 4 * q < A.ld ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
-                                                               : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        if (KIND == UPD_NZSL_L2)   // the column's factor f2 / nnz_j, the same product the update forms
-            for (int c = lane; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
-        __syncthreads();
+        lds_await(flags + 0, k);   // the mover has staged this visit's columns
         const unsigned long long c2 = A.prof ? clock64() : 0;
         float4 hq[VEC], wq[VEC];
 #pragma unroll
@@ -538,31 +646,39 @@ __device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, flo
                 qa[u] = load_rec(Gr, rbase + 64 * u);
             }
         }
-        __syncthreads();
+        lds_publish(flags + 1, k);   // the buffer goes back to the mover
+        ++k;
         if (A.prof) {
             const unsigned long long c3 = clock64();
-            c_wait += c1c - c0;
-            c_stage += c2 - c1c;
+            c_wait += c2 - c0;
             c_loop += c3 - c2;
             n_groups += (unsigned long long)ng;
         }
-        for (int e = lane; e < ncol * 16 * VEC; e += 64) {
-            const int c = e / (16 * VEC), q = e % (16 * VEC);
-            if (4 * q < A.ld) Vec4<T>::store(A.H + (int64_t)(col0 + c) * A.ld + 4 * q, *(const float4*)(hblk + c * ROW + 4 * q));
-        }
-        dataflow_signal(A.ready, b2);
     }
     if (A.prof && lane == 0) {
         A.prof[5 * t + 0] = c_wait;
-        A.prof[5 * t + 1] = c_stage;
         A.prof[5 * t + 2] = c_loop;
         A.prof[5 * t + 3] = clock64() - c_begin;
         A.prof[5 * t + 4] = n_groups;
     }
 }
 
+// One task = one workgroup of two waves: wave 0 computes, wave 1 moves the
+// column blocks in and out and talks to the other tasks.
 template <typename T, int KIND, bool TRUNC, int VEC>
-__global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
+__device__ __forceinline__ void grouped_task(const GArgs<T>& A, const int t, float* lds) {
+    int32_t* const flags = (int32_t*)(lds + GROUP_RING * GROUP_SLOTS * 64 * VEC + 2 * hbuf_floats(A.max_cols, VEC));
+    if (threadIdx.x == 0) {
+        flags[0] = -1;
+        flags[1] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) grouped_compute<T, KIND, TRUNC, VEC>(A, t, lds);
+    else grouped_mover<T, KIND, VEC>(A, t, lds);
+}
+
+template <typename T, int KIND, bool TRUNC, int VEC>
+__global__ __launch_bounds__(128) void k_epoch_grouped(GArgs<T> A) {
     extern __shared__ float lds[];
     grouped_task<T, KIND, TRUNC, VEC>(A, blockIdx.x, lds);
 }
@@ -571,7 +687,7 @@ __global__ __launch_bounds__(64) void k_epoch_grouped(GArgs<T> A) {
 // `tmap`; each problem's arguments are read from device memory, so a captured
 // launch replays with new step sizes and plans.
 template <typename T, int KIND, bool TRUNC>
-__global__ __launch_bounds__(64) void k_epoch_grouped_batch(const GArgs<T>* __restrict__ probs,
+__global__ __launch_bounds__(128) void k_epoch_grouped_batch(const GArgs<T>* __restrict__ probs,
                                                             const int2* __restrict__ tmap) {
     extern __shared__ float lds[];
     const int p = uniform_i32(tmap[blockIdx.x].x), t = uniform_i32(tmap[blockIdx.x].y);
@@ -614,7 +730,7 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     int per_cu = 0, cus = 0;
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return no("LDS attribute");
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, lds) != hipSuccess) return no("occupancy query");
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 128, lds) != hipSuccess) return no("occupancy query");
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return no("no device");
     if ((int64_t)per_cu * cus < P.d) return no("not all tasks can be resident");
     return true;
@@ -681,7 +797,7 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(64), args, bytes, s));
+    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(128), args, bytes, s));
 }
 
 template <typename T, int KIND, bool TRUNC>
@@ -759,7 +875,7 @@ int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device) 
     const void* fn = precision == PREC_FP32 ? batch_fn_p<float>(kind, trunc) : batch_fn_p<bf16_t>(kind, trunc);
     int per_cu = 0, cus = 0;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    HIPK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, lds));
+    HIPK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 128, lds));
     HIPK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     return per_cu * cus;
 }
@@ -771,7 +887,7 @@ void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args,
     for (int k = 0; k < nprob; ++k) k_zero_i32<<<1, 256, 0, s>>>(ready[k], 2 * dsz[k]);
     const void* fn = precision == PREC_FP32 ? batch_fn_p<float>(kind, trunc) : batch_fn_p<bf16_t>(kind, trunc);
     void* args[] = {const_cast<void**>(&d_args), const_cast<int2**>(&d_tmap)};
-    HIPK(hipLaunchKernel(fn, dim3(nblocks), dim3(64), args, lds, s));
+    HIPK(hipLaunchKernel(fn, dim3(nblocks), dim3(128), args, lds, s));
 }
 
 }  // namespace dsgdpp

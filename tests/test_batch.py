@@ -15,8 +15,8 @@ def test_plan_tasks_balances_predicted_time():
              dict(m=10_000, n=5_000, nnz=1_000_000, rank=64),
              dict(m=40, n=30, nnz=600, rank=128)]
     mins = [B.min_tasks(p) for p in probs]
-    # 5000 columns need 8 tasks for the H tile; 8M points need 16 for the prep state
-    assert mins == [16, 8, 1]
+    # two rank-64 H tiles of 5000 / 2d columns need d = 16; 8M points need 16 for the prep state
+    assert mins == [16, 16, 1]
     assert all(B.lds_need(p["n"], m, p["rank"]) <= B.LDS_BYTES for p, m in zip(probs, mins))
     assert B.prep_lds_estimate(10_000, 5_000, 8_000_000, 8) > B.LDS_BYTES
     d = B.plan_tasks(probs, budget=60, rates=rates)
