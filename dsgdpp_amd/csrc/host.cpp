@@ -167,6 +167,7 @@ struct EpochPlan {
     std::vector<int32_t> need;        // 2d x d
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
+    int64_t maxBlock = 0;             // largest block visited
     int nSmall = 0;                   // grouped prep: leading chains whose blocks are <= PREP_SPLIT points
     int64_t capSmall = 0;
     uint32_t* dSched = nullptr;
@@ -604,8 +605,9 @@ void drawPlan(dsgdpp_ctx* c, EpochPlan& P) {
 }
 
 // Visit-order jobs, dataflow dependencies and the visit-order kernel of a
-// drawn plan, all enqueued on `s`.
-void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
+// drawn plan, all enqueued on `s`.  With launchPrep false the caller launches
+// the grouped prep (one launch for many plans) and records P.evEnd.
+void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = true) {
     const uint32_t d = c->d, d2 = 2 * d;
     const std::vector<int64_t>& bs = c->blocks[0].start;
     auto visitBlock = [&](uint32_t id, uint32_t sub) { return (size_t)id * d2 + P.schedule[(size_t)sub * d + id]; };
@@ -758,7 +760,13 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s) {
         HIPC(hipMemsetAsync(G.pprof, 0, 4 * sizeof(unsigned long long), s));
     }
     const int nchains = (int)P.chains.size() - 1;
+    P.maxBlock = maxBlock;
     HIPC(hipEventRecord(P.evStart, s));
+    if (!launchPrep && P.grouped) {
+        P.built = true;
+        P.dataVersion = c->dataVersion;
+        return;
+    }
     if (P.grouped) {
         if (P.nSmall > 0)
             launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s);
@@ -1411,7 +1419,13 @@ struct dsgdpp_batch {
     size_t argBytes = 0, lds = 0;
     int nblocks = 0;
     hipStream_t stream = nullptr;
+    hipStream_t prepStream = nullptr;   // the look-ahead preps of all contexts, one launch per index width
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<PrepDesc> desc;
+    std::vector<int2> cmap;
+    PrepDesc* dDesc = nullptr;
+    int2* dCmap = nullptr;
+    size_t descCap = 0, cmapCap = 0;
     uint8_t* hArgs = nullptr;    // pinned host staging of the argument table
     void* dArgs = nullptr;
     int2* dTmap = nullptr;
@@ -1438,6 +1452,12 @@ void batchDestroy(dsgdpp_batch* b) {
     dfree(q);
     q = b->dTmap;
     dfree(q);
+    q = b->dDesc;
+    dfree(q);
+    q = b->dCmap;
+    dfree(q);
+    if (b->prepStream) (void)hipStreamSynchronize(b->prepStream);
+    if (b->prepStream) (void)hipStreamDestroy(b->prepStream);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1493,18 +1513,57 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
                             b->ready.data(), b->dsz.data(), (int)K, b->stream);
     HIPC(hipEventRecord(b->ev1, b->stream));
     const auto t_launch = std::chrono::steady_clock::now();
+    // every context's next plan is prepared while the batch runs: the host
+    // parts one by one, the grouped preps of all of them in one launch per
+    // index width
+    HIPC(hipStreamWaitEvent(b->prepStream, b->ev0, 0));
+    b->desc.clear();
+    b->cmap.clear();
+    std::vector<EpochPlan*> ahead;
+    std::vector<int2> wide;
+    size_t ldsNarrow = 0, ldsWide = 0;
     for (size_t k = 0; k < K; ++k) {
         dsgdpp_ctx* c = b->ctx[k];
         c->lastPlan = c->cur;
         c->plan[c->cur].drawn = false;
         c->cur ^= 1;
-        // every context's next plan is prepared while the batch runs
-        HIPC(hipStreamWaitEvent(c->prepStream, b->ev0, 0));
-        if (c->lookahead) {
-            EpochPlan& N = c->plan[c->cur];
-            drawPlan(c, N);
-            buildPlan(c, N, c->prepStream);
+        if (!c->lookahead) continue;
+        EpochPlan& N = c->plan[c->cur];
+        drawPlan(c, N);
+        buildPlan(c, N, b->prepStream, false);
+        DevProblem Dp = c->P;
+        Dp.perm = N.perm;
+        GroupedBuffers G = c->G;
+        G.grec = N.grec;
+        G.ngroups = N.ngroups;
+        const PrepDesc D = prep_desc(Dp, G, N.dJobs, N.dChains, c->sgdOrder, N.maxBlock);
+        const int idx = (int)b->desc.size();
+        b->desc.push_back(D);
+        const bool narrow = prep_narrow(N.maxBlock);
+        for (int ch = 0; ch + 1 < (int)N.chains.size(); ++ch) (narrow ? b->cmap : wide).push_back(make_int2(idx, ch));
+        size_t& l = narrow ? ldsNarrow : ldsWide;
+        l = std::max(l, prep_desc_lds(D));
+        ahead.push_back(&N);
+    }
+    if (!ahead.empty()) {
+        const int nNarrow = (int)b->cmap.size();
+        b->cmap.insert(b->cmap.end(), wide.begin(), wide.end());
+        if (b->desc.size() > b->descCap) {
+            b->descCap = b->desc.size();
+            dalloc(b->dDesc, b->descCap * sizeof(PrepDesc));
         }
+        if (b->cmap.size() > b->cmapCap) {
+            b->cmapCap = b->cmap.size();
+            dalloc(b->dCmap, b->cmapCap * sizeof(int2));
+        }
+        HIPC(hipMemcpyAsync(b->dDesc, b->desc.data(), b->desc.size() * sizeof(PrepDesc), hipMemcpyHostToDevice,
+                            b->prepStream));
+        if (!b->cmap.empty())
+            HIPC(hipMemcpyAsync(b->dCmap, b->cmap.data(), b->cmap.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                b->prepStream));
+        launch_prep_grouped_multi(b->dDesc, b->dCmap, nNarrow, true, ldsNarrow, b->prepStream);
+        launch_prep_grouped_multi(b->dDesc, b->dCmap + nNarrow, (int)wide.size(), false, ldsWide, b->prepStream);
+        for (EpochPlan* N : ahead) HIPC(hipEventRecord(N->evEnd, b->prepStream));
     }
     const auto t_ahead = std::chrono::steady_clock::now();
     HIPC(hipEventSynchronize(b->ev1));
@@ -1580,6 +1639,7 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             throw Error(DSGDPP_E_UNSUPPORTED, "the batch has " + std::to_string(b->nblocks) + " tasks; at most " +
                                                   std::to_string(cap) + " can be resident at once");
         HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&b->prepStream, hipStreamNonBlocking));
         HIPC(hipEventCreate(&b->ev0));
         HIPC(hipEventCreate(&b->ev1));
         HIPC(hipHostMalloc((void**)&b->hArgs, (size_t)count * b->argBytes, hipHostMallocDefault));

@@ -107,8 +107,26 @@ struct GroupedBuffers {
     unsigned long long* prof = nullptr;   // optional kernel clock counters (5 per task)
     unsigned long long* pprof = nullptr;  // optional prep clock sums {visit order, scheduling, fill, visits}
 };
+// The prep kernel's inputs for one problem (many problems share one launch).
+struct PrepDesc {
+    const PermJob* jobs;
+    const int32_t* chain_start;
+    const Pt32* pts;
+    uint32_t* perm;
+    void* grec;
+    int32_t* ngroups;
+    int32_t order, cap, rows_cap, cols_cap;
+    uint32_t dummy_row, dummy_col;
+};
+static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols,
                        int device, std::string* why);
+PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs, const int32_t* d_chain_start,
+                   int order, int64_t max_block);
+bool prep_narrow(int64_t max_block);
+size_t prep_desc_lds(const PrepDesc& D);
+void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,
+                               hipStream_t s);
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block,
                          hipStream_t s);

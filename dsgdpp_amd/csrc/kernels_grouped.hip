@@ -54,6 +54,14 @@ __global__ void k_zero_i32(int32_t* p, int n) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
 }
 
+// a value every lane holds alike, in a scalar register
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // old with lane `lane` (uniform) replaced by v
 __device__ __forceinline__ uint32_t writelane_u32(uint32_t v, int lane, uint32_t old) {
     return (int)threadIdx.x == lane ? v : old;
@@ -72,15 +80,14 @@ static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow) {
     return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + up4(2 * (size_t)cap + GROUP_PAD);
 }
 
+// One chain of visits (one wave): visit order, groups, records.
 template <typename IX>
-__global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
-                                                     const int32_t* __restrict__ chain_start, int order,
-                                                     const Pt32* __restrict__ pts, uint32_t* __restrict__ perm,
-                                                     uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
-                                                     int rows_cap,
-                                                     int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
-                                                     unsigned long long* __restrict__ pprof) {
-    extern __shared__ uint32_t sm[];
+__device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, const int32_t* __restrict__ chain_start,
+                                           const int chain, const int order, const Pt32* __restrict__ pts,
+                                           uint32_t* __restrict__ perm, uint4* __restrict__ grec,
+                                           int32_t* __restrict__ ngroups, const int cap, const int rows_cap,
+                                           const int cols_cap, const uint32_t dummy_row, const uint32_t dummy_col,
+                                           unsigned long long* __restrict__ pprof, uint32_t* sm) {
     auto up4 = [](int k) { return (k * (int)sizeof(IX) + 3) >> 2; };   // words
     IX* vis = (IX*)sm;                               // visit order
     IX* lastR = (IX*)(sm + up4(cap));                // per local row: tag of its last update
@@ -88,7 +95,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
     uint32_t* gmw = sm + up4(cap) + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 per word
     const int lane = threadIdx.x;
     unsigned long long t_order = 0, t_sched = 0, t_fill = 0;
-    for (int jx = chain_start[blockIdx.x]; jx < chain_start[blockIdx.x + 1]; ++jx) {
+    for (int jx = chain_start[chain]; jx < chain_start[chain + 1]; ++jx) {
         const PermJob J = jobs[jx];
         const int n = (int)J.n;   // < 2^22
         const unsigned long long t0 = pprof ? clock64() : 0;
@@ -253,8 +260,36 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
         atomicAdd(pprof + 0, t_order);
         atomicAdd(pprof + 1, t_sched);
         atomicAdd(pprof + 2, t_fill);
-        atomicAdd(pprof + 3, (unsigned long long)(chain_start[blockIdx.x + 1] - chain_start[blockIdx.x]));
+        atomicAdd(pprof + 3, (unsigned long long)(chain_start[chain + 1] - chain_start[chain]));
     }
+}
+
+template <typename IX>
+__global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
+                                                     const int32_t* __restrict__ chain_start, int order,
+                                                     const Pt32* __restrict__ pts, uint32_t* __restrict__ perm,
+                                                     uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
+                                                     int rows_cap, int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
+                                                     unsigned long long* __restrict__ pprof) {
+    extern __shared__ uint32_t sm[];
+    prep_chain<IX>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap, dummy_row,
+                   dummy_col, pprof, sm);
+}
+
+// The chains of many problems in one launch: workgroup -> (problem, chain) by
+// `cmap`, each problem's buffers and bounds from its descriptor.
+template <typename IX>
+__global__ __launch_bounds__(64) void k_prep_grouped_multi(const PrepDesc* __restrict__ descs,
+                                                           const int2* __restrict__ cmap) {
+    extern __shared__ uint32_t sm[];
+    const int p = uniform_i32(cmap[blockIdx.x].x), chain = uniform_i32(cmap[blockIdx.x].y);
+    PrepDesc D;
+    const uint32_t* src = (const uint32_t*)(descs + p);
+    uint32_t* dst = (uint32_t*)&D;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(PrepDesc) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
+    prep_chain<IX>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap, D.rows_cap,
+                   D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,13 +374,6 @@ struct GArgs {
     int d, ld, max_cols;
     float eps, lambda, tmin, tmax;
 };
-
-__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
 
 // Records of group K sit at K * 64 bytes (+ 16 per slot).  The offset goes in
 // the vector offset, which the buffer range check covers: the loop's reads of
@@ -753,6 +781,38 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
         k_prep_grouped<uint32_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
                                                             (uint4*)B.grec, B.ngroups, cap,
                                                             B.max_rows, B.max_cols, drow, dcol, B.pprof);
+    HIPK(hipGetLastError());
+}
+
+PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs, const int32_t* d_chain_start,
+                   int order, int64_t max_block) {
+    PrepDesc D{};
+    D.jobs = d_jobs;
+    D.chain_start = d_chain_start;
+    D.pts = (const Pt32*)P.pts;
+    D.perm = P.perm;
+    D.grec = B.grec;
+    D.ngroups = B.ngroups;
+    D.order = order;
+    D.cap = (int32_t)std::max<int64_t>(max_block, 1);
+    D.rows_cap = B.max_rows;
+    D.cols_cap = B.max_cols;
+    D.dummy_row = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2);
+    D.dummy_col = (uint32_t)B.max_cols;
+    return D;
+}
+
+bool prep_narrow(int64_t max_block) { return std::max<int64_t>(max_block, 1) < PREP_NARROW; }
+
+size_t prep_desc_lds(const PrepDesc& D) { return prep_lds(D.cap, D.rows_cap, D.cols_cap, D.cap < PREP_NARROW); }
+
+void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,
+                               hipStream_t s) {
+    if (nchains <= 0) return;
+    const void* fn = narrow ? (const void*)k_prep_grouped_multi<uint16_t> : (const void*)k_prep_grouped_multi<uint32_t>;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (narrow) k_prep_grouped_multi<uint16_t><<<nchains, 64, lds, s>>>(d_descs, d_cmap);
+    else k_prep_grouped_multi<uint32_t><<<nchains, 64, lds, s>>>(d_descs, d_cmap);
     HIPK(hipGetLastError());
 }
 
