@@ -146,7 +146,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(p, update, tasks, args.cpu_threads)
+        cpu = cpu_baseline(p, update, args.cpu_threads)
 
     out = {
         "metric": "nnz updates/s (DSGD++ epoch, 100M-nonzero Zipf workload)" if name == "zipf100m"
@@ -219,15 +219,32 @@ def measured_traffic(name, precision, rank, kernel):
     return None
 
 
-def cpu_baseline(p, update, tasks, threads, min_seconds=10.0, max_epochs=40):
-    """The oracle (FP64 C++ restatement of mfdsgdpp) on the host cores: whole
-    epochs of the same workload with the same d x 2d blocking, as many as
-    fit in about `min_seconds` (at least one, at most `max_epochs`)."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def _cpu_cores(threads):
+    return threads or min(16, len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline(p, update, threads, min_seconds=10.0, max_epochs=40):
+    """The oracle (FP64 C++ restatement of mfdsgdpp) on the host cores, run as
+    the reference would be on them: d = --tasks-per-rank = the core count
+    (d x 2d blocks), one worker thread per task.  Whole epochs of the same
+    workload, as many as fit in about `min_seconds` (at least one, at most
+    `max_epochs`)."""
     try:
         from oracle import pyoracle as po
     except Exception as e:   # noqa: BLE001
         return {"error": f"oracle unavailable: {e}"}
-    cores = threads or min(16, len(os.sched_getaffinity(0)))
+    cores = _cpu_cores(threads)
+    tasks = cores
     t0 = time.time()
     o = po.Oracle(update=update, loss=update, rank=p["rank"], tasks_per_rank=tasks, seed=777,
                   threads=cores)
@@ -242,8 +259,40 @@ def cpu_baseline(p, update, tasks, threads, min_seconds=10.0, max_epochs=40):
     log(f"[bench] cpu baseline: {epochs} epochs in {secs:.2f}s")
     o.close()
     return {"value": epochs * nnz / secs, "unit": "nnz updates/s", "cores": cores, "kind": "port",
-            "sample": f"{epochs} full FP64 epochs of the same workload ({nnz} updates each, d={tasks}, "
-                      f"eps 0.01), {cores} threads, {secs:.2f}s of epoch time"}
+            "sample": f"{epochs} full FP64 epochs of the same workload ({nnz} updates each, eps 0.01), "
+                      f"d = --tasks-per-rank = {tasks} ({tasks}x{2 * tasks} blocks), {cores} threads of "
+                      f"nproc {os.cpu_count()} ({_cpu_model()}), {secs:.2f}s of epoch time"}
+
+
+def cpu_time_to_rmse(p, update, decay0, target, threads, max_epochs):
+    """The CPU side of the time-to-RMSE comparison: the oracle (FP64, d = the
+    core count) with BoldDriver, training and test loss after every epoch as
+    the reference runner evaluates them, until the test RMSE reaches
+    `target` or `max_epochs` epochs have run."""
+    from oracle import pyoracle as po
+    import dsgdpp_amd as D
+    cores = _cpu_cores(threads)
+    o = po.Oracle(update=update, loss=update, rank=p["rank"], tasks_per_rank=cores, seed=12345,
+                  threads=cores)
+    o.load_coo(0, p["m"], p["n"], *p["train"])
+    o.load_coo(1, p["m"], p["n"], *p["test"])
+    o.set_factors(p["W0"], p["H0"])
+    nt = len(p["test"][2])
+    rule = D.BoldDriver(decay0)
+    t0 = time.perf_counter()
+    prev, cur = None, o.loss(0)
+    hit, rmse, e = None, None, 0
+    for e in range(1, max_epochs + 1):
+        o.epoch(rule(prev, cur))
+        prev, cur = cur, o.loss(0)
+        rmse = math.sqrt(o.loss(1) / nt)
+        log(f"[time-to-rmse] cpu epoch {e}: test rmse {rmse:.5f} at {time.perf_counter() - t0:.1f}s")
+        if rmse <= target:
+            hit = time.perf_counter() - t0
+            break
+    o.close()
+    return {"epochs": e, "seconds": hit, "final_test_rmse": rmse, "reached": hit is not None,
+            "cores": cores, "tasks_per_rank": cores, "cpu": _cpu_model()}
 
 
 def _ctx(D, p, update, tasks, precision, seed, kernel=0):
@@ -347,6 +396,9 @@ def report_time_to_rmse(args):
         res[name] = {"epochs": len(h), "seconds": hit[0] if hit else None, "final_test_rmse": h[-1][1],
                      "reached": bool(hit), "speedup_vs_fp64": (t64 / hit[0]) if hit else None}
         log(f"[time-to-rmse] {name}: {res[name]}")
+    if not args.no_cpu_baseline:
+        res["cpu_fp64_oracle"] = cpu_time_to_rmse(p, update, decay0, target, args.cpu_threads, epochs)
+        log(f"[time-to-rmse] cpu_fp64_oracle: {res['cpu_fp64_oracle']}")
     print(json.dumps({"report": "time-to-rmse", "workload": f"zipf100m: {p['m']}x{p['n']}, "
                       f"{len(p['train'][2])} train / {nt} test nonzeros, rank 50, {update}, start "
                       f"BoldDriver({decay0}), {tasks}x{2 * tasks} blocks",
@@ -364,7 +416,7 @@ def bench_batch(args):
     import dsgdpp_amd as D
     from dsgdpp_amd import batch as B
     t0 = time.time()
-    probs = B.batch_problems(64, seed=3, shrink=args.shrink)
+    probs = B.batch_problems(64, seed=3, shrink=args.shrink, log=log)
     nnz = [len(p["train"][2]) for p in probs]
     log(f"[bench] generated 64 problems, {sum(nnz)} nonzeros in {time.time() - t0:.1f}s")
     rates = B.measure_rates(log=log)
@@ -386,6 +438,8 @@ def bench_batch(args):
             g.close()
             d[k] *= 2
         ctxs.append(g)
+        if (k + 1) % 8 == 0:
+            log(f"[bench] {k + 1} contexts loaded ({time.time() - t0:.1f}s)")
     parts = B.partition(d, budget)
     batches = [B.DsgdPpBatch([ctxs[k] for k in part], use_graph=True) for part in parts]
     log(f"[bench] contexts loaded + batches created in {time.time() - t0:.1f}s")

@@ -201,7 +201,7 @@ def partition(tasks, budget):
     return [sorted(p) for p in parts]
 
 
-def batch_problems(count=64, seed=0, shrink=1, max_nnz=20_000_000):
+def batch_problems(count=64, seed=0, shrink=1, max_nnz=20_000_000, log=None):
     """Independent problems: rows log-uniform in [1e4, 1e6], columns
     log-uniform in [1e3, 1e5], nnz = U(1e-4, 1e-2) * rows * cols capped at
     max_nnz, rank-10 N(0,1) ground truth plus N(0, 0.1^2) noise, rank drawn
@@ -222,6 +222,8 @@ def batch_problems(count=64, seed=0, shrink=1, max_nnz=20_000_000):
                                  init=0.1)
         p["lambda"] = lam
         out.append(p)
+        if log and (k + 1) % 8 == 0:
+            log(f"[batch] {k + 1}/{count} problems generated")
     return out
 
 
