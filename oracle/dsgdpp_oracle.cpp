@@ -319,7 +319,7 @@ struct Ctx {
     std::vector<uint64_t> seeds;
     bool record = false;
     std::vector<std::vector<uint32_t>> recPerm;   // [task*2d + subepoch]
-    int arith = 0;   // 0 FP64 reference; 1 FP32 grouped-GPU emulation; 2 BF16 grouped-GPU emulation
+    int arith = 0;   // 0 FP64 reference; 1 FP32 / 2 BF16 / 3 FP64 grouped-GPU emulation
 };
 
 static void setErr(Ctx* c, const std::string& e) { c->err = e; }
@@ -503,6 +503,43 @@ static inline void updatePointF(const UpdateArgs& a, int mode, double* w, double
     }
 }
 
+// Double emulation of the GPU's grouped FP64 update (not part of the
+// reference; rank <= 64): the lane layout and summation order of the float
+// emulation above, in double with fused multiply-adds.
+static inline void updatePointD(const UpdateArgs& a, double* w, double* h, double x, double eps, double n1,
+                                double n2) {
+    const int64_t r = a.r;
+    double wd[64], hd[64];
+    for (int64_t z = 0; z < 64; ++z) {
+        wd[z] = z < r ? w[z] : 0.;
+        hd[z] = z < r ? h[z] : 0.;
+    }
+    double dl[16];
+    for (int l = 0; l < 16; ++l) {
+        double p = wd[4 * l] * hd[4 * l];
+        for (int e = 1; e < 4; ++e) p = std::fma(wd[4 * l + e], hd[4 * l + e], p);
+        dl[l] = p;
+    }
+    double q[4];
+    for (int k = 0; k < 4; ++k) q[k] = (dl[4 * k] + dl[4 * k + 1]) + (dl[4 * k + 2] + dl[4 * k + 3]);
+    const double wh = (q[0] + q[1]) + (q[2] + q[3]);
+    const double f1 = (eps * -2.) * (x - wh);
+    const double f2 = (eps * 2.) * a.lambda;
+    double aa = 0., bb = 0.;
+    if (a.kind == U_NZSL_L2) { aa = f2 * (1. / n1); bb = f2 * (1. / n2); }
+    else if (a.kind == U_NZSL_NZL2) { aa = f2; bb = f2; }
+    for (int64_t z = 0; z < r; ++z) {
+        double wn = std::fma(-f1, hd[z], std::fma(-aa, wd[z], wd[z]));
+        double hn = std::fma(-f1, wd[z], std::fma(-bb, hd[z], hd[z]));
+        if (a.truncate) {
+            wn = (wn <= a.tmin) ? a.tmin + eps : ((wn >= a.tmax) ? a.tmax - eps : wn);
+            hn = (hn <= a.tmin) ? a.tmin + eps : ((hn >= a.tmax) ? a.tmax - eps : hn);
+        }
+        w[z] = wn;
+        h[z] = hn;
+    }
+}
+
 // One task's SgdRunner (sgd.h:111; one per task per epoch, dsgdpp_impl.h:112).
 struct TaskRunner {
     Rng rng;
@@ -525,6 +562,8 @@ static void runBlock(Ctx* c, TaskRunner& tr, uint32_t b1, uint32_t b2, double ep
         int64_t gi = o1 + B.i[p], gj = o2 + B.j[p];
         if (mode == 0)
             updatePoint(ua, W + gi * r, H + gj * r, B.x[p], eps, (double)c->nnz1[gi], (double)c->nnz2[gj]);
+        else if (mode == 3)
+            updatePointD(ua, W + gi * r, H + gj * r, B.x[p], eps, (double)c->nnz1[gi], (double)c->nnz2[gj]);
         else
             updatePointF(ua, mode, W + gi * r, H + gj * r, B.x[p], eps, (double)c->nnz1[gi], (double)c->nnz2[gj]);
     };
@@ -989,11 +1028,12 @@ int orc_get_schedule(orc_ctx* c, uint32_t* out) {
 int orc_set_record(orc_ctx* c, int on) { c->record = on != 0; return 0; }
 
 int orc_set_arith(orc_ctx* c, int mode) {
-    if (mode < 0 || mode > 2) { setErr(c, "invalid arithmetic mode"); return -1; }
-    if (mode != 0 && c->r > 128) { setErr(c, "float emulation needs rank <= 128"); return -1; }
+    if (mode < 0 || mode > 3) { setErr(c, "invalid arithmetic mode"); return -1; }
+    if ((mode == 1 || mode == 2) && c->r > 128) { setErr(c, "float emulation needs rank <= 128"); return -1; }
+    if (mode == 3 && c->r > 64) { setErr(c, "FP64 grouped emulation needs rank <= 64"); return -1; }
     c->arith = mode;
     // values as the GPU stores them: x as float, factors rounded to the storage type
-    if (mode != 0) {
+    if (mode == 1 || mode == 2) {
         for (auto* blocks : {&c->train, &c->test})
             for (auto& B : *blocks)
                 for (auto& v : B.x) v = (float)v;

@@ -54,8 +54,8 @@ int orc_balance(orc_ctx* c, int type);
 /* parity hooks (last epoch) */
 int orc_get_schedule(orc_ctx* c, uint32_t* out /* 2d x d row-major */);
 int orc_set_record(orc_ctx* c, int on);
-/* 0: FP64 reference arithmetic; 1/2: emulate the GPU grouped FP32/BF16 update
- * (call after loading data and factors) */
+/* 0: FP64 reference arithmetic; 1/2/3: emulate the GPU grouped FP32/BF16/FP64
+ * update (call after loading data and factors) */
 int orc_set_arith(orc_ctx* c, int mode);
 int orc_get_permutation(orc_ctx* c, int task, int subepoch, uint32_t* out,
                         int64_t cap, int64_t* n);

@@ -139,8 +139,8 @@ class Oracle:
         self._check(lib().orc_balance(self._h, {"None": 0, "L2": 1, "NZL2": 2}[kind]))
 
     def set_arith(self, mode):
-        """0 FP64 reference; 'FP32'/'BF16': emulate the GPU grouped update."""
-        m = {"FP64": 0, "FP32": 1, "BF16": 2}.get(mode, mode)
+        """0 FP64 reference; 'FP32'/'BF16'/'FP64_GROUPED': emulate the GPU grouped update."""
+        m = {"FP64": 0, "FP32": 1, "BF16": 2, "FP64_GROUPED": 3}.get(mode, mode)
         self._check(lib().orc_set_arith(self._h, int(m)))
 
     def record(self, on=True):
