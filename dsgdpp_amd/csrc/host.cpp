@@ -499,10 +499,11 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
         c->P.r = c->r;
         c->P.d = (int)c->d;
         c->P.precision = c->precision;
-        // rows padded to 16 bytes; above rank 64 to 8 elements (the grouped
-        // kernel's lanes then hold 8 elements of a row, never part of the next)
+        // rows padded to 16 bytes; FP64 rows to 4 elements and rows above rank
+        // 64 to 8 (the grouped kernel's lanes hold 4 or 8 elements of a row,
+        // never part of the next)
         const int64_t vec = c->r > 64 ? std::max<int64_t>(8, 16 / (int64_t)c->factorBytes())
-                                      : 16 / (int64_t)c->factorBytes();
+                                      : std::max<int64_t>(c->factorBytes() == 8 ? 4 : 1, 16 / (int64_t)c->factorBytes());
         c->P.ld = (c->r + vec - 1) / vec * vec;
         c->dataflowOk = dataflow_fits(c->P, c->device);
     }
@@ -1606,6 +1607,9 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             for (int q = 0; q < k; ++q)
                 if (ctxs[q] == c) throw Error(DSGDPP_E_ARG, "a context appears twice");
             requireReady(c);
+            if (c->precision != PREC_FP32 && c->precision != PREC_BF16)
+                throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
+                                                      ": batched epochs run FP32/BF16 factors (grouped kernel)");
             if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3))
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " cannot use the grouped kernel: " +
                                                       (c->groupedOk ? std::string("kernel forced") : c->groupedWhy));

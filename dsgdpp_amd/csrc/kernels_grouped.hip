@@ -80,10 +80,21 @@ static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow) {
     return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + up4(2 * (size_t)cap + GROUP_PAD);
 }
 
+// A record's value words: x as float and the row's 1/nnz (FP32/BF16 points),
+// or x as a double (FP64 points; the row factor comes from its own array).
+__device__ __forceinline__ void rec_value(const Pt32& e, uint4& o) {
+    o.z = __float_as_uint(e.x);
+    o.w = e.aux;
+}
+__device__ __forceinline__ void rec_value(const Pt64& e, uint4& o) {
+    o.z = (uint32_t)__double2loint(e.x);
+    o.w = (uint32_t)__double2hiint(e.x);
+}
+
 // One chain of visits (one wave): visit order, groups, records.
-template <typename IX>
+template <typename IX, typename PT>
 __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, const int32_t* __restrict__ chain_start,
-                                           const int chain, const int order, const Pt32* __restrict__ pts,
+                                           const int chain, const int order, const PT* __restrict__ pts,
                                            uint32_t* __restrict__ perm, uint4* __restrict__ grec,
                                            int32_t* __restrict__ ngroups, const int cap, const int rows_cap,
                                            const int cols_cap, const uint32_t dummy_row, const uint32_t dummy_col,
@@ -164,7 +175,9 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         // the next chunk's points are fetched while the current chunk is scanned
         // (a training point carries 1/nnz of its row in .aux)
         auto fetch = [&](int v) {
-            Pt32 e = Pt32{J.row0, J.col0, 0.f, 0u};
+            PT e{};
+            e.i = J.row0;
+            e.j = J.col0;
             if (v < n) {
                 const uint32_t q = vis[v];
                 perm[J.base + v] = q;
@@ -172,10 +185,10 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             }
             return e;
         };
-        Pt32 en = fetch(lane);
+        PT en = fetch(lane);
         for (int v0 = 0; v0 < n; v0 += 64) {
             const int v = v0 + lane;
-            const Pt32 e = en;
+            const PT e = en;
             const uint32_t r = (uint32_t)(e.i - J.row0), c = (uint32_t)(e.j - J.col0);
             uint32_t myR = v < n ? (uint32_t)lastR[r] : 0u, myC = v < n ? (uint32_t)lastC[c] : 0u;
             en = fetch(v + 64);
@@ -230,8 +243,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 o.x = (uint32_t)e.i |
                       (wf ? (REC_WFWD | (((((uint32_t)gR & (GROUP_RING - 1)) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
                 o.y = c | (((res >> 2) & 1u) ? REC_HREG : 0u);
-                o.z = __float_as_uint(e.x);
-                o.w = e.aux;
+                rec_value(e, o);
                 grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
             }
             __syncthreads();
@@ -264,16 +276,16 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
     }
 }
 
-template <typename IX>
+template <typename IX, typename PT>
 __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
                                                      const int32_t* __restrict__ chain_start, int order,
-                                                     const Pt32* __restrict__ pts, uint32_t* __restrict__ perm,
+                                                     const PT* __restrict__ pts, uint32_t* __restrict__ perm,
                                                      uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
                                                      int rows_cap, int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
                                                      unsigned long long* __restrict__ pprof) {
     extern __shared__ uint32_t sm[];
-    prep_chain<IX>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap, dummy_row,
-                   dummy_col, pprof, sm);
+    prep_chain<IX, PT>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
+                       dummy_row, dummy_col, pprof, sm);
 }
 
 // The chains of many problems in one launch: workgroup -> (problem, chain) by
@@ -288,8 +300,8 @@ __global__ __launch_bounds__(64) void k_prep_grouped_multi(const PrepDesc* __res
     uint32_t* dst = (uint32_t*)&D;
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(PrepDesc) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
-    prep_chain<IX>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap, D.rows_cap,
-                   D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm);
+    prep_chain<IX, Pt32>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap,
+                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -711,6 +723,323 @@ __global__ __launch_bounds__(128) void k_epoch_grouped(GArgs<T> A) {
     grouped_task<T, KIND, TRUNC, VEC>(A, blockIdx.x, lds);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 grouped epoch (precision FP64, rank <= 64)
+// ---------------------------------------------------------------------------
+// The same prep, records, dataflow and two-wave structure with double factors
+// and math.  Lane l16 of a slot holds elements [4 l16, 4 l16 + 4) of W_i and
+// H_j (32 bytes, two 16-byte accesses); a staged row is 64 doubles.  A record
+// carries x as a double in .z/.w, and the row's 1/nnz is prefetched from the
+// FP64 array together with the W row.  The association (4-term FMA chain, the
+// DPP row tree, fused updates) is the oracle's FP64 grouped emulation.
+struct GArgs64 {
+    const uint4* __restrict__ grec;
+    const int32_t* __restrict__ ngroups;
+    const int32_t* __restrict__ off2;
+    const int64_t* __restrict__ blk;
+    const uint32_t* __restrict__ sched;
+    const int64_t* __restrict__ goff;
+    const int32_t* __restrict__ need;
+    int32_t* ready;
+    double* W;
+    double* H;
+    const double* __restrict__ inv1;
+    const double* __restrict__ inv2;
+    unsigned long long* prof;
+    uint32_t w_bytes, ztail_b, dummy_b, i_bytes;
+    int d, ld, max_cols, unused;
+    double eps, lambda, tmin, tmax;
+};
+constexpr int GQ64 = 16;      // record prefetch distance = unroll of the group loop
+constexpr int GPREF64 = 8;    // W prefetch distance (groups)
+constexpr int RB64 = 512;     // bytes per staged row
+static_assert(GROUP_PAD % GQ64 == 0 && GQ64 % GPREF64 == 0 && GPREF64 <= GROUP_RING, "FP64 queues");
+
+__host__ __device__ inline int hbuf_bytes64(int max_cols) {
+    return ((max_cols + 1) * RB64 + (max_cols + 1) * 8 + 15) & ~15;
+}
+static size_t epoch_lds64(int max_cols) {
+    return (size_t)GROUP_RING * GROUP_SLOTS * RB64 + 2 * (size_t)hbuf_bytes64(max_cols) + 2 * sizeof(int32_t);
+}
+
+struct D4 {
+    double x, y, z, w;
+};
+__device__ __forceinline__ double dbl(uint32_t lo, uint32_t hi) { return __hiloint2double((int)hi, (int)lo); }
+__device__ __forceinline__ D4 d4_of(const uint4& a, const uint4& b) {
+    return D4{dbl(a.x, a.y), dbl(a.z, a.w), dbl(b.x, b.y), dbl(b.z, b.w)};
+}
+__device__ __forceinline__ uint4 lo2(double p, double q) {
+    return make_uint4((uint32_t)__double2loint(p), (uint32_t)__double2hiint(p), (uint32_t)__double2loint(q),
+                      (uint32_t)__double2hiint(q));
+}
+__device__ __forceinline__ D4 bload_d4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + 16), 0, 0);
+    return d4_of(make_uint4(a.x, a.y, a.z, a.w), make_uint4(b.x, b.y, b.z, b.w));
+}
+__device__ __forceinline__ void bstore_d4(__amdgpu_buffer_rsrc_t r, uint32_t off, const D4& v) {
+    const uint4 a = lo2(v.x, v.y), b = lo2(v.z, v.w);
+    u32x4 ua, ub;
+    ua.x = a.x; ua.y = a.y; ua.z = a.z; ua.w = a.w;
+    ub.x = b.x; ub.y = b.y; ub.z = b.z; ub.w = b.w;
+    __builtin_amdgcn_raw_buffer_store_b128(ua, r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(ub, r, (int)(off + 16), 0, 0);
+}
+__device__ __forceinline__ D4 ld_d4(const char* p) { return d4_of(*(const uint4*)p, *(const uint4*)(p + 16)); }
+__device__ __forceinline__ void st_d4(char* p, const D4& v) {
+    *(uint4*)p = lo2(v.x, v.y);
+    *(uint4*)(p + 16) = lo2(v.z, v.w);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+// row16_sum in double: the same four steps; each adds a lane pair in both
+// orders, so every lane of the row gets the bitwise identical total
+__device__ __forceinline__ double row16_sum_d(double v) {
+    v += dpp_d<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);   // row_half_mirror
+    v += dpp_d<0x140>(v);   // row_mirror
+    return v;
+}
+__device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+
+template <int KIND>
+__device__ __forceinline__ void grouped_mover_f64(const GArgs64& A, const int t, float* lds_f) {
+    constexpr int CH = 4;                                  // 32-byte chunks in flight per lane
+    char* const lds = (char*)lds_f;
+    const int lane = threadIdx.x & 63;
+    const int hb = hbuf_bytes64(A.max_cols);
+    char* const hbuf0 = lds + GROUP_RING * GROUP_SLOTS * RB64;
+    int32_t* const flags = (int32_t*)(hbuf0 + 2 * hb);     // [0] staged, [1] done
+    const double f2 = (A.eps * 2.) * A.lambda;
+    const int d = A.d;
+    unsigned long long c_df = 0;
+    int k = 0, pend_b2 = -1, pend_col0 = 0, pend_ncol = 0;
+    auto write_back = [&]() {   // visit k - 1, buffer (k - 1) & 1
+        const char* hbp = hbuf0 + ((k - 1) & 1) * hb;
+        const int total = pend_ncol * 16;
+        for (int base = 0; base < total; base += 64 * CH) {
+            D4 v[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e >> 4, q = e & 15;
+                v[u] = e < total ? ld_d4(hbp + c * RB64 + q * 32) : D4{0., 0., 0., 0.};
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e >> 4, q = e & 15;
+                if (e < total && 4 * q < A.ld) {
+                    uint4* dst = (uint4*)(A.H + (int64_t)(pend_col0 + c) * A.ld + 4 * q);
+                    dst[0] = lo2(v[u].x, v[u].y);
+                    dst[1] = lo2(v[u].z, v[u].w);
+                }
+            }
+        }
+        dataflow_signal_wave(A.ready, pend_b2);
+        pend_b2 = -1;
+    };
+    for (int s = 0; s < 2 * d; ++s) {
+        const int visit = s * d + t;
+        const int b2 = uniform_i32((int)A.sched[visit]);
+        const int b = t * 2 * d + b2;
+        if (A.blk[b + 1] == A.blk[b]) continue;
+        const int col0 = uniform_i32(A.off2[b2]);
+        const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
+        const int32_t need = uniform_i32(A.need[visit]);
+        const unsigned long long c0 = A.prof ? clock64() : 0;
+        while (need > 0 && __hip_atomic_load(A.ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            if (pend_b2 >= 0 && lds_reached(flags + 1, k - 1)) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                write_back();
+            } else {
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (A.prof) c_df += clock64() - c0;
+        char* const hbp = hbuf0 + (k & 1) * hb;
+        const int total = ncol * 16;
+        for (int base = 0; base < total; base += 64 * CH) {
+            D4 v[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e >> 4, q = e & 15;
+                if (e < total && 4 * q < A.ld) {
+                    const uint4* src = (const uint4*)(A.H + (int64_t)(col0 + c) * A.ld + 4 * q);
+                    v[u] = d4_of(src[0], src[1]);
+                } else {
+                    v[u] = D4{0., 0., 0., 0.};
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int e = base + u * 64 + lane, c = e >> 4, q = e & 15;
+                if (e < total) st_d4(hbp + c * RB64 + q * 32, v[u]);
+            }
+        }
+        if (KIND == UPD_NZSL_L2) {   // the column's factor f2 / nnz_j, the same product the update forms
+            double* const hinv = (double*)(hbp + (A.max_cols + 1) * RB64);
+            for (int c = lane; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
+        }
+        lds_publish(flags + 0, k);
+        if (pend_b2 >= 0) {
+            lds_await(flags + 1, k - 1);
+            write_back();
+        }
+        pend_b2 = b2;
+        pend_col0 = col0;
+        pend_ncol = ncol;
+        ++k;
+    }
+    if (pend_b2 >= 0) {
+        lds_await(flags + 1, k - 1);
+        write_back();
+    }
+    if (A.prof && lane == 0) A.prof[5 * t + 1] = c_df;
+}
+
+template <int KIND, bool TRUNC>
+__device__ __forceinline__ void grouped_compute_f64(const GArgs64& A, const int t, float* lds_f) {
+    char* const ring = (char*)lds_f;                       // GROUP_RING x 4 slots x RB64
+    const int hb = hbuf_bytes64(A.max_cols);
+    char* const hbuf0 = ring + GROUP_RING * GROUP_SLOTS * RB64;
+    int32_t* const flags = (int32_t*)(hbuf0 + 2 * hb);
+    const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
+    const int e0 = 4 * l16;
+    const bool pad = e0 >= A.ld;   // whole 4-element chunks: ld is a multiple of 4
+    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * 8u;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * 8u : (uint32_t)e0 * 8u;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * 8u : (uint32_t)e0 * 8u;
+    const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
+    const __amdgpu_buffer_rsrc_t Ir = make_rsrc(A.inv1, A.i_bytes);   // rows past m read 0
+    const uint32_t rec_off = 16u * slot;
+    char* const ring_lane = ring + slot * RB64 + e0 * 8;
+    const double eps = A.eps;
+    const double c1 = eps * -2.;
+    const double f2 = (eps * 2.) * A.lambda;
+    const int d = A.d;
+    for (int e = lane; e < GROUP_RING * GROUP_SLOTS * RB64 / 16; e += 64) ((uint4*)ring)[e] = make_uint4(0u, 0u, 0u, 0u);
+    unsigned long long c_wait = 0, c_loop = 0, n_groups = 0;
+    const unsigned long long c_begin = A.prof ? clock64() : 0;
+    D4 hprev{0., 0., 0., 0.};
+    int k = 0;
+    for (int s = 0; s < 2 * d; ++s) {
+        const int visit = s * d + t;
+        const int b2 = uniform_i32((int)A.sched[visit]);
+        const int b = t * 2 * d + b2;
+        if (A.blk[b + 1] == A.blk[b]) continue;
+        const int ng = uniform_i32(A.ngroups[visit]);
+        const int ngp = (ng + GQ64 - 1) / GQ64 * GQ64;   // the prep pads to GROUP_PAD, a multiple of GQ64
+        const int64_t gbase = uniform_i64(A.goff[visit]);
+        const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
+        const char* const hbp = hbuf0 + (k & 1) * hb;
+        char* const hblk_lane = (char*)hbp + e0 * 8;                         // + column * RB64
+        const char* const hinv_b = hbp + (A.max_cols + 1) * RB64;            // + column << 3
+        uint4 qa[GQ64];
+        D4 qb[GPREF64];
+        double qi[GPREF64];
+        auto load_w = [&](const uint4& rec, D4& out, double& inv) {
+            out = bload_d4(Wr, __umul24(rec.x, ldb) + lbase);
+            if (KIND == UPD_NZSL_L2) {
+                const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(Ir, (int)__umul24(rec.x, 8u), 0, 0);
+                inv = dbl(u.x, u.y);
+            } else {
+                inv = 0.;
+            }
+        };
+#pragma unroll
+        for (int u = 0; u < GQ64; ++u) qa[u] = load_rec(Gr, (int)rec_off + 64 * u);
+#pragma unroll
+        for (int u = 0; u < GPREF64; ++u) load_w(qa[u], qb[u], qi[u]);
+        const unsigned long long c0 = A.prof ? clock64() : 0;
+        lds_await(flags + 0, k);
+        const unsigned long long c2 = A.prof ? clock64() : 0;
+        D4 hq = ld_d4(hblk_lane + (qa[0].y << 9));
+        D4 wq = ld_d4(ring + (qa[0].x >> REC_RING_SHIFT) * RB64 + e0 * 8);
+        double ihq = KIND == UPD_NZSL_L2 ? *(const double*)(hinv_b + (qa[0].y << 3)) : 0.;
+        for (int K0 = 0; K0 < ngp; K0 += GQ64) {
+            const int rbase = (int)rec_off + 64 * (K0 + GQ64);
+#pragma unroll
+            for (int u = 0; u < GQ64; ++u) {
+                const int K = K0 + u;
+                const uint4 rec = qa[u];
+                D4 w = qb[u % GPREF64];
+                const double ai = qi[u % GPREF64];
+                D4 h = hq;
+                const D4 fwd = wq;
+                const double ih = ihq;
+                // operands of group K+1, read before group K writes (as in the FP32 loop)
+                const uint4 nrec = qa[(u + 1) % GQ64];
+                hq = ld_d4(hblk_lane + (nrec.y << 9));
+                wq = ld_d4(ring + (nrec.x >> REC_RING_SHIFT) * RB64 + e0 * 8);
+                if (KIND == UPD_NZSL_L2) ihq = *(const double*)(hinv_b + (nrec.y << 3));
+                const bool wf = (rec.x & REC_WFWD) != 0;
+                w.x = sel(wf, fwd.x, w.x); w.y = sel(wf, fwd.y, w.y); w.z = sel(wf, fwd.z, w.z); w.w = sel(wf, fwd.w, w.w);
+                const bool hr = (rec.y & REC_HREG) != 0;
+                h.x = sel(hr, hprev.x, h.x); h.y = sel(hr, hprev.y, h.y); h.z = sel(hr, hprev.z, h.z); h.w = sel(hr, hprev.w, h.w);
+                double part = w.x * h.x;
+                part = fma(w.y, h.y, part);
+                part = fma(w.z, h.z, part);
+                part = fma(w.w, h.w, part);
+                const double wh = row16_sum_d(part);
+                const double f1 = c1 * (dbl(rec.z, rec.w) - wh);
+                double a = 0., bb = 0.;
+                if (KIND == UPD_NZSL_L2) { a = f2 * ai; bb = ih; }
+                if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+                D4 wn, hn;
+                wn.x = fma(-f1, h.x, fma(-a, w.x, w.x)); wn.y = fma(-f1, h.y, fma(-a, w.y, w.y));
+                wn.z = fma(-f1, h.z, fma(-a, w.z, w.z)); wn.w = fma(-f1, h.w, fma(-a, w.w, w.w));
+                hn.x = fma(-f1, w.x, fma(-bb, h.x, h.x)); hn.y = fma(-f1, w.y, fma(-bb, h.y, h.y));
+                hn.z = fma(-f1, w.z, fma(-bb, h.z, h.z)); hn.w = fma(-f1, w.w, fma(-bb, h.w, h.w));
+                if (TRUNC) {   // update-truncate.h:50-64
+#define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
+                    TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(wn.z); TRUNC_(wn.w);
+                    TRUNC_(hn.x); TRUNC_(hn.y); TRUNC_(hn.z); TRUNC_(hn.w);
+#undef TRUNC_
+                }
+                hprev = hn;
+                bstore_d4(Wr, __umul24(rec.x, ldb) + sbase, wn);
+                st_d4(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB64), wn);
+                st_d4(hblk_lane + (rec.y << 9), hn);
+                load_w(qa[(u + GPREF64) % GQ64], qb[u % GPREF64], qi[u % GPREF64]);
+                qa[u] = load_rec(Gr, rbase + 64 * u);
+            }
+        }
+        lds_publish(flags + 1, k);
+        ++k;
+        if (A.prof) {
+            const unsigned long long c3 = clock64();
+            c_wait += c2 - c0;
+            c_loop += c3 - c2;
+            n_groups += (unsigned long long)ng;
+        }
+    }
+    if (A.prof && lane == 0) {
+        A.prof[5 * t + 0] = c_wait;
+        A.prof[5 * t + 2] = c_loop;
+        A.prof[5 * t + 3] = clock64() - c_begin;
+        A.prof[5 * t + 4] = n_groups;
+    }
+}
+
+template <int KIND, bool TRUNC>
+__global__ __launch_bounds__(128) void k_epoch_grouped64(GArgs64 A) {
+    extern __shared__ float lds[];
+    int32_t* const flags = (int32_t*)((char*)lds + GROUP_RING * GROUP_SLOTS * RB64 + 2 * hbuf_bytes64(A.max_cols));
+    if (threadIdx.x == 0) {
+        flags[0] = -1;
+        flags[1] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) grouped_compute_f64<KIND, TRUNC>(A, blockIdx.x, lds);
+    else grouped_mover_f64<KIND>(A, blockIdx.x, lds);
+}
+
 // Many problems in one persistent launch: workgroup -> (problem, task) by
 // `tmap`; each problem's arguments are read from device memory, so a captured
 // launch replays with new step sizes and plans.
@@ -739,21 +1068,24 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
         if (why) *why = m;
         return false;
     };
-    if (P.precision != PREC_FP32 && P.precision != PREC_BF16) return no("grouped kernel runs FP32/BF16 factors");
+    const bool f64 = P.precision == PREC_FP64;
+    if (P.precision == PREC_FP64_STRICT) return no("FP64_STRICT keeps the reference association (dataflow kernel)");
+    if (f64 && (P.ld > 64 || P.ld % 4 != 0)) return no("FP64 grouped kernel needs rank <= 64, rows padded to 4");
     if (P.ld > 128) return no("grouped kernel needs rank <= 128");
     if (P.ld > 64 && P.ld % 8 != 0) return no("rows above rank 64 must be padded to 8 elements");
     if (max_rows >= 65536 || max_cols >= 65536) return no("block dimensions above 65535");
     if (max_block >= (1 << 22)) return no("block above 2^22 points");
-    const int64_t es = P.precision == PREC_FP32 ? 4 : 2;
+    const int64_t es = f64 ? 8 : (P.precision == PREC_FP32 ? 4 : 2);
     if ((P.m + FACTOR_TAIL_ROWS) * P.ld * es >= ((int64_t)1 << 32) || (P.n + 1) * P.ld >= ((int64_t)1 << 31))
         return no("factor offsets above 2^32 bytes");
     if (P.m + FACTOR_TAIL_ROWS >= (1 << 24)) return no("2^24 rows or more (packed records)");
     int lmax = 0;
     if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return no("no device");
     if (prep_lds(max_block, max_rows, max_cols, max_block < PREP_NARROW) > (size_t)lmax) return no("prep LDS too large");
-    const size_t lds = epoch_lds(max_cols, vec_of(P.ld));
+    const size_t lds = f64 ? epoch_lds64(max_cols) : epoch_lds(max_cols, vec_of(P.ld));
     if (lds > (size_t)lmax) return no("H block does not fit in LDS");
-    const void* fn = vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
+    const void* fn = f64 ? (const void*)k_epoch_grouped64<UPD_NZSL_L2, false>
+                   : vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
                                        : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
     int per_cu = 0, cus = 0;
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
@@ -764,24 +1096,32 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     return true;
 }
 
+template <typename IX, typename PT>
+static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
+                          const int32_t* d_chain_start, int nchains, int order, int cap, size_t bytes, hipStream_t s) {
+    const void* fn = (const void*)k_prep_grouped<IX, PT>;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
+    k_prep_grouped<IX, PT><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const PT*)P.pts, P.perm,
+                                                      (uint4*)B.grec, B.ngroups, cap, B.max_rows, B.max_cols, drow,
+                                                      dcol, B.pprof);
+    HIPK(hipGetLastError());
+}
+
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s) {
     if (nchains <= 0) return;
     const int cap = (int)std::max<int64_t>(max_block, 1);
     const bool narrow = cap < PREP_NARROW;
     const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow);
-    const void* fn = narrow ? (const void*)k_prep_grouped<uint16_t> : (const void*)k_prep_grouped<uint32_t>;
-    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
-    if (narrow)
-        k_prep_grouped<uint16_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
-                                                            (uint4*)B.grec, B.ngroups, cap,
-                                                            B.max_rows, B.max_cols, drow, dcol, B.pprof);
-    else
-        k_prep_grouped<uint32_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const Pt32*)P.pts, P.perm,
-                                                            (uint4*)B.grec, B.ngroups, cap,
-                                                            B.max_rows, B.max_cols, drow, dcol, B.pprof);
-    HIPK(hipGetLastError());
+    const bool f64 = P.precision == PREC_FP64 || P.precision == PREC_FP64_STRICT;
+    if (f64) {
+        if (narrow) prep_launch_t<uint16_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
+        else prep_launch_t<uint32_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
+    } else {
+        if (narrow) prep_launch_t<uint16_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
+        else prep_launch_t<uint32_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
+    }
 }
 
 PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs, const int32_t* d_chain_start,
@@ -885,11 +1225,67 @@ static void epoch_grouped_p(const DevProblem& P, const UpdParams& U, const Group
     }
 }
 
+static GArgs64 make_gargs64(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L) {
+    GArgs64 A{};
+    A.grec = (const uint4*)B.grec;
+    A.ngroups = B.ngroups;
+    A.off2 = B.off2;
+    A.blk = P.blk;
+    A.sched = L.sched;
+    A.goff = L.goff;
+    A.need = L.need;
+    A.ready = L.ready;
+    A.W = (double*)P.W;
+    A.H = (double*)P.H;
+    A.inv1 = (const double*)P.inv1;
+    A.inv2 = (const double*)P.inv2;
+    A.w_bytes = (uint32_t)((P.m + FACTOR_TAIL_ROWS) * P.ld * 8);
+    A.ztail_b = (uint32_t)(P.m * P.ld * 8);
+    A.dummy_b = (uint32_t)((P.m + FACTOR_TAIL_ROWS / 2) * P.ld * 8);
+    A.i_bytes = (uint32_t)(P.m * 8);
+    A.d = P.d;
+    A.ld = (int)P.ld;
+    A.max_cols = B.max_cols;
+    A.eps = L.eps;
+    A.lambda = U.lambda;
+    A.tmin = U.tmin;
+    A.tmax = U.tmax;
+    A.prof = B.prof;
+    return A;
+}
+
+template <int KIND, bool TRUNC>
+static void epoch_grouped64_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                              const EpochLaunch& L, hipStream_t s) {
+    GArgs64 A = make_gargs64(P, U, B, L);
+    const size_t bytes = epoch_lds64(B.max_cols);
+    const void* fn = (const void*)k_epoch_grouped64<KIND, TRUNC>;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+    HIPK(hipGetLastError());
+    void* args[] = {&A};
+    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(128), args, bytes, s));
+}
+
+template <int KIND>
+static void epoch_grouped64_k(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
+                              const EpochLaunch& L, hipStream_t s) {
+    if (U.truncate) epoch_grouped64_t<KIND, true>(P, U, B, L, s);
+    else epoch_grouped64_t<KIND, false>(P, U, B, L, s);
+}
+
 void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                           const EpochLaunch& L, hipStream_t s) {
     if (P.precision == PREC_FP32) epoch_grouped_p<float>(P, U, B, L, s);
     else if (P.precision == PREC_BF16) epoch_grouped_p<bf16_t>(P, U, B, L, s);
-    else throw std::invalid_argument("grouped kernel runs FP32/BF16 factors");
+    else if (P.precision == PREC_FP64) {
+        switch (U.kind) {
+            case UPD_NZSL: epoch_grouped64_k<UPD_NZSL>(P, U, B, L, s); break;
+            case UPD_NZSL_L2: epoch_grouped64_k<UPD_NZSL_L2>(P, U, B, L, s); break;
+            case UPD_NZSL_NZL2: epoch_grouped64_k<UPD_NZSL_NZL2>(P, U, B, L, s); break;
+            default: throw std::invalid_argument("unknown update kind");
+        }
+    } else throw std::invalid_argument("grouped kernel runs FP64/FP32/BF16 factors (not FP64_STRICT)");
 }
 
 // ---------------------------------------------------------------------------

@@ -265,6 +265,41 @@ def test_grouped_bitexact_vs_emulation(precision, update, truncate):
         assert abs(lg - lo) <= 1e-10 * abs(lo)
 
 
+@pytest.mark.parametrize("rank,update,truncate", [(50, "Nzsl_L2(0.05)", None), (16, "Nzsl", None),
+                                                  (64, "Nzsl_Nzl2(0.02)", None), (37, "Nzsl_L2(0.05)", "(-0.4,0.4)")])
+def test_grouped_fp64_bitexact_vs_emulation(rank, update, truncate):
+    """FP64 grouped kernel against the oracle replaying its double association
+    (4-term FMA chains, the DPP row tree, fused updates) point by point in
+    visit order; the epoch loss also stays within 1e-9 of the reference
+    arithmetic's."""
+    print(f"case: grouped FP64 rank {rank} {update} {truncate}", flush=True)
+    p = skewed_problem(300, 200, 12000, rank, seed=31 + rank)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=43, precision="FP64", kernel=3)
+    _, ref = make_pair(p, update=update, truncate=truncate, tasks=4, seed=43)
+    o.set_arith("FP64_GROUPED")
+    for e in range(3):
+        st = g.epoch(0.01)
+        assert st["kernel"] == 3
+        o.epoch(0.01)
+        ref.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+        lg, lr = g.loss(0), ref.loss(0)
+        assert abs(lg - lr) <= 1e-9 * abs(lr), (e, lg, lr)
+
+
+def test_fp64_strict_stays_on_the_reference_association():
+    p = skewed_problem(100, 80, 2000, 8, seed=3)
+    D = _dsgdpp()
+    g = D.DsgdPp(update="Nzsl_L2(0.05)", loss="Nzsl_L2(0.05)", rank=8, tasks_per_rank=2, precision="FP64_STRICT")
+    g.load_coo(0, p["m"], p["n"], *p["train"])
+    ok, why = g.grouped_ok()
+    assert not ok and "FP64_STRICT" in why
+    g.close()
+
+
 @pytest.mark.parametrize("rank", [1, 13, 32, 64])
 @pytest.mark.parametrize("sgd,stratum", [("WOR", "COWOR"), ("WR", "WR"), ("SEQ", "SEQ")])
 def test_grouped_orders_ranks(rank, sgd, stratum):
