@@ -41,14 +41,16 @@ namespace dsgdpp {
 
 
 // packed record of one slot (uint4):
-//   .x  global row (bits 0-23) | REC_WFWD | ring entry << REC_RING_SHIFT
-//   .y  column index inside the column block | REC_HREG
+//   .x  global row (bits 0-23) | ring entry << REC_RING_SHIFT | REC_WFWD (sign bit)
+//   .y  column index inside the column block | REC_HREG (sign bit)
+// The flags sit in sign bits, so testing one is a single signed compare.
 //   .z  value x (float bits)
 //   .w  1/nnz of the row (float bits)
-constexpr uint32_t REC_WFWD = 1u << 24;   // W_i: ring entry (group % GROUP_RING) * 4 + slot of its last update (bit 24: the blend mask relies on it)
-constexpr int REC_RING_SHIFT = 25;        // 7 bits: GROUP_RING * GROUP_SLOTS <= 128 entries
-constexpr uint32_t REC_HREG = 1u << 30;   // in .y: H_j is the previous group's result in this slot (LDS offsets .y << 8, .y << 2 drop it)
-static_assert(GROUP_RING * GROUP_SLOTS <= (1 << (32 - REC_RING_SHIFT)), "ring entry field");
+constexpr uint32_t REC_WFWD = 1u << 31;   // W_i from ring entry (group % GROUP_RING) * 4 + slot of its last update
+constexpr int REC_RING_SHIFT = 24;        // 6 bits: GROUP_RING * GROUP_SLOTS <= 64 entries
+constexpr uint32_t REC_RING_MASK = 63u;
+constexpr uint32_t REC_HREG = 1u << 31;   // in .y: H_j is the previous group's result in this slot (LDS offsets .y << 8, .y << 2 drop it)
+static_assert(GROUP_RING * GROUP_SLOTS <= (int)REC_RING_MASK + 1, "ring entry field");
 
 __global__ void k_zero_i32(int32_t* p, int n) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
@@ -591,7 +593,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
             hq[v] = *(const float4*)(hblk_lane + (qa[0].y << RSH) + 16 * v);
-            wq[v] = *(const float4*)(ring + (qa[0].x >> REC_RING_SHIFT) * ROW + e0 + 4 * v);
+            wq[v] = *(const float4*)(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
         }
         float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
         for (int K0 = 0; K0 < ngp; K0 += GQ) {
@@ -613,7 +615,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                 // registers), and its ring entries were written by groups <= K-1
                 // (a row's points are at least two groups apart)
                 const uint4 nrec = qa[(u + 1) % GQ];
-                const float* const nsrc = ring + (nrec.x >> REC_RING_SHIFT) * ROW + e0;
+                const float* const nsrc = ring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     hq[v] = *(const float4*)(hblk_lane + (nrec.y << RSH) + 16 * v);
@@ -621,7 +623,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                 }
                 if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (nrec.y << 2));
                 {   // W_i from the ring where the prefetch may predate its last update
-                    const uint32_t msk = (uint32_t)((int32_t)(rec.x << (31 - 24)) >> 31);   // REC_WFWD
+                    const uint32_t msk = (uint32_t)((int32_t)rec.x >> 31);   // REC_WFWD
 #pragma unroll
                     for (int v = 0; v < VEC; ++v) {
                         w[v].x = __uint_as_float((__float_as_uint(fwd[v].x) & msk) | (__float_as_uint(w[v].x) & ~msk));
@@ -630,7 +632,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                         w[v].w = __uint_as_float((__float_as_uint(fwd[v].w) & msk) | (__float_as_uint(w[v].w) & ~msk));
                     }
                 }
-                const bool hr = (rec.y & REC_HREG) != 0;
+                const bool hr = (int32_t)rec.y < 0;   // REC_HREG
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     h[v].x = hr ? hprev[v].x : h[v].x; h[v].y = hr ? hprev[v].y : h[v].y;
@@ -960,7 +962,7 @@ __device__ __forceinline__ void grouped_compute_f64(const GArgs64& A, const int 
         lds_await(flags + 0, k);
         const unsigned long long c2 = A.prof ? clock64() : 0;
         D4 hq = ld_d4(hblk_lane + (qa[0].y << 9));
-        D4 wq = ld_d4(ring + (qa[0].x >> REC_RING_SHIFT) * RB64 + e0 * 8);
+        D4 wq = ld_d4(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
         double ihq = KIND == UPD_NZSL_L2 ? *(const double*)(hinv_b + (qa[0].y << 3)) : 0.;
         for (int K0 = 0; K0 < ngp; K0 += GQ64) {
             const int rbase = (int)rec_off + 64 * (K0 + GQ64);
@@ -976,11 +978,11 @@ __device__ __forceinline__ void grouped_compute_f64(const GArgs64& A, const int 
                 // operands of group K+1, read before group K writes (as in the FP32 loop)
                 const uint4 nrec = qa[(u + 1) % GQ64];
                 hq = ld_d4(hblk_lane + (nrec.y << 9));
-                wq = ld_d4(ring + (nrec.x >> REC_RING_SHIFT) * RB64 + e0 * 8);
+                wq = ld_d4(ring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
                 if (KIND == UPD_NZSL_L2) ihq = *(const double*)(hinv_b + (nrec.y << 3));
-                const bool wf = (rec.x & REC_WFWD) != 0;
+                const bool wf = (int32_t)rec.x < 0;   // REC_WFWD
                 w.x = sel(wf, fwd.x, w.x); w.y = sel(wf, fwd.y, w.y); w.z = sel(wf, fwd.z, w.z); w.w = sel(wf, fwd.w, w.w);
-                const bool hr = (rec.y & REC_HREG) != 0;
+                const bool hr = (int32_t)rec.y < 0;   // REC_HREG
                 h.x = sel(hr, hprev.x, h.x); h.y = sel(hr, hprev.y, h.y); h.z = sel(hr, hprev.z, h.z); h.w = sel(hr, hprev.w, h.w);
                 double part = w.x * h.x;
                 part = fma(w.y, h.y, part);
