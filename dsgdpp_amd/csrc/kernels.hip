@@ -256,6 +256,32 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // NZSL = sum (x - w_i . h_j)^2 (mf/loss/nzsl.h:30-53)
+// Four consecutive elements of a factor row, widened to double.
+template <typename T>
+struct St4;
+template <>
+struct St4<double> {
+    static __device__ __forceinline__ void wide(const double* p, double* o) {
+        const double2 a = *(const double2*)p, b = *(const double2*)(p + 2);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+};
+template <>
+struct St4<float> {
+    static __device__ __forceinline__ void wide(const float* p, double* o) {
+        const float4 a = *(const float4*)p;
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    }
+};
+template <>
+struct St4<bf16_t> {
+    static __device__ __forceinline__ void wide(const bf16_t* p, double* o) {
+        const uint2 a = *(const uint2*)p;
+        o[0] = __uint_as_float(a.x << 16); o[1] = __uint_as_float(a.x & 0xffff0000u);
+        o[2] = __uint_as_float(a.y << 16); o[3] = __uint_as_float(a.y & 0xffff0000u);
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_nzsl(const typename PtOf<T>::type* __restrict__ pts,
                                                       int64_t nnz, const T* __restrict__ W,
@@ -268,8 +294,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_nzsl(const typename PtOf<T>::ty
         const auto pt = pts[p];
         const T* w = W + (int64_t)pt.i * ld;
         const T* h = H + (int64_t)pt.j * ld;
+        // four elements per load (rows are padded to a multiple of 4 elements),
+        // summed one by one in element order as nzsl.h:30-53 does
         double ip = 0;
-        for (int z = 0; z < r; ++z) ip += St<T>::wide(w + z) * St<T>::wide(h + z);
+        for (int z0 = 0; z0 < r; z0 += 4) {
+            double wv[4], hv[4];
+            St4<T>::wide(w + z0, wv);
+            St4<T>::wide(h + z0, hv);
+            const int k = min(4, r - z0);
+            for (int e = 0; e < k; ++e) ip += wv[e] * hv[e];
+        }
         const double diff = (double)pt.x - ip;
         acc += diff * diff;
     }
