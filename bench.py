@@ -44,7 +44,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
-    ap.add_argument("--report", choices=["precision", "time-to-rmse"], default=None,
+    ap.add_argument("--report", choices=["precision", "time-to-rmse", "controller"], default=None,
                     help="instead of the throughput line: FP32/BF16 vs FP64 on identical sequences, or wall "
                          "time to the plain FP64 run's final test RMSE")
     ap.add_argument("--epochs", type=int, default=None)
@@ -57,6 +57,9 @@ def main():
         return
     if args.report == "time-to-rmse":
         report_time_to_rmse(args)
+        return
+    if args.report == "controller":
+        report_controller(args)
         return
     if args.workload == "batch64":
         bench_batch(args)
@@ -407,6 +410,61 @@ def report_time_to_rmse(args):
                                      "epochs": len(h64), "seconds_total": h64[-1][0], "final_test_rmse": h64[-1][1],
                                      "seconds_to_target": t64},
                       "target_test_rmse": target, **res}))
+
+
+def report_controller(args):
+    """The step-size controller at large scale (dsgdpp_amd.data.controller_250m:
+    1M x 625k, 250M training nonzeros, Zipf(1.1), rank 100, FP32,
+    Nzsl_L2(0.05)): plain BoldDriver(0.001) for `epochs` epochs sets the target
+    (its final test RMSE + 0.5%); the learned controller, started from the same
+    step, runs until it reaches the target.  Both evaluate training and test
+    loss after every epoch.  The column blocks are too wide for an LDS tile, so
+    epochs run on the dataflow kernel (d = 512 by default)."""
+    import torch
+    import dsgdpp_amd as D
+    from dsgdpp_amd import controller, data
+    epochs = args.epochs or 20
+    t0 = time.time()
+    p = data.controller_250m(device="cuda", shrink=args.shrink)
+    torch.cuda.synchronize()
+    log(f"[controller] generated {len(p['train'][2])} nonzeros in {time.time() - t0:.1f}s")
+    update, decay0, tasks = "Nzsl_L2(0.05)", 0.001, args.tasks or 512
+    nt = len(p["test"][2])
+
+    def run(name, rule, target=None):
+        t1 = time.time()
+        g = _ctx(D, p, update, tasks, "FP32", seed=12345)
+        log(f"[controller] {name}: context loaded in {time.time() - t1:.1f}s")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prev, cur = None, g.loss(D.TRAIN)
+        hist, kernel = [], None
+        for e in range(epochs):
+            st = g.epoch(rule(prev, cur))
+            kernel = st["kernel"]
+            prev, cur = cur, g.loss(D.TRAIN)
+            rmse = math.sqrt(g.loss(D.TEST) / nt)
+            hist.append((time.perf_counter() - t0, rmse))
+            log(f"[controller] {name} epoch {e + 1}: test rmse {rmse:.5f} at {hist[-1][0]:.1f}s")
+            if target is not None and rmse <= target:
+                break
+        g.close()
+        return hist, kernel
+
+    hb, kern = run("bolddriver", D.BoldDriver(decay0))
+    target = hb[-1][1] * 1.005
+    tb = next(t for t, r in hb if r <= target)
+    hl, _ = run("learned", controller.LearnedDecay(decay0), target)
+    hit = [t for t, r in hl if r <= target]
+    print(json.dumps({"report": "controller", "workload": f"controller_250m: {p['m']}x{p['n']}, "
+                      f"{len(p['train'][2])} train / {nt} test nonzeros, rank 100, {update}, FP32, start "
+                      f"eps {decay0}, {tasks}x{2 * tasks} blocks, kernel {kern}",
+                      "bolddriver": {"epochs": len(hb), "seconds_total": hb[-1][0], "final_test_rmse": hb[-1][1],
+                                     "seconds_to_target": tb},
+                      "target_test_rmse": target,
+                      "learned": {"epochs": len(hl), "seconds": hit[0] if hit else None,
+                                  "final_test_rmse": hl[-1][1], "reached": bool(hit),
+                                  "speedup": (tb / hit[0]) if hit else None}}))
 
 
 def bench_batch(args):

@@ -169,6 +169,16 @@ def zipf_100m(seed=2, device="cpu", shrink=1, rank=50):
                         s=1.0, truth_std=0.3, noise=0.8, seed=seed, device=device)
 
 
+def controller_250m(seed=5, device="cpu", shrink=1):
+    """The large Zipf shape of the step-size controller study: 1,000,000 x
+    625,000, 250M training + 2.5M test nonzeros, Zipf(1.1) degrees, values
+    clip(50 + 10 W*_i.H*_j + N(0, 5^2), 0, 100) (not rounded) with rank-30
+    N(0, 0.2^2) ground truth, rank 100."""
+    return zipf_problem(1_000_000 // shrink, 625_000 // shrink, 250_000_000 // shrink, 2_500_000 // shrink,
+                        rank=100, truth_rank=30, s=1.1, truth_std=0.2, noise=5.0, offset=50.0, scale=10.0,
+                        lo=0.0, hi=100.0, round_values=False, seed=seed, device=device)
+
+
 def quickstart_1m_r128(seed=1, device="cpu", shrink=1):
     p = quickstart_1m(seed=seed, device=device, shrink=shrink)
     rng = np.random.default_rng(seed + 100)
