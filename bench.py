@@ -416,9 +416,10 @@ def report_controller(args):
     """The step-size controller at large scale (dsgdpp_amd.data.controller_250m:
     1M x 625k, 250M training nonzeros, Zipf(1.1), rank 100, FP32,
     Nzsl_L2(0.05)): plain BoldDriver(0.001) for `epochs` epochs sets the target
-    (its final test RMSE + 0.5%); the learned controller, started from the same
-    step, runs until it reaches the target.  Both evaluate training and test
-    loss after every epoch.  The column blocks are too wide for an LDS tile, so
+    (its best test RMSE + 0.5%: with rank 100 and noise 5 the test RMSE turns up
+    after a few epochs); the learned controller, started from the same step,
+    runs as many epochs.  Both evaluate training and test loss after every
+    epoch; reported are the times to the target and each run's best RMSE.  The column blocks are too wide for an LDS tile, so
     epochs run on the dataflow kernel (d = 512 by default)."""
     import torch
     import dsgdpp_amd as D
@@ -452,19 +453,23 @@ def report_controller(args):
         return hist, kernel
 
     hb, kern = run("bolddriver", D.BoldDriver(decay0))
-    target = hb[-1][1] * 1.005
+    target = min(r for _, r in hb) * 1.005
     tb = next(t for t, r in hb if r <= target)
-    hl, _ = run("learned", controller.LearnedDecay(decay0), target)
+    hl, _ = run("learned", controller.LearnedDecay(decay0))
     hit = [t for t, r in hl if r <= target]
+
+    def best(h):
+        k = min(range(len(h)), key=lambda q: h[q][1])
+        return {"best_test_rmse": h[k][1], "best_epoch": k + 1, "seconds_at_best": h[k][0]}
+
     print(json.dumps({"report": "controller", "workload": f"controller_250m: {p['m']}x{p['n']}, "
                       f"{len(p['train'][2])} train / {nt} test nonzeros, rank 100, {update}, FP32, start "
-                      f"eps {decay0}, {tasks}x{2 * tasks} blocks, kernel {kern}",
-                      "bolddriver": {"epochs": len(hb), "seconds_total": hb[-1][0], "final_test_rmse": hb[-1][1],
-                                     "seconds_to_target": tb},
+                      f"eps {decay0}, {tasks}x{2 * tasks} blocks, kernel {kern}, {epochs} epochs each",
                       "target_test_rmse": target,
-                      "learned": {"epochs": len(hl), "seconds": hit[0] if hit else None,
-                                  "final_test_rmse": hl[-1][1], "reached": bool(hit),
-                                  "speedup": (tb / hit[0]) if hit else None}}))
+                      "bolddriver": {"seconds_to_target": tb, "seconds_total": hb[-1][0], **best(hb)},
+                      "learned": {"seconds_to_target": hit[0] if hit else None, "reached": bool(hit),
+                                  "seconds_total": hl[-1][0], **best(hl),
+                                  "speedup_to_target": (tb / hit[0]) if hit else None}}))
 
 
 def bench_batch(args):
