@@ -323,6 +323,7 @@ void configure(dsgdpp_ctx* c, const dsgdpp_cfg* cfg) {
     c->precision = cfg->precision;
     c->device = cfg->device;
     c->kernel = cfg->kernel;
+    if (c->kernel < 0 || c->kernel > 4) throw Error(DSGDPP_E_ARG, "kernel must be 0..4");
     master_key(c->seed, &c->master.k0, &c->master.k1);
     c->master.next = 0;
 }
@@ -458,7 +459,11 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
         for (size_t b = 0; b + 1 < c->off2.size(); ++b) maxCols = std::max<int>(maxCols, (int)(c->off2[b + 1] - c->off2[b]));
         c->G.max_rows = maxRows;
         c->G.max_cols = maxCols;
-        c->groupedOk = grouped_supported(c->P, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy);
+        bool wide = false;
+        c->groupedOk = grouped_supported(c->P, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
+                                         c->kernel == 4);
+        c->G.wide = wide;
+        if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
         if (c->groupedOk) {
             std::vector<int32_t> o2(c->off2.begin(), c->off2.end());
             int32_t* d2p = const_cast<int32_t*>(c->G.off2);
@@ -626,7 +631,7 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
             total += nb;
             gtotal += (2 * nb + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;   // groups < 2 points, padded
         }
-    P.grouped = c->groupedOk && (c->kernel == 0 || c->kernel == 3);
+    P.grouped = c->groupedOk && (c->kernel == 0 || c->kernel == 3 || c->kernel == 4);
     P.dataflow = P.grouped || (c->kernel != 1 && c->dataflowOk);
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
@@ -865,7 +870,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
         st->device_seconds = b / 1e3;
         st->updates = c->blocks[0].nnz;
         st->launches = P.dataflow ? 1 : d2;
-        st->kernel = P.grouped ? 3 : P.dataflow ? 2 : 1;
+        st->kernel = P.grouped ? (c->G.wide ? 4 : 3) : P.dataflow ? 2 : 1;
     }
 }
 
@@ -1610,6 +1615,9 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             if (c->precision != PREC_FP32 && c->precision != PREC_BF16)
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
                                                       ": batched epochs run FP32/BF16 factors (grouped kernel)");
+            if (c->G.wide)
+                throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
+                                                      ": its column blocks are too wide for the batched kernel's LDS tile");
             if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3))
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " cannot use the grouped kernel: " +
                                                       (c->groupedOk ? std::string("kernel forced") : c->groupedWhy));

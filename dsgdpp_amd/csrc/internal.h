@@ -106,6 +106,7 @@ struct GroupedBuffers {
     int max_rows = 0;             // tallest row block
     unsigned long long* prof = nullptr;   // optional kernel clock counters (5 per task)
     unsigned long long* pprof = nullptr;  // optional prep clock sums {visit order, scheduling, fill, visits}
+    bool wide = false;                    // H rows streamed from HBM (blocks too wide for the LDS tile)
 };
 // The prep kernel's inputs for one problem (many problems share one launch).
 struct PrepDesc {
@@ -119,8 +120,10 @@ struct PrepDesc {
     uint32_t dummy_row, dummy_col;
 };
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
+// grouped epoch eligibility; *wide (optional) is set when only the variant
+// that streams H rows from HBM fits (FP32 blocks too wide for the LDS tile)
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols,
-                       int device, std::string* why);
+                       int device, std::string* why, bool* wide = nullptr, bool force_wide = false);
 PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs, const int32_t* d_chain_start,
                    int order, int64_t max_block);
 bool prep_narrow(int64_t max_block);

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "device.h"
 #include "internal.h"
@@ -50,6 +51,11 @@ constexpr uint32_t REC_WFWD = 1u << 31;   // W_i from ring entry (group % GROUP_
 constexpr int REC_RING_SHIFT = 24;        // 6 bits: GROUP_RING * GROUP_SLOTS <= 64 entries
 constexpr uint32_t REC_RING_MASK = 63u;
 constexpr uint32_t REC_HREG = 1u << 31;   // in .y: H_j is the previous group's result in this slot (LDS offsets .y << 8, .y << 2 drop it)
+// Wide blocks (H rows streamed from HBM, no LDS tile): .y also carries the
+// H ring entry of the column's last update in the last GROUP_RING groups
+// (bits 24-29) and REC_HFWD; the column index stays in bits 0-15.
+constexpr uint32_t REC_HFWD = 1u << 30;
+constexpr int REC_HRING_SHIFT = 24;
 static_assert(GROUP_RING * GROUP_SLOTS <= (int)REC_RING_MASK + 1, "ring entry field");
 
 __global__ void k_zero_i32(int32_t* p, int n) {
@@ -100,7 +106,8 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                                            uint32_t* __restrict__ perm, uint4* __restrict__ grec,
                                            int32_t* __restrict__ ngroups, const int cap, const int rows_cap,
                                            const int cols_cap, const uint32_t dummy_row, const uint32_t dummy_col,
-                                           unsigned long long* __restrict__ pprof, uint32_t* sm) {
+                                           unsigned long long* __restrict__ pprof, uint32_t* sm,
+                                           const bool wide = false) {
     auto up4 = [](int k) { return (k * (int)sizeof(IX) + 3) >> 2; };   // words
     IX* vis = (IX*)sm;                               // visit order
     IX* lastR = (IX*)(sm + up4(cap));                // per local row: tag of its last update
@@ -196,6 +203,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             en = fetch(v + 64);
             uint32_t res = 0;    // G << 3 | hreg << 2 | slot
             uint32_t prevR = 0;  // the row's tag before this point
+            uint32_t prevC = 0;  // the column's
             const int cnt = min(64, n - v0);
             for (int l = 0; l < cnt; ++l) {
                 const uint32_t R = readlane_u32(r, l), C = readlane_u32(c, l);
@@ -232,6 +240,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 const bool me = lane == l;
                 res = me ? ((uint32_t)g << 3) | (hreg << 2) | sl : res;
                 prevR = me ? LR : prevR;
+                prevC = me ? LC : prevC;
             }
             if (v < n) {
                 lastR[r] = (IX)myR;   // lanes of one row hold the same final tag
@@ -244,7 +253,13 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 uint4 o;
                 o.x = (uint32_t)e.i |
                       (wf ? (REC_WFWD | (((((uint32_t)gR & (GROUP_RING - 1)) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
-                o.y = c | (((res >> 2) & 1u) ? REC_HREG : 0u);
+                const bool hreg = ((res >> 2) & 1u) != 0;
+                o.y = c | (hreg ? REC_HREG : 0u);
+                if (wide && !hreg) {   // H_j from the H ring when the column was updated in the last GROUP_RING groups
+                    const int gC = (int)(prevC >> 2) - 2;
+                    if (prevC != 0u && (int)g - gC < GROUP_RING)
+                        o.y |= REC_HFWD | (((((uint32_t)gC & (GROUP_RING - 1)) << 2) | (prevC & 3u)) << REC_HRING_SHIFT);
+                }
                 rec_value(e, o);
                 grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
             }
@@ -284,10 +299,10 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                                                      const PT* __restrict__ pts, uint32_t* __restrict__ perm,
                                                      uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
                                                      int rows_cap, int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
-                                                     unsigned long long* __restrict__ pprof) {
+                                                     unsigned long long* __restrict__ pprof, int wide) {
     extern __shared__ uint32_t sm[];
     prep_chain<IX, PT>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
-                       dummy_row, dummy_col, pprof, sm);
+                       dummy_row, dummy_col, pprof, sm, wide != 0);
 }
 
 // The chains of many problems in one launch: workgroup -> (problem, chain) by
@@ -1042,6 +1057,198 @@ __global__ __launch_bounds__(128) void k_epoch_grouped64(GArgs64 A) {
     else grouped_mover_f64<KIND>(A, blockIdx.x, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Wide column blocks (FP32): H rows streamed from HBM
+// ---------------------------------------------------------------------------
+// For column blocks whose H tile does not fit in LDS.  One wave per task.  H_j
+// is prefetched from HBM like W_i, GPREF_W groups ahead, together with the
+// column's 1/nnz, through a buffer resource over the visit's column block, so
+// padding lanes and the dummy column read zeros and their stores are dropped.
+// Where the prefetch may predate the column's last update it is forwarded from
+// an LDS ring of the last GROUP_RING groups' H results (REC_HFWD, read one
+// group ahead: the prep places a column's points at least two groups apart
+// unless they share a slot in consecutive groups, REC_HREG).  The task waits
+// for and signals its column blocks itself.
+constexpr int GQ_W = 16;       // record prefetch distance = unroll
+constexpr int GPREF_W = 8;     // W and H prefetch distance
+static_assert(GROUP_PAD % GQ_W == 0 && GQ_W % GPREF_W == 0 && GPREF_W <= GROUP_RING, "wide queues");
+constexpr uint32_t OOB_OFF = 0x7ffffff0u;   // past any buffer: loads read 0, stores are dropped
+
+static size_t epoch_lds_wide(int vec) { return (size_t)2 * GROUP_RING * GROUP_SLOTS * 64 * vec * sizeof(float); }
+
+template <int KIND, bool TRUNC, int VEC>
+__global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
+    extern __shared__ float lds[];
+    constexpr int ROW = 64 * VEC;
+    float* const wring = lds;
+    float* const hring = lds + GROUP_RING * GROUP_SLOTS * ROW;
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
+    const int e0 = 4 * VEC * l16;
+    const bool pad = e0 >= A.ld;
+    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * 4u;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * 4u : (uint32_t)e0 * 4u;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * 4u : (uint32_t)e0 * 4u;
+    const uint32_t hbase = pad ? OOB_OFF : (uint32_t)e0 * 4u;
+    const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
+    const uint32_t rec_off = 16u * slot;
+    float* const wring_lane = wring + slot * ROW + e0;
+    float* const hring_lane = hring + slot * ROW + e0;
+    const float eps = A.eps;
+    const float c1 = eps * -2.f;
+    const float f2 = (eps * 2.f) * A.lambda;
+    const int d = A.d;
+    for (int e = lane; e < 2 * GROUP_RING * GROUP_SLOTS * ROW; e += 64) lds[e] = 0.f;
+    unsigned long long c_wait = 0, c_loop = 0, n_groups = 0;
+    const unsigned long long c_begin = A.prof ? clock64() : 0;
+    float4 hprev[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) hprev[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = 0; s < 2 * d; ++s) {
+        const int visit = s * d + t;
+        const int b2 = uniform_i32((int)A.sched[visit]);
+        const int b = t * 2 * d + b2;
+        if (A.blk[b + 1] == A.blk[b]) continue;
+        const int ng = uniform_i32(A.ngroups[visit]);
+        const int ngp = (ng + GQ_W - 1) / GQ_W * GQ_W;
+        const int64_t gbase = uniform_i64(A.goff[visit]);
+        const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
+        const int col0 = uniform_i32(A.off2[b2]);
+        const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * 4u);
+        const __amdgpu_buffer_rsrc_t Ir = make_rsrc(A.inv2 + col0, (uint32_t)ncol * 4u);
+        uint4 qa[GQ_W];
+        float4 qw[GPREF_W][VEC], qh[GPREF_W][VEC];
+        float qi[GPREF_W];
+        auto load_w = [&](const uint4& rec, float4* out) {
+            const uint32_t o = __umul24(rec.x, ldb) + lbase;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) out[v] = Vec4<float>::bload(Wr, o + 16 * v);
+        };
+        auto load_h = [&](const uint4& rec, float4* out, float& inv) {
+            const uint32_t o = __umul24(rec.y, ldb) + hbase;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) out[v] = Vec4<float>::bload(Hr, o + 16 * v);
+            if (KIND == UPD_NZSL_L2)
+                inv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Ir, (int)__umul24(rec.y, 4u), 0, 0));
+            else
+                inv = 0.f;
+        };
+#pragma unroll
+        for (int u = 0; u < GQ_W; ++u) qa[u] = load_rec(Gr, (int)rec_off + 64 * u);
+#pragma unroll
+        for (int u = 0; u < GPREF_W; ++u) load_w(qa[u], qw[u]);
+        const unsigned long long c0 = A.prof ? clock64() : 0;
+        dataflow_wait(A.ready, b2, A.need[visit]);   // H rows of this column block may be read from here on
+        const unsigned long long c2 = A.prof ? clock64() : 0;
+#pragma unroll
+        for (int u = 0; u < GPREF_W; ++u) load_h(qa[u], qh[u], qi[u]);
+        float4 wq[VEC], hq[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            wq[v] = *(const float4*)(wring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
+            hq[v] = *(const float4*)(hring + ((qa[0].y >> REC_HRING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
+        }
+        for (int K0 = 0; K0 < ngp; K0 += GQ_W) {
+            const int rbase = (int)rec_off + 64 * (K0 + GQ_W);
+#pragma unroll
+            for (int u = 0; u < GQ_W; ++u) {
+                const int K = K0 + u;
+                const uint4 rec = qa[u];
+                float4 w[VEC], h[VEC], fw[VEC], fh[VEC];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    w[v] = qw[u % GPREF_W][v];
+                    h[v] = qh[u % GPREF_W][v];
+                    fw[v] = wq[v];
+                    fh[v] = hq[v];
+                }
+                const float icol = qi[u % GPREF_W];
+                // ring operands of group K+1 (written by groups <= K-1)
+                const uint4 nrec = qa[(u + 1) % GQ_W];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    wq[v] = *(const float4*)(wring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
+                    hq[v] = *(const float4*)(hring + ((nrec.y >> REC_HRING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
+                }
+                const bool wf = (int32_t)rec.x < 0;          // REC_WFWD
+                const bool hf = (rec.y & REC_HFWD) != 0;
+                const bool hr = (int32_t)rec.y < 0;          // REC_HREG
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    w[v].x = wf ? fw[v].x : w[v].x; w[v].y = wf ? fw[v].y : w[v].y;
+                    w[v].z = wf ? fw[v].z : w[v].z; w[v].w = wf ? fw[v].w : w[v].w;
+                    h[v].x = hr ? hprev[v].x : (hf ? fh[v].x : h[v].x);
+                    h[v].y = hr ? hprev[v].y : (hf ? fh[v].y : h[v].y);
+                    h[v].z = hr ? hprev[v].z : (hf ? fh[v].z : h[v].z);
+                    h[v].w = hr ? hprev[v].w : (hf ? fh[v].w : h[v].w);
+                }
+                float part = w[0].x * h[0].x;
+                part = fmaf(w[0].y, h[0].y, part);
+                part = fmaf(w[0].z, h[0].z, part);
+                part = fmaf(w[0].w, h[0].w, part);
+#pragma unroll
+                for (int v = 1; v < VEC; ++v) {
+                    part = fmaf(w[v].x, h[v].x, part);
+                    part = fmaf(w[v].y, h[v].y, part);
+                    part = fmaf(w[v].z, h[v].z, part);
+                    part = fmaf(w[v].w, h[v].w, part);
+                }
+                const float wh = row16_sum(part);
+                const float f1 = c1 * (__uint_as_float(rec.z) - wh);
+                float a = 0.f, bb = 0.f;
+                if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = f2 * icol; }
+                if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+                float4 wn[VEC], hn[VEC];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    wn[v].x = fmaf(-f1, h[v].x, fmaf(-a, w[v].x, w[v].x));
+                    wn[v].y = fmaf(-f1, h[v].y, fmaf(-a, w[v].y, w[v].y));
+                    wn[v].z = fmaf(-f1, h[v].z, fmaf(-a, w[v].z, w[v].z));
+                    wn[v].w = fmaf(-f1, h[v].w, fmaf(-a, w[v].w, w[v].w));
+                    hn[v].x = fmaf(-f1, w[v].x, fmaf(-bb, h[v].x, h[v].x));
+                    hn[v].y = fmaf(-f1, w[v].y, fmaf(-bb, h[v].y, h[v].y));
+                    hn[v].z = fmaf(-f1, w[v].z, fmaf(-bb, h[v].z, h[v].z));
+                    hn[v].w = fmaf(-f1, w[v].w, fmaf(-bb, h[v].w, h[v].w));
+                    if (TRUNC) {   // update-truncate.h:50-64
+#define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
+                        TRUNC_(wn[v].x); TRUNC_(wn[v].y); TRUNC_(wn[v].z); TRUNC_(wn[v].w);
+                        TRUNC_(hn[v].x); TRUNC_(hn[v].y); TRUNC_(hn[v].z); TRUNC_(hn[v].w);
+#undef TRUNC_
+                    }
+                    hprev[v] = hn[v];
+                }
+                const uint32_t so = __umul24(rec.x, ldb) + sbase;
+                const uint32_t sh = __umul24(rec.y, ldb) + hbase;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    Vec4<float>::bstore(Wr, so + 16 * v, wn[v]);
+                    Vec4<float>::bstore(Hr, sh + 16 * v, hn[v]);
+                    *(float4*)(wring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = wn[v];
+                    *(float4*)(hring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = hn[v];
+                }
+                load_w(qa[(u + GPREF_W) % GQ_W], qw[u % GPREF_W]);
+                load_h(qa[(u + GPREF_W) % GQ_W], qh[u % GPREF_W], qi[u % GPREF_W]);
+                qa[u] = load_rec(Gr, rbase + 64 * u);
+            }
+        }
+        if (A.prof) {
+            const unsigned long long c3 = clock64();
+            c_wait += c2 - c0;
+            c_loop += c3 - c2;
+            n_groups += (unsigned long long)ng;
+        }
+        dataflow_signal_wave(A.ready, b2);
+    }
+    if (A.prof && lane == 0) {
+        A.prof[5 * t + 0] = c_wait;
+        A.prof[5 * t + 1] = 0;
+        A.prof[5 * t + 2] = c_loop;
+        A.prof[5 * t + 3] = clock64() - c_begin;
+        A.prof[5 * t + 4] = n_groups;
+    }
+}
+
 // Many problems in one persistent launch: workgroup -> (problem, task) by
 // `tmap`; each problem's arguments are read from device memory, so a captured
 // launch replays with new step sizes and plans.
@@ -1065,11 +1272,12 @@ __global__ __launch_bounds__(128) void k_epoch_grouped_batch(const GArgs<T>* __r
 static int vec_of(int64_t ld) { return ld <= 64 ? 1 : 2; }
 
 bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols, int device,
-                       std::string* why) {
+                       std::string* why, bool* wide, bool force_wide) {
     auto no = [&](const char* m) {
         if (why) *why = m;
         return false;
     };
+    if (wide) *wide = false;
     const bool f64 = P.precision == PREC_FP64;
     if (P.precision == PREC_FP64_STRICT) return no("FP64_STRICT keeps the reference association (dataflow kernel)");
     if (f64 && (P.ld > 64 || P.ld % 4 != 0)) return no("FP64 grouped kernel needs rank <= 64, rows padded to 4");
@@ -1084,18 +1292,31 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     int lmax = 0;
     if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return no("no device");
     if (prep_lds(max_block, max_rows, max_cols, max_block < PREP_NARROW) > (size_t)lmax) return no("prep LDS too large");
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return no("no device");
+    auto resident = [&](const void* fn, int threads, size_t bytes) {
+        int per_cu = 0;
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return false;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, bytes) != hipSuccess) return false;
+        return (int64_t)per_cu * cus >= P.d;
+    };
     const size_t lds = f64 ? epoch_lds64(max_cols) : epoch_lds(max_cols, vec_of(P.ld));
-    if (lds > (size_t)lmax) return no("H block does not fit in LDS");
     const void* fn = f64 ? (const void*)k_epoch_grouped64<UPD_NZSL_L2, false>
                    : vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
                                        : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
-    int per_cu = 0, cus = 0;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return no("LDS attribute");
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 128, lds) != hipSuccess) return no("occupancy query");
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return no("no device");
-    if ((int64_t)per_cu * cus < P.d) return no("not all tasks can be resident");
-    return true;
+    if (!force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
+    // FP32 blocks too wide for the LDS tile: H rows streamed from HBM
+    if (P.precision == PREC_FP32 && wide) {
+        const size_t lw = epoch_lds_wide(vec_of(P.ld));
+        const void* fw = vec_of(P.ld) == 1 ? (const void*)k_epoch_wide<UPD_NZSL_L2, false, 1>
+                                           : (const void*)k_epoch_wide<UPD_NZSL_L2, false, 2>;
+        if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
+            *wide = true;
+            return true;
+        }
+    }
+    if (lds > (size_t)lmax) return no("H block does not fit in LDS");
+    return no("not all tasks can be resident");
 }
 
 template <typename IX, typename PT>
@@ -1106,7 +1327,7 @@ static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const Pe
     const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
     k_prep_grouped<IX, PT><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const PT*)P.pts, P.perm,
                                                       (uint4*)B.grec, B.ngroups, cap, B.max_rows, B.max_cols, drow,
-                                                      dcol, B.pprof);
+                                                      dcol, B.pprof, B.wide ? 1 : 0);
     HIPK(hipGetLastError());
 }
 
@@ -1202,9 +1423,29 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(128), args, bytes, s));
 }
 
+template <int KIND, bool TRUNC, int VEC>
+static void epoch_wide_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
+                         hipStream_t s) {
+    GArgs<float> A = make_gargs<float>(P, U, B, L);
+    const size_t bytes = epoch_lds_wide(VEC);
+    const void* fn = (const void*)k_epoch_wide<KIND, TRUNC, VEC>;
+    HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+    HIPK(hipGetLastError());
+    void* args[] = {&A};
+    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(64), args, bytes, s));
+}
+
 template <typename T, int KIND, bool TRUNC>
 static void epoch_grouped_v(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                             const EpochLaunch& L, hipStream_t s) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (B.wide) {
+            if (vec_of(P.ld) == 1) epoch_wide_t<KIND, TRUNC, 1>(P, U, B, L, s);
+            else epoch_wide_t<KIND, TRUNC, 2>(P, U, B, L, s);
+            return;
+        }
+    }
     if (vec_of(P.ld) == 1) epoch_grouped_t<T, KIND, TRUNC, 1>(P, U, B, L, s);
     else epoch_grouped_t<T, KIND, TRUNC, 2>(P, U, B, L, s);
 }

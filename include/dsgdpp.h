@@ -77,7 +77,9 @@ typedef struct dsgdpp_cfg {
     int32_t kernel;          /* 0 = auto (grouped dataflow when eligible, else dataflow);
                                 1 = one launch per stratum, one wave per block, sequential;
                                 2 = dataflow (one persistent launch per epoch), sequential blocks;
-                                3 = grouped dataflow (FP32/BF16, rank <= 64) */
+                                3 = grouped dataflow (FP32/BF16 up to rank 128, FP64 up to rank 64);
+                                4 = grouped dataflow with H rows streamed from HBM (FP32; chosen by
+                                    0 and 3 when a column block is too wide for the LDS tile) */
     int32_t reserved[5];
 } dsgdpp_cfg;
 
@@ -88,7 +90,7 @@ typedef struct dsgdpp_epoch_stats {
     double sgd_seconds;      /* HIP-event time of the stratum kernels */
     int64_t updates;         /* points visited */
     int64_t launches;        /* stratum kernel launches */
-    int32_t kernel;          /* epoch kernel used: 1 per-stratum, 2 dataflow, 3 grouped dataflow */
+    int32_t kernel;          /* epoch kernel used: 1 per-stratum, 2 dataflow, 3 grouped dataflow, 4 grouped with H from HBM */
     int32_t reserved;
 } dsgdpp_epoch_stats;
 

@@ -290,6 +290,51 @@ def test_grouped_fp64_bitexact_vs_emulation(rank, update, truncate):
         assert abs(lg - lr) <= 1e-9 * abs(lr), (e, lg, lr)
 
 
+@pytest.mark.parametrize("rank,update,truncate", [(50, "Nzsl_L2(0.05)", None), (100, "Nzsl_L2(0.05)", None),
+                                                  (16, "Nzsl", None), (72, "Nzsl_Nzl2(0.02)", "(-0.4,0.4)")])
+def test_grouped_wide_bitexact_vs_emulation(rank, update, truncate):
+    """The grouped kernel with H rows streamed from HBM (kernel 4: H
+    prefetch, H forwarding ring, no LDS tile) against the oracle's FP32
+    emulation, bitwise: a missed H forwarding shows up as a bit difference."""
+    print(f"case: grouped wide rank {rank} {update} {truncate}", flush=True)
+    p = skewed_problem(300, 200, 12000, rank, seed=77 + rank)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=45, precision="FP32", kernel=4)
+    o.set_arith("FP32")
+    for e in range(3):
+        st = g.epoch(0.01)
+        assert st["kernel"] == 4
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+
+
+def test_wide_column_blocks_choose_the_streaming_kernel():
+    """A 600 x 4,000 problem at d = 2: two LDS tiles of 1,000-column blocks
+    (545 KB) cannot be staged, so the automatic choice is the variant that
+    streams H rows."""
+    print("case: wide blocks choose kernel 4", flush=True)
+    rng = np.random.default_rng(11)
+    m, n, nnz, rank = 600, 4_000, 20_000, 32
+    cells = rng.choice(m * n, size=nnz, replace=False)
+    i = (cells // n).astype(np.int32)
+    j = (cells % n).astype(np.int32)
+    x = rng.normal(size=nnz)
+    p = dict(m=m, n=n, train=(i, j, x), rank=rank, W0=rng.uniform(-0.5, 0.5, (m, rank)),
+             H0=rng.uniform(-0.5, 0.5, (rank, n)))
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=2, seed=3, precision="FP32")
+    o.set_arith("FP32")
+    assert g.grouped_ok()[0]
+    for e in range(2):
+        assert g.epoch(0.01)["kernel"] == 4
+        o.epoch(0.01)
+        np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"epoch {e}")
+    from dsgdpp_amd import batch as B
+    with pytest.raises(RuntimeError, match="too wide"):
+        B.DsgdPpBatch([g])
+
+
 def test_fp64_strict_stays_on_the_reference_association():
     p = skewed_problem(100, 80, 2000, 8, seed=3)
     D = _dsgdpp()
