@@ -107,7 +107,7 @@ def measure_rates(precision="FP32", update="Nzsl_L2(0.05)", device=0, cache=CACH
             g.epoch(1e-4)   # plans and caches warm
             st = g.epoch(1e-4)
             g.close()
-            if st["kernel"] != 3:
+            if st["kernel"] not in (3, 4):
                 raise RuntimeError(f"autotune probe rank {r} d {d} did not run the grouped kernel")
             rates[r][d] = len(p["train"][2]) / st["sgd_seconds"] / d
             if log:
@@ -128,9 +128,13 @@ def row_floats(rank):
     return 64 if rank <= 64 else 128
 
 
-def lds_need(n, d, rank):
-    """LDS of one task (bytes): W ring, two H tiles of ceil(n / 2d) columns
-    plus the spare column and the columns' factors, two flags."""
+def lds_need(n, d, rank, wide=False):
+    """LDS of one task (bytes).  Tiled: W ring, two H tiles of ceil(n / 2d)
+    columns plus the spare column and the columns' factors, two flags.  Wide
+    (FP32 column blocks too wide for the tile, H rows streamed from HBM): the
+    W and H rings only."""
+    if wide:
+        return 2 * RING_ROWS * row_floats(rank) * 4
     cols = -(-n // (2 * d))
     tile = ((cols + 1) * row_floats(rank) + cols + 1 + 3) // 4 * 4
     return (RING_ROWS * row_floats(rank) + 2 * tile) * 4 + 8
@@ -140,27 +144,43 @@ def prep_lds_estimate(m, n, nnz, d):
     """LDS of the prep kernel for blocks of about 1.5 nnz / 2d^2 points (the
     kernel keeps the visit order and row/column tags, 16-bit below 8,000
     points, plus one byte per group, fewer than two groups per point)."""
-    cap = int(1.5 * nnz / (2 * d * d)) + 1
-    es = 2 if cap < 8000 else 4
+    cap = block_cap(nnz, d)
+    es = 2 if cap < PREP_NARROW else 4
     return (cap + -(-m // d) + -(-n // (2 * d))) * es + 2 * cap + 64
 
 
-def min_tasks(p):
-    """Smallest power-of-two d whose H tile and prep state fit in LDS."""
+PREP_NARROW = 8000   # blocks below this many points take the 16-bit prep
+
+
+def block_cap(nnz, d):
+    """Capacity the planner assumes for the largest of the 2d^2 blocks."""
+    return int(1.5 * nnz / (2 * d * d)) + 1
+
+
+def min_tasks(p, wide=True):
+    """Smallest power-of-two d whose prep state fits in LDS and whose H tiles
+    fit too, unless the kernel may stream H rows (`wide`, FP32).  Blocks are
+    further kept below PREP_NARROW points while the shape allows (d <= m,
+    2d <= n): the next epoch's prep runs beside the epoch launches, and the
+    32-bit prep of larger blocks holds a CU's LDS with one wave, at several
+    times the cost per point."""
     d = 1
-    while (lds_need(p["n"], d, p["rank"]) > LDS_BYTES
-           or prep_lds_estimate(p["m"], p["n"], p["nnz"], d) > LDS_BYTES):
+    while ((not wide and lds_need(p["n"], d, p["rank"]) > LDS_BYTES)
+           or prep_lds_estimate(p["m"], p["n"], p["nnz"], d) > LDS_BYTES
+           or (block_cap(p["nnz"], d) >= PREP_NARROW and 2 * d <= p["m"] and 4 * d <= p["n"])):
         d *= 2
     return d
 
 
-def plan_tasks(problems, budget, rates):
+def plan_tasks(problems, budget, rates, wide=True):
     """Tasks per problem.  problems: list of dicts with m, n, nnz, rank.
-    Greedy: start at the smallest d whose H tile fits in LDS; double d of the
-    problem with the largest predicted epoch time nnz / (d * rate(d)) while
+    `wide`: the problems are FP32, so column blocks too wide for the LDS tile
+    may run with H rows streamed from HBM (BF16 needs the tile).
+    Greedy: start at the smallest d the LDS allows (min_tasks); double d of
+    the problem with the largest predicted epoch time nnz / (d * rate(d)) while
     the total stays within `budget` and the problem has at least d rows and
     2d columns per block."""
-    d = [min_tasks(p) for p in problems]
+    d = [min_tasks(p, wide) for p in problems]
 
     def pred(k, dk):
         r = rates[rank_class(problems[k]["rank"])]

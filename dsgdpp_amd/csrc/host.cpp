@@ -1615,10 +1615,7 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             if (c->precision != PREC_FP32 && c->precision != PREC_BF16)
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
                                                       ": batched epochs run FP32/BF16 factors (grouped kernel)");
-            if (c->G.wide)
-                throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
-                                                      ": its column blocks are too wide for the batched kernel's LDS tile");
-            if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3))
+            if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3 || c->kernel == 4))
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " cannot use the grouped kernel: " +
                                                       (c->groupedOk ? std::string("kernel forced") : c->groupedWhy));
             if (k == 0) {
@@ -1639,7 +1636,7 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
         for (int k = 0; k < count; ++k) {
             dsgdpp_ctx* c = b->ctx[k];
             for (int t = 0; t < (int)c->d; ++t) tmap.push_back(make_int2(k, t));
-            b->lds = std::max(b->lds, batch_epoch_lds(c->G.max_cols, c->P.ld));
+            b->lds = std::max(b->lds, batch_epoch_lds(c->G.max_cols, c->P.ld, c->G.wide));
             if (!c->dReady) dalloc(c->dReady, (size_t)2 * c->d * sizeof(int32_t));
             b->ready.push_back(c->dReady);
             b->dsz.push_back((int)c->d);

@@ -118,6 +118,7 @@ struct PrepDesc {
     int32_t* ngroups;
     int32_t order, cap, rows_cap, cols_cap;
     uint32_t dummy_row, dummy_col;
+    int32_t wide, unused;   // wide: H forwarding entries for the streaming kernel
 };
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 // grouped epoch eligibility; *wide (optional) is set when only the variant
@@ -140,7 +141,7 @@ void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const Grouped
 size_t grouped_args_bytes(int precision);
 void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
                        void* out);
-size_t batch_epoch_lds(int max_cols, int64_t max_ld);
+size_t batch_epoch_lds(int max_cols, int64_t max_ld, bool wide);
 int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device);
 void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
                         size_t lds, int32_t* const* ready, const int* dsz, int nprob, hipStream_t s);

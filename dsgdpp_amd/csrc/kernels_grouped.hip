@@ -70,9 +70,16 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// old with lane `lane` (uniform) replaced by v
+// old with lane `lane` (uniform) replaced by v (uniform): v_writelane_b32
+__device__ int amdgcn_writelane(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 __device__ __forceinline__ uint32_t writelane_u32(uint32_t v, int lane, uint32_t old) {
-    return (int)threadIdx.x == lane ? v : old;
+    return (uint32_t)amdgcn_writelane((int)v, lane, (int)old);
+}
+// x rotated right by k in [0, 63]
+__device__ __forceinline__ uint64_t rotr64(uint64_t x, int k) { return (x >> k) | (x << ((64 - k) & 63)); }
+// v from lane `src` in [0, 63] (per lane)
+__device__ __forceinline__ uint32_t lane_gather(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
 }
 
 // ---------------------------------------------------------------------------
@@ -127,7 +134,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             for (int k = lane; k < n; k += 64)
                 vis[k] = (IX)bounded(stream_draw(J.k0, J.k1, J.draw + k), (uint64_t)n);
             __syncthreads();
-        } else {                      // WOR: Fisher-Yates, 64 steps at a time in registers
+        } else {                      // WOR: Fisher-Yates, 64 steps at a time
             if (J.carry_base >= 0) {
                 for (int k = lane; k < n; k += 64) vis[k] = (IX)perm[J.carry_base + k];
             } else {
@@ -135,31 +142,37 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             }
             __syncthreads();
             for (int base = 0; base < n - 1; base += 64) {
-                // lane l owns position base + l (cur) and, when step base + l swaps
-                // with a position past this chunk, the current value there (pre)
+                // Step s swaps positions s and t_s >= s; the 64 steps of a chunk
+                // resolve at once.  The value step s moves out of s (a_s) is the
+                // one the last earlier step aimed at s moved in, else the chunk's
+                // entry value at s: follow those links to a step nobody aimed at.
+                // Position s keeps the value at t_s before step s: a_q of the last
+                // earlier step q aimed at t_s, else t_s's entry value.  A position
+                // past the chunk receives a_s of the last step aimed at it.
                 const int s = base + lane, hi = base + 64;
                 int t = s;
                 if (s < n - 1) t = s + (int)bounded(stream_draw(J.k0, J.k1, J.draw + s), (uint64_t)(n - s));
                 const bool outside = t >= hi;
-                uint32_t cur = s < n ? (uint32_t)vis[s] : 0u;
-                uint32_t pre = outside ? (uint32_t)vis[t] : 0u;
-                const int cnt = min(64, n - 1 - base);
-                for (int l = 0; l < cnt; ++l) {
-                    const int T = (int)readlane_u32((uint32_t)t, l);
-                    const uint32_t a = readlane_u32(cur, l);
-                    if (T < hi) {
-                        const int tl = T - base;
-                        const uint32_t b = readlane_u32(cur, tl);
-                        cur = writelane_u32(b, l, cur);
-                        cur = writelane_u32(a, tl, cur);
-                    } else {
-                        const uint32_t b = readlane_u32(pre, l);
-                        cur = writelane_u32(b, l, cur);
-                        pre = t == T ? a : pre;   // every lane aimed at T sees its new value
-                    }
+                const uint32_t cur = s < n ? (uint32_t)vis[s] : 0u;
+                const uint32_t pre = outside ? (uint32_t)vis[t] : 0u;
+                int f = lane, q = -1;   // f: the last earlier step aimed at s (the lane itself if none)
+                bool last = true;       // no later step aimed at t_s
+#pragma unroll
+                for (int k = 0; k < 64; ++k) {
+                    const int T = __builtin_amdgcn_readlane(t, k);
+                    const bool earlier = k < lane;
+                    f = earlier && T == s ? k : f;
+                    q = earlier && T == t ? k : q;
+                    last = k > lane && T == t ? false : last;
                 }
-                if (s < n) vis[s] = (IX)cur;
-                if (outside) vis[t] = (IX)pre;   // lanes aimed at the same position store the same value
+#pragma unroll
+                for (int r = 0; r < 6; ++r) f = (int)lane_gather((uint32_t)f, f);   // links cover < 64 steps
+                const uint32_t a = lane_gather(cur, f);
+                const uint32_t aq = lane_gather(a, q < 0 ? 0 : q);
+                const uint32_t ct = lane_gather(cur, (t - base) & 63);
+                const uint32_t keep = t == s ? a : q >= 0 ? aq : t < hi ? ct : pre;
+                if (s < n) vis[s] = (IX)keep;
+                if (outside && last) vis[t] = (IX)a;
                 __syncthreads();
             }
         }
@@ -176,11 +189,15 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         //    for an untouched column.  Row and column tags sit in the chunk's
         //    lanes; every placement updates all lanes that share the row or
         //    column.  The slot masks of groups maxG-61 .. maxG sit in lanes (group
-        //    G in lane G % 64) and the lanes of maxG+1 and maxG+2 are kept clear:
-        //    a point starts at most two groups past maxG, and its search ends at
-        //    the first clear lane.
-        int maxG = 1, lo = 2;
-        uint32_t win = 0;
+        //    G in lane G % 64) and the groups maxG+1 and maxG+2 are kept empty: a
+        //    point starts at most two groups past maxG, and its search ends at the
+        //    first group with a free slot.  A 64-bit scalar mask marks the full
+        //    groups of the window, so the search is one rotate and find-first-zero.
+        //    A group that left the window is never searched: a point starts at
+        //    maxG - 61 or later.
+        int maxG = 1;
+        uint32_t win = 0;    // lane b: G << 4 | slot mask of the window's group G in lane b
+        uint64_t full = 0;   // bit b: that group has no free slot
         // the next chunk's points are fetched while the current chunk is scanned
         // (a training point carries 1/nnz of its row in .aux)
         auto fetch = [&](int v) {
@@ -208,39 +225,41 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             for (int l = 0; l < cnt; ++l) {
                 const uint32_t R = readlane_u32(r, l), C = readlane_u32(c, l);
                 const uint32_t LR = readlane_u32(myR, l), LC = readlane_u32(myC, l);
-                const int gH = (int)(LC >> 2) + 1;                 // H_j from registers in this group
-                const uint32_t only = 15u ^ (1u << (LC & 3u));     // the other slots
-                // two groups after the row's last update: the epoch kernel reads a
-                // forwarded W row from the ring one group ahead
-                int g = max(max((int)(LR >> 2) + 2, (int)(LC >> 2) + 1), lo);
-                uint32_t mask, em;
-                for (;;) {
-                    mask = readlane_u32(win, g & 63);
-                    em = g == gH ? (mask | only) : mask;
-                    if ((em & 15u) != 15u) break;
-                    ++g;
-                }
-                const uint32_t hreg = g == gH ? 1u : 0u;
-                const uint32_t sl = (uint32_t)__builtin_ctz(~em & 15u);
-                if (g > maxG) {   // g <= maxG + 2
-                    maxG = g;
-                    lo = max(lo, g - 61);
-                    win = (lane == ((g + 1) & 63) || lane == ((g + 2) & 63)) ? 0u : win;
-                }
-                win = writelane_u32(mask | (1u << sl), g & 63, win);
-                // lo: the first group of the window with a free slot.  Every group
-                // below it is full, so starting the search there places each
-                // point where the plain scan from maxG - 61 would.  The clear lane
-                // of maxG + 1 ends the advance.
-                if (g == lo)
-                    while ((readlane_u32(win, lo & 63) & 15u) == 15u) ++lo;
+                // the first group from g0 with a free slot: two groups after the
+                // row's last update (the epoch kernel reads a forwarded W row from
+                // the ring one group ahead), after the column's, inside the window.
+                // At gH = the group after the column's last update the point may
+                // only take that update's slot (H_j stays in registers).  Uniform
+                // integer arithmetic throughout: the scan stays on the scalar unit.
+                const int gH = (int)(LC >> 2) + 1;
+                const uint32_t sc = LC & 3u;
+                const int g0 = max(max((int)(LR >> 2) + 2, gH), maxG - 61);
+                const uint32_t w0 = readlane_u32(win, g0 & 63);
+                const uint32_t m0 = (int)(w0 >> 4) == g0 ? w0 : 0u;
+                const int at_h = 1 - min(g0 - gH, 1);   // g0 == gH (g0 >= gH)
+                const int hit = at_h & (int)(~(m0 >> sc) & 1u);
+                // otherwise the first group past gH, or from g0, that is not full;
+                // the clear lane of maxG + 1 ends the search
+                const int k = g0 + at_h;
+                const int gs = k + (int)__builtin_ctzll(~rotr64(full, k & 63));
+                const int g = gs + hit * (g0 - gs);
+                const uint32_t w = readlane_u32(win, g & 63);
+                const uint32_t mask = (int)(w >> 4) == g ? w & 15u : 0u;   // a lane left by an older group reads empty
+                const uint32_t ss = (uint32_t)__builtin_ctz(~mask & 15u);
+                const uint32_t sl = ss + (uint32_t)hit * (sc - ss);
+                const uint32_t nm = mask | (1u << sl);
+                // g <= maxG + 2; the groups maxG + 1 and maxG + 2 of the new maxG
+                // are empty (a no-op unless maxG grew)
+                maxG = max(maxG, g);
+                const uint64_t clr = (1ull << ((maxG + 1) & 63)) | (1ull << ((maxG + 2) & 63));
+                full = (full & ~clr) | ((uint64_t)((nm + 1u) >> 4) << (g & 63));   // nm + 1 == 16: full
+                win = writelane_u32(((uint32_t)g << 4) | nm, g & 63, win);
                 const uint32_t tag = ((uint32_t)g << 2) | sl;
                 myR = r == R ? tag : myR;
                 myC = c == C ? tag : myC;
-                const bool me = lane == l;
-                res = me ? ((uint32_t)g << 3) | (hreg << 2) | sl : res;
-                prevR = me ? LR : prevR;
-                prevC = me ? LC : prevC;
+                res = writelane_u32(((uint32_t)g << 3) | ((uint32_t)hit << 2) | sl, l, res);
+                prevR = writelane_u32(LR, l, prevR);
+                prevC = writelane_u32(LC, l, prevC);
             }
             if (v < n) {
                 lastR[r] = (IX)myR;   // lanes of one row hold the same final tag
@@ -318,7 +337,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped_multi(const PrepDesc* __res
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(PrepDesc) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
     prep_chain<IX, Pt32>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap,
-                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm);
+                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm, D.wide != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -401,6 +420,7 @@ struct GArgs {
     uint32_t ztail_b;          // byte offset of the zero rows after the last row
     uint32_t dummy_b;          // byte offset of the dummy row
     int d, ld, max_cols;
+    int wide, unused;          // wide: H rows streamed from HBM (batched launches)
     float eps, lambda, tmin, tmax;
 };
 
@@ -1077,12 +1097,10 @@ constexpr uint32_t OOB_OFF = 0x7ffffff0u;   // past any buffer: loads read 0, st
 static size_t epoch_lds_wide(int vec) { return (size_t)2 * GROUP_RING * GROUP_SLOTS * 64 * vec * sizeof(float); }
 
 template <int KIND, bool TRUNC, int VEC>
-__global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
-    extern __shared__ float lds[];
+__device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, float* lds) {
     constexpr int ROW = 64 * VEC;
     float* const wring = lds;
     float* const hring = lds + GROUP_RING * GROUP_SLOTS * ROW;
-    const int t = blockIdx.x;
     const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
     const int e0 = 4 * VEC * l16;
     const bool pad = e0 >= A.ld;
@@ -1249,6 +1267,12 @@ __global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
     }
 }
 
+template <int KIND, bool TRUNC, int VEC>
+__global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
+    extern __shared__ float lds[];
+    wide_task<KIND, TRUNC, VEC>(A, blockIdx.x, lds);
+}
+
 // Many problems in one persistent launch: workgroup -> (problem, task) by
 // `tmap`; each problem's arguments are read from device memory, so a captured
 // launch replays with new step sizes and plans.
@@ -1262,6 +1286,15 @@ __global__ __launch_bounds__(128) void k_epoch_grouped_batch(const GArgs<T>* __r
     uint32_t* dst = (uint32_t*)&A;
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(GArgs<T>) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
+    if constexpr (std::is_same<T, float>::value) {
+        if (A.wide) {   // the whole workgroup takes this branch; one wave does the task
+            if (threadIdx.x < 64) {
+                if (A.ld <= 64) wide_task<KIND, TRUNC, 1>(A, t, lds);
+                else wide_task<KIND, TRUNC, 2>(A, t, lds);
+            }
+            return;
+        }
+    }
     if (A.ld <= 64) grouped_task<T, KIND, TRUNC, 1>(A, t, lds);
     else grouped_task<T, KIND, TRUNC, 2>(A, t, lds);
 }
@@ -1362,6 +1395,7 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
     D.cols_cap = B.max_cols;
     D.dummy_row = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2);
     D.dummy_col = (uint32_t)B.max_cols;
+    D.wide = B.wide ? 1 : 0;
     return D;
 }
 
@@ -1401,6 +1435,8 @@ static GArgs<T> make_gargs(const DevProblem& P, const UpdParams& U, const Groupe
     A.d = P.d;
     A.ld = (int)P.ld;
     A.max_cols = B.max_cols;
+    A.wide = B.wide ? 1 : 0;
+    A.unused = 0;
     A.eps = (float)L.eps;
     A.lambda = (float)U.lambda;
     A.tmin = (float)U.tmin;
@@ -1553,7 +1589,9 @@ void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuf
     }
 }
 
-size_t batch_epoch_lds(int max_cols, int64_t max_ld) { return epoch_lds(max_cols, vec_of(max_ld)); }
+size_t batch_epoch_lds(int max_cols, int64_t max_ld, bool wide) {
+    return wide ? epoch_lds_wide(vec_of(max_ld)) : epoch_lds(max_cols, vec_of(max_ld));
+}
 
 template <typename T, int KIND, bool TRUNC>
 static const void* batch_fn_t() {

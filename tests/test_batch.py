@@ -10,18 +10,30 @@ from dsgdpp_amd import batch as B
 
 
 def test_plan_tasks_balances_predicted_time():
-    rates = {r: {1: 1e7, 2: 1e7, 4: 9e6, 8: 8e6, 16: 6e6, 32: 4e6} for r in B.RANK_CLASSES}
+    rates = {r: {1: 1e7, 2: 1e7, 4: 9e6, 8: 8e6, 16: 6e6, 32: 4e6, 64: 3e6} for r in B.RANK_CLASSES}
     probs = [dict(m=10_000, n=5_000, nnz=8_000_000, rank=16),
              dict(m=10_000, n=5_000, nnz=1_000_000, rank=64),
              dict(m=40, n=30, nnz=600, rank=128)]
     mins = [B.min_tasks(p) for p in probs]
-    # two rank-64 H tiles of 5000 / 2d columns need d = 16; 8M points need 16 for the prep state
-    assert mins == [16, 16, 1]
-    assert all(B.lds_need(p["n"], m, p["rank"]) <= B.LDS_BYTES for p, m in zip(probs, mins))
+    # blocks below 8000 points (16-bit prep): 8M points need d = 32, 1M points d = 16
+    assert mins == [32, 16, 1]
+    assert all(B.block_cap(p["nnz"], m) < B.PREP_NARROW for p, m in zip(probs, mins))
+    assert B.block_cap(8_000_000, 16) >= B.PREP_NARROW and B.block_cap(1_000_000, 8) >= B.PREP_NARROW
+    # the LDS alone: 8M points need d = 16 for the prep state; the wide kernel
+    # streams H, so the tiles do not bind, and without it two rank-64 tiles of
+    # 5000 / 2d columns need d = 16
     assert B.prep_lds_estimate(10_000, 5_000, 8_000_000, 8) > B.LDS_BYTES
+    assert B.prep_lds_estimate(10_000, 5_000, 8_000_000, 16) <= B.LDS_BYTES
+    assert B.lds_need(5_000, 8, 64) > B.LDS_BYTES >= B.lds_need(5_000, 8, 64, wide=True)
+    assert B.lds_need(5_000, 16, 64) <= B.LDS_BYTES
+    # the narrow-prep rule stops at the shape (d <= m, 2d <= n): 600k points on 40 x 30 stop at d = 8
+    assert B.min_tasks(dict(m=40, n=30, nnz=600_000, rank=16)) == 8
+    d = B.plan_tasks(probs, budget=120, rates=rates)
+    assert sum(d) <= 120 and all(a >= b for a, b in zip(d, mins))
+    # the big problem is at the largest measured d; the others grow until the shape stops them
+    assert d == [32, 32, 8]
     d = B.plan_tasks(probs, budget=60, rates=rates)
-    assert sum(d) <= 60 and all(a >= b for a, b in zip(d, mins))
-    assert d[0] >= d[1] >= d[2] and d[0] > mins[0]   # the big problem grows first
+    assert d[0] == 32 and d[1] == 16 and sum(d) <= 60   # no room to grow the 1M problem
     pred = [p["nnz"] / (dk * rates[16][dk]) for p, dk in zip(probs, d)]
     assert max(pred) == pred[0]
     assert probs[2]["m"] >= d[2] and probs[2]["n"] >= 2 * d[2]
