@@ -183,9 +183,7 @@ def plan_tasks(problems, budget, rates, wide=True):
     d = [min_tasks(p, wide) for p in problems]
 
     def pred(k, dk):
-        r = rates[rank_class(problems[k]["rank"])]
-        rate = (r.get(dk) or r[max(x for x in r if x <= dk)]) if dk >= min(r) else r[min(r)]
-        return problems[k]["nnz"] / (dk * rate)
+        return predict_time(problems[k], dk, rates)
 
     while True:
         order = sorted(range(len(problems)), key=lambda k: -pred(k, d[k]))
@@ -204,10 +202,22 @@ def plan_tasks(problems, budget, rates, wide=True):
             return d
 
 
-def partition(tasks, budget):
-    """Problem indices split into launches of at most `budget` tasks each
-    (first fit, largest first); a problem larger than the budget runs alone."""
-    order = sorted(range(len(tasks)), key=lambda k: -tasks[k])
+def predict_time(p, d, rates):
+    """Predicted epoch time (s) of problem p on d tasks: nnz / (d * per-task
+    rate), the rate measured at the largest candidate d not above d (the
+    smallest candidate below the measured range)."""
+    r = rates[rank_class(p["rank"])]
+    rate = (r.get(d) or r[max(x for x in r if x <= d)]) if d >= min(r) else r[min(r)]
+    return p["nnz"] / (d * rate)
+
+
+def partition(tasks, budget, times=None):
+    """Problem indices split into launches of at most `budget` tasks each;
+    a problem larger than the budget runs alone.  First fit, in decreasing
+    order of `times` when given (a launch lasts as long as its slowest
+    problem, so problems of like time share launches), else of tasks."""
+    key = times if times is not None else tasks
+    order = sorted(range(len(tasks)), key=lambda k: -key[k])
     parts, load = [], []
     for k in order:
         for q in range(len(parts)):

@@ -1526,8 +1526,14 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     b->desc.clear();
     b->cmap.clear();
     std::vector<EpochPlan*> ahead;
-    std::vector<int2> wide;
-    size_t ldsNarrow = 0, ldsWide = 0;
+    // chains by (index width, workgroups per CU that the problem's prep LDS
+    // allows): one launch per class, so a problem with tall blocks does not
+    // set the LDS, and with it the occupancy, of every other problem's prep
+    struct PrepClass {
+        std::vector<int2> chains;
+        size_t lds = 0;
+    };
+    std::map<std::pair<int, int>, PrepClass> classes;
     for (size_t k = 0; k < K; ++k) {
         dsgdpp_ctx* c = b->ctx[k];
         c->lastPlan = c->cur;
@@ -1545,15 +1551,23 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         const PrepDesc D = prep_desc(Dp, G, N.dJobs, N.dChains, c->sgdOrder, N.maxBlock);
         const int idx = (int)b->desc.size();
         b->desc.push_back(D);
-        const bool narrow = prep_narrow(N.maxBlock);
-        for (int ch = 0; ch + 1 < (int)N.chains.size(); ++ch) (narrow ? b->cmap : wide).push_back(make_int2(idx, ch));
-        size_t& l = narrow ? ldsNarrow : ldsWide;
-        l = std::max(l, prep_desc_lds(D));
+        const size_t l = prep_desc_lds(D);
+        PrepClass& pc = classes[{prep_narrow(N.maxBlock) ? 1 : 0, (int)std::max<size_t>(1, PREP_LDS_PER_CU / l)}];
+        for (int ch = 0; ch + 1 < (int)N.chains.size(); ++ch) pc.chains.push_back(make_int2(idx, ch));
+        pc.lds = std::max(pc.lds, l);
         ahead.push_back(&N);
     }
     if (!ahead.empty()) {
-        const int nNarrow = (int)b->cmap.size();
-        b->cmap.insert(b->cmap.end(), wide.begin(), wide.end());
+        // the classes with the fewest workgroups per CU first: their long tails
+        // overlap the launches that follow
+        std::vector<std::pair<std::pair<int, int>, PrepClass*>> order;
+        for (auto& kv : classes) order.push_back({kv.first, &kv.second});
+        std::sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first.second < y.first.second; });
+        std::vector<int> start;
+        for (auto& o : order) {
+            start.push_back((int)b->cmap.size());
+            b->cmap.insert(b->cmap.end(), o.second->chains.begin(), o.second->chains.end());
+        }
         if (b->desc.size() > b->descCap) {
             b->descCap = b->desc.size();
             dalloc(b->dDesc, b->descCap * sizeof(PrepDesc));
@@ -1567,8 +1581,9 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         if (!b->cmap.empty())
             HIPC(hipMemcpyAsync(b->dCmap, b->cmap.data(), b->cmap.size() * sizeof(int2), hipMemcpyHostToDevice,
                                 b->prepStream));
-        launch_prep_grouped_multi(b->dDesc, b->dCmap, nNarrow, true, ldsNarrow, b->prepStream);
-        launch_prep_grouped_multi(b->dDesc, b->dCmap + nNarrow, (int)wide.size(), false, ldsWide, b->prepStream);
+        for (size_t q = 0; q < order.size(); ++q)
+            launch_prep_grouped_multi(b->dDesc, b->dCmap + start[q], (int)order[q].second->chains.size(),
+                                      order[q].first.first == 1, order[q].second->lds, b->prepStream);
         for (EpochPlan* N : ahead) HIPC(hipEventRecord(N->evEnd, b->prepStream));
     }
     const auto t_ahead = std::chrono::steady_clock::now();

@@ -94,6 +94,7 @@ constexpr int GROUP_SLOTS = 4;
 constexpr int GROUP_RING = 16;    // W forwarding window, in groups (power of two)
 constexpr int GROUP_PAD = 32;     // every visit's group list is padded to a multiple of this
 constexpr int64_t PREP_SPLIT = 4096;   // grouped prep: blocks up to this size run in a launch of their own
+constexpr size_t PREP_LDS_PER_CU = 160 * 1024;   // LDS of a CU, shared by the prep workgroups on it
 // Factor buffers carry FACTOR_TAIL_ROWS spare rows after the last one: the
 // first half stays zero (read by lanes past the row length), the second half
 // takes the stores of empty group slots.

@@ -41,6 +41,12 @@ def test_plan_tasks_balances_predicted_time():
     parts = B.partition([8, 8, 1, 20, 5], budget=21)
     assert sorted(sum(parts, [])) == [0, 1, 2, 3, 4]
     assert all(sum([8, 8, 1, 20, 5][k] for k in q) <= 21 for q in parts)
+    # by time: the two slow problems share a launch, the fast ones the other
+    tasks, times = [8, 8, 8, 8], [1.0, 9.0, 1.5, 8.0]
+    parts = B.partition(tasks, budget=16, times=times)
+    assert sorted(parts) == [[0, 2], [1, 3]]
+    assert sum(max(times[k] for k in q) for q in parts) == 10.5
+    assert sum(max(times[k] for k in q) for q in B.partition(tasks, budget=16)) > 10.5
 
 
 def test_rank_class():
