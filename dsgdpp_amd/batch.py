@@ -143,10 +143,10 @@ def lds_need(n, d, rank, wide=False):
 def prep_lds_estimate(m, n, nnz, d):
     """LDS of the prep kernel for blocks of about 1.5 nnz / 2d^2 points (the
     kernel keeps the visit order and row/column tags, 16-bit below 8,000
-    points, plus one byte per group, fewer than two groups per point)."""
+    points, plus 4 bits per group, fewer than two groups per point)."""
     cap = block_cap(nnz, d)
     es = 2 if cap < PREP_NARROW else 4
-    return (cap + -(-m // d) + -(-n // (2 * d))) * es + 2 * cap + 64
+    return (cap + -(-m // d) + -(-n // (2 * d))) * es + cap + 64
 
 
 PREP_NARROW = 8000   # blocks below this many points take the 16-bit prep

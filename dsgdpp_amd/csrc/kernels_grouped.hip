@@ -92,7 +92,7 @@ constexpr int64_t PREP_NARROW = 8000;    // tag ((group + 2) << 2 | slot) < 2^16
 static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow) {
     const size_t es = narrow ? 2 : 4;
     auto up4 = [](size_t b) { return (b + 3) & ~(size_t)3; };
-    return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + up4(2 * (size_t)cap + GROUP_PAD);
+    return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + ((2 * (size_t)cap + GROUP_PAD + 7) >> 3) * 4;
 }
 
 // A record's value words: x as float and the row's 1/nnz (FP32/BF16 points),
@@ -119,7 +119,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
     IX* vis = (IX*)sm;                               // visit order
     IX* lastR = (IX*)(sm + up4(cap));                // per local row: tag of its last update
     IX* lastC = (IX*)(sm + up4(cap) + up4(rows_cap));  // per local column: the same
-    uint32_t* gmw = sm + up4(cap) + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 per word
+    uint32_t* gmw = sm + up4(cap) + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 bits each
     const int lane = threadIdx.x;
     unsigned long long t_order = 0, t_sched = 0, t_fill = 0;
     for (int jx = chain_start[chain]; jx < chain_start[chain + 1]; ++jx) {
@@ -177,7 +177,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             }
         }
         const unsigned long long t1 = pprof ? clock64() : 0;
-        const int gwords = (2 * n + GROUP_PAD + 3) >> 2;   // a visit has fewer than 2 n groups
+        const int gwords = (2 * n + GROUP_PAD + 7) >> 3;   // a visit has fewer than 2 n groups
         for (int k = lane; k < rows_cap; k += 64) lastR[k] = 0;
         for (int k = lane; k < cols_cap; k += 64) lastC[k] = 0;
         for (int k = lane; k < gwords; k += 64) gmw[k] = 0;
@@ -268,7 +268,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 // W_i from the LDS ring when the row was updated in the last GROUP_RING groups
                 const int gR = (int)(prevR >> 2) - 2;
                 const bool wf = prevR != 0u && (int)g - gR < GROUP_RING;
-                atomicOr(gmw + (g >> 2), (1u << sl) << (8 * (g & 3u)));
+                atomicOr(gmw + (g >> 3), (1u << sl) << (4 * (g & 7u)));
                 uint4 o;
                 o.x = (uint32_t)e.i |
                       (wf ? (REC_WFWD | (((((uint32_t)gR & (GROUP_RING - 1)) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
@@ -290,7 +290,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         const int ngp = (ng + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;
         // 3. empty slots (and the padding groups) get the dummy record
         for (int g = lane; g < ngp; g += 64) {
-            const uint32_t mk = (gmw[g >> 2] >> (8 * (g & 3))) & 0xffu;
+            const uint32_t mk = (gmw[g >> 3] >> (4 * (g & 7))) & 15u;
             uint4* gp = grec + ((uint64_t)J.gbase + g) * GROUP_SLOTS;
             for (int sl = 0; sl < GROUP_SLOTS; ++sl)
                 if (!((mk >> sl) & 1u)) gp[sl] = make_uint4(dummy_row, dummy_col, 0u, 0u);
