@@ -410,6 +410,46 @@ def test_grouped_carry_over_and_many_tasks():
     np.testing.assert_array_equal(Hg, Ho)
 
 
+@pytest.mark.parametrize("sgd", ["WOR", "WR"])
+def test_grouped_prep_edge_blocks(sgd):
+    """Blocks whose sizes sit on the prep's 64-point chunk boundaries (1, 2,
+    63, 64, 65, 129), one block whose points all share a row (every point two
+    groups after the last: the scan's window turns over many times) and one
+    whose points all share a column (a chain of register-forwarded H): visit
+    orders and factors equal the oracle's bit for bit."""
+    print(f"case: grouped prep edge blocks sgd={sgd}", flush=True)
+    R, C, d = 160, 160, 2
+    m, n = d * R, 2 * d * C
+    rng = np.random.default_rng(41)
+    i, j = [], []
+    i += [0] * 150                                  # block (0, 0): one row
+    j += list(range(150))
+    i += list(range(150))                           # block (0, 1): one column
+    j += [C + 7] * 150
+    sizes = {(0, 2): 1, (0, 3): 64, (1, 0): 63, (1, 1): 65, (1, 2): 129, (1, 3): 2}
+    for (b1, b2), k in sizes.items():
+        cells = rng.choice(R * C, size=k, replace=False)
+        i += list(b1 * R + cells // C)
+        j += list(b2 * C + cells % C)
+    p = dict(m=m, n=n, train=(np.array(i), np.array(j), rng.normal(size=len(i))), rank=8,
+             W0=rng.uniform(-0.5, 0.5, (m, 8)), H0=rng.uniform(-0.5, 0.5, (8, n)))
+    g, o = make_pair(p, update="Nzsl_L2(0.1)", tasks=d, seed=13, precision="FP32", kernel=3, sgd=sgd,
+                     stratum="COWOR")
+    o.set_arith("FP32")
+    for e in range(3):
+        g.epoch(0.05)
+        o.epoch(0.05)
+        for t in range(d):
+            for s_ in range(2 * d):
+                np.testing.assert_array_equal(g.permutation(t, s_), o.permutation(t, s_),
+                                              err_msg=f"epoch {e} task {t} sub-epoch {s_}")
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo)
+        np.testing.assert_array_equal(Hg, Ho)
+    g.close()
+
+
 @pytest.mark.parametrize("precision,kernel", [("FP64_STRICT", 0), ("FP32", 3)])
 def test_lookahead_off_matches_oracle(monkeypatch, precision, kernel):
     """Without the look-ahead plan, every epoch still draws a fresh plan in
