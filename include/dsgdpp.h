@@ -174,8 +174,10 @@ int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
  * factorization at a time: dsgdpp_impl.h:282-349).  `count` contexts --
  * independent problems of any shape, rank <= 128, own lambda -- sharing the
  * device, precision (FP32 or BF16), update kind and truncation advance one
- * epoch each in ONE persistent launch of the grouped kernel; every context's
- * result is bit-identical to dsgdpp_epoch on it alone.  With use_graph the
+ * epoch each in ONE persistent launch of the grouped kernel (an FP32 context
+ * whose column blocks are too wide for the LDS tile runs the H-streaming body,
+ * kernel 4, inside the same launch); every context's result is bit-identical
+ * to dsgdpp_epoch on it alone.  With use_graph the
  * launch is captured once and replayed each epoch.  The sum of the contexts'
  * tasks must fit on the device at once (E_UNSUPPORTED otherwise).  The
  * contexts stay owned by the caller and usable on their own.
