@@ -916,11 +916,21 @@ void balance(dsgdpp_ctx* c, int type) {
 // Matrix Market (read_impl.h:46-256, write_impl.h:94-118)
 // ---------------------------------------------------------------------------
 std::string readAll(const std::string& fn) {
-    std::ifstream in(fn, std::ios::binary);
-    if (!in.is_open()) throw Error(DSGDPP_E_IO, "Cannot open file " + fn);
-    std::stringstream ss;
-    ss << in.rdbuf();
-    return ss.str();
+    FILE* f = fopen(fn.c_str(), "rb");
+    if (!f) throw Error(DSGDPP_E_IO, "Cannot open file " + fn);
+    std::string s;
+    if (fseeko(f, 0, SEEK_END) == 0) {   // a regular file: one allocation
+        const off_t n = ftello(f);
+        if (n > 0) s.reserve((size_t)n);
+        fseeko(f, 0, SEEK_SET);
+    }
+    std::vector<char> buf(1 << 24);
+    size_t got;
+    while ((got = fread(buf.data(), 1, buf.size(), f)) > 0) s.append(buf.data(), got);
+    const bool bad = ferror(f) != 0;
+    fclose(f);
+    if (bad) throw Error(DSGDPP_E_IO, "Cannot read file " + fn);
+    return s;
 }
 
 struct LineReader {
@@ -944,6 +954,75 @@ bool onlySpace(const char* b, const char* e) {
     for (; b < e; ++b)
         if (!isspace((unsigned char)*b)) return false;
     return true;
+}
+
+// The data lines of a Matrix Market body, parsed on all host threads: the
+// `count` lines from byte `start` (file line lineNo0 + 1 on) go to
+// parse(p, line) (a null-terminated copy), which returns 0, or an error
+// message prefix ("Parse error", "Index out of range"); lines past them must
+// be blank.  The error reported is the sequential reader's: the first one in
+// file order, then a short file.
+template <typename F>
+void parseBody(const std::string& text, size_t start, int64_t lineNo0, int64_t count, const std::string& fn,
+               F parse) {
+    const size_t end = text.size();
+    const int th = hostThreads();
+    std::vector<size_t> cut(th + 1, end);
+    cut[0] = std::min(start, end);
+    for (int k = 1; k < th; ++k) {   // chunks begin at line starts
+        size_t c = std::max(cut[k - 1], cut[0] + (end - cut[0]) / th * k);
+        if (c > cut[0] && c < end) {
+            const void* nl = memchr(text.data() + c - 1, '\n', end - (c - 1));
+            c = nl ? (size_t)((const char*)nl - text.data()) + 1 : end;
+        }
+        cut[k] = c;
+    }
+    auto lines_in = [&](size_t a, size_t b) {   // LineReader's lines in [a, b)
+        int64_t n = 0;
+        for (const char* q = text.data() + a; (q = (const char*)memchr(q, '\n', text.data() + b - q)); ++q) ++n;
+        return n + ((b > a && text[b - 1] != '\n') ? 1 : 0);
+    };
+    std::vector<int64_t> first(th + 1, 0);
+    {
+        std::vector<std::thread> ts;
+        for (int k = 0; k < th; ++k) ts.emplace_back([&, k] { first[k + 1] = lines_in(cut[k], cut[k + 1]); });
+        for (auto& t : ts) t.join();
+    }
+    for (int k = 0; k < th; ++k) first[k + 1] += first[k];
+    std::vector<int64_t> errLine(th, INT64_MAX);
+    std::vector<std::string> errMsg(th);
+    {
+        std::vector<std::thread> ts;
+        for (int k = 0; k < th; ++k)
+            ts.emplace_back([&, k] {
+                std::string line;
+                int64_t L = first[k];
+                for (size_t pos = cut[k]; pos < cut[k + 1]; ++L) {
+                    const char* nl = (const char*)memchr(text.data() + pos, '\n', cut[k + 1] - pos);
+                    const size_t e = nl ? (size_t)(nl - text.data()) : cut[k + 1];
+                    const char* msg = nullptr;
+                    if (L < count) {
+                        line.assign(text.data() + pos, e - pos);
+                        msg = parse(L, line);
+                    } else if (!onlySpace(text.data() + pos, text.data() + e)) {
+                        msg = "Unexpected input";
+                    }
+                    if (msg) {
+                        errLine[k] = lineNo0 + L + 1;
+                        errMsg[k] = msg;
+                        return;
+                    }
+                    pos = e + 1;
+                }
+            });
+        for (auto& t : ts) t.join();
+    }
+    int best = -1;
+    for (int k = 0; k < th; ++k)
+        if (errLine[k] != INT64_MAX && (best < 0 || errLine[k] < errLine[best])) best = k;
+    if (best >= 0)
+        throw Error(DSGDPP_E_IO, errMsg[best] + " at line " + std::to_string(errLine[best]) + " of " + fn);
+    if (first[th] < count) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
 }
 
 void readHeader(LineReader& lr, const std::string& fn, const char* banner, std::string& dims) {
@@ -971,31 +1050,23 @@ void loadMmc(dsgdpp_ctx* c, int which, const std::string& fn) {
         throw Error(DSGDPP_E_IO, "Invalid matrix dimensions in file " + fn + ": " + dims);
     std::vector<int32_t> I(nnz), J(nnz);
     std::vector<double> X(nnz);
-    std::string line;
-    for (long long p = 0; p < nnz; ++p) {
-        const char *b, *e;
-        if (!lr.next(b, e)) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
-        line.assign(b, e);
+    parseBody(text, lr.pos, lr.lineNo, nnz, fn, [&](int64_t p, const std::string& line) -> const char* {
         const char* s = line.c_str();
         char* q = nullptr;
         const unsigned long i = strtoul(s, &q, 10);
-        if (q == s) throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        if (q == s) return "Parse error";
         s = q;
         const unsigned long j = strtoul(s, &q, 10);
-        if (q == s) throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        if (q == s) return "Parse error";
         s = q;
         const double x = strtod(s, &q);
-        if (q == s || !onlySpace(q, line.c_str() + line.size()))
-            throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
-        if (i < 1 || j < 1 || (long long)i > m || (long long)j > n)
-            throw Error(DSGDPP_E_IO, "Index out of range at line " + std::to_string(lr.lineNo) + " of " + fn);
+        if (q == s || !onlySpace(q, line.c_str() + line.size())) return "Parse error";
+        if (i < 1 || j < 1 || (long long)i > m || (long long)j > n) return "Index out of range";
         I[p] = (int32_t)(i - 1);
         J[p] = (int32_t)(j - 1);
         X[p] = x;
-    }
-    const char *b, *e;
-    while (lr.next(b, e))
-        if (!onlySpace(b, e)) throw Error(DSGDPP_E_IO, "Unexpected input at line " + std::to_string(lr.lineNo) + " of " + fn);
+        return nullptr;
+    });
     loadCoo(c, which, m, n, nnz, I.data(), J.data(), X.data());
 }
 
@@ -1012,20 +1083,13 @@ void readMma(const std::string& fn, int64_t& rows, int64_t& cols, std::vector<do
     rows = a;
     cols = b2;
     cm.resize((size_t)a * b2);
-    std::string line;
-    for (size_t p = 0; p < cm.size(); ++p) {
-        const char *b, *e;
-        if (!lr.next(b, e)) throw Error(DSGDPP_E_IO, "Unexpected EOF in file " + fn);
-        line.assign(b, e);
+    parseBody(text, lr.pos, lr.lineNo, (int64_t)cm.size(), fn, [&](int64_t p, const std::string& line) -> const char* {
         char* q = nullptr;
         const double x = strtod(line.c_str(), &q);
-        if (q == line.c_str() || !onlySpace(q, line.c_str() + line.size()))
-            throw Error(DSGDPP_E_IO, "Parse error at line " + std::to_string(lr.lineNo) + " of " + fn);
+        if (q == line.c_str() || !onlySpace(q, line.c_str() + line.size())) return "Parse error";
         cm[p] = x;
-    }
-    const char *b, *e;
-    while (lr.next(b, e))
-        if (!onlySpace(b, e)) throw Error(DSGDPP_E_IO, "Unexpected input at line " + std::to_string(lr.lineNo) + " of " + fn);
+        return nullptr;
+    });
 }
 
 void writeMma(const std::string& fn, int64_t rows, int64_t cols, const std::vector<double>& cm) {

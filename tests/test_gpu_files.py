@@ -1,0 +1,103 @@
+"""The file boundary on the GPU path: the mfdsgdpp command line against the
+oracle's command line on the same Matrix Market inputs (identical factor files
+and trace losses, FP64_STRICT), and the library's Matrix Market reader (its
+parsing is spread over the host threads) against the data it was written from
+and the reader's error messages (mfdsgdpp.cc:361-567, read_impl.h:46-116)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dsgdpp_amd", "bin", "mfdsgdpp")
+ORACLE_CLI = os.path.join(ROOT, "oracle", "mfdsgdpp_cpu")
+
+
+def _write(tmp, m, n, r, nnz, ntest, seed=0):
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(m * n, size=nnz + ntest, replace=False)
+    for name, part in (("v.mmc", cells[:nnz]), ("t.mmc", cells[nnz:])):
+        with open(tmp / name, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate real general\n% written by the test\n")
+            f.write(f"{m} {n} {len(part)}\n")
+            for c in part:
+                f.write(f"{c // n + 1} {c % n + 1} {rng.normal():.17g}\n")
+    for name, a, b in (("w.mma", m, r), ("h.mma", r, n)):
+        with open(tmp / name, "w") as f:
+            f.write(f"%%MatrixMarket matrix array real general\n{a} {b}\n")
+            for v in rng.uniform(-0.5, 0.5, a * b):
+                f.write(f"{v:.17g}\n")
+    return cells
+
+
+def _losses(trace):
+    # each entry's training and test loss (the time=list(...) sub-entry has keys of the same names)
+    return [(float(a), float(b)) for a, b in re.findall(r"it=\d+, epoch=\d+, loss=([^,]+), loss\.test=([^,]+),", trace)]
+
+
+def test_cli_matches_oracle_cli(tmp_path):
+    print("case: mfdsgdpp CLI vs oracle CLI, FP64_STRICT", flush=True)
+    _write(tmp_path, 60, 80, 4, 2000, 300, seed=3)
+    common = ["--input-file", str(tmp_path / "v.mmc"), "--input-test-file", str(tmp_path / "t.mmc"),
+              "--input-row-file", str(tmp_path / "w.mma"), "--input-col-file", str(tmp_path / "h.mma"),
+              "--update", "Nzsl_L2(0.1)", "--regularize", "None", "--loss", "Nzsl_L2(0.1)", "--rank", "4",
+              "--decay", "BoldDriver(0.05)", "--epochs", "4", "--tasks-per-rank", "2", "--seed", "11"]
+    outs = {}
+    for tag, exe in (("gpu", CLI), ("cpu", ORACLE_CLI)):
+        r = subprocess.run([exe] + common + ["--output-row-file", str(tmp_path / f"w_{tag}.mma"),
+                                             "--output-col-file", str(tmp_path / f"h_{tag}.mma"),
+                                             "--trace", str(tmp_path / f"t_{tag}.R")],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (tag, r.stderr)
+        outs[tag] = r
+    for f in ("w", "h"):
+        assert (tmp_path / f"{f}_gpu.mma").read_text() == (tmp_path / f"{f}_cpu.mma").read_text(), f
+    lg = _losses((tmp_path / "t_gpu.R").read_text())
+    lc = _losses((tmp_path / "t_cpu.R").read_text())
+    assert len(lg) == 5 and lg == lc
+
+
+def test_mmc_reader_matches_coo_and_reports_errors(tmp_path):
+    print("case: library Matrix Market reader, 150k lines + error lines", flush=True)
+    import dsgdpp_amd as D
+    m, n, r = 700, 500, 3
+    cells = _write(tmp_path, m, n, r, 150_000, 0, seed=5)
+    # the same points through load_coo, read back from the text the reader parsed
+    body = np.loadtxt(tmp_path / "v.mmc", skiprows=3)
+    i, j, x = body[:, 0].astype(np.int64) - 1, body[:, 1].astype(np.int64) - 1, body[:, 2]
+    assert len(i) == 150_000 and np.array_equal(i * n + j, cells[:150_000])
+    rng = np.random.default_rng(1)
+    W0, H0 = rng.uniform(-0.5, 0.5, (m, r)), rng.uniform(-0.5, 0.5, (r, n))
+    losses = []
+    for via_file in (True, False):
+        g = D.DsgdPp(update="Nzsl_L2(0.1)", loss="Nzsl_L2(0.1)", rank=r, tasks_per_rank=2, seed=2, precision="FP64")
+        if via_file:
+            g.load_mmc(D.TRAIN, tmp_path / "v.mmc")
+        else:
+            g.load_coo(D.TRAIN, m, n, i, j, x)
+        g.set_factors(W0, H0)
+        g.epoch(0.01)
+        losses.append(g.loss(D.TRAIN))
+        g.close()
+    assert losses[0] == losses[1]
+    head = "%%MatrixMarket matrix coordinate real general\n% c\n4 4 3\n"
+    cases = [("1 1 1\n2 x 2\n3 3 3\n", "Parse error at line 5 of"),
+             ("1 1 1\n2 2 2\n5 3 3\n", "Index out of range at line 6 of"),
+             ("1 1 1\n2 2 2\n3 3 3 z\n", "Parse error at line 6 of"),
+             ("1 1 1\n2 2 2\n", "Unexpected EOF in file"),
+             ("1 1 1\n2 2 2\n3 3 3\n\n4 4 4\n", "Unexpected input at line 8 of"),
+             ("1 1 1\r\n2 2 2\r\n3 3 3\r\n  \n", None)]
+    for k, (rest, msg) in enumerate(cases):
+        path = tmp_path / f"e{k}.mmc"
+        path.write_text(head + rest)
+        g = D.DsgdPp(update="Nzsl_L2(0.1)", loss="Nzsl_L2(0.1)", rank=2, tasks_per_rank=1, seed=2, precision="FP64")
+        if msg is None:
+            g.load_mmc(D.TRAIN, path)
+        else:
+            with pytest.raises(RuntimeError, match=re.escape(msg)):
+                g.load_mmc(D.TRAIN, path)
+        g.close()
