@@ -19,6 +19,22 @@ namespace dsgdpp {
                                      " at " #call);                                 \
     } while (0)
 
+// A cooperative launch (all workgroups resident at once: the dataflow tasks
+// wait on each other).  The runtime serves cooperative launches of one
+// process per device at a time; another process holding them makes this
+// launch fail with hipErrorIllegalState, which the message names.
+inline void launch_cooperative(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t s,
+                               const char* what) {
+    const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, lds, s);
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " at the cooperative launch of " +
+                                 what +
+                                 (e == hipErrorIllegalState
+                                      ? " (cooperative launches serve one process per device at a time: another "
+                                        "process on this GPU may be running dataflow epochs; kernel 1 needs none)"
+                                      : ""));
+}
+
 // ---------------------------------------------------------------------------
 // storage types
 // ---------------------------------------------------------------------------

@@ -346,14 +346,37 @@ void buildBlocks(dsgdpp_ctx* c, int64_t nnz, const int32_t* I, const int32_t* J,
         for (int64_t i = c->off1[b]; i < c->off1[b + 1]; ++i) rb[i] = b;
     for (uint32_t b = 0; b < d2; ++b)
         for (int64_t j = c->off2[b]; j < c->off2[b + 1]; ++j) cb[j] = b;
+    // counting sort by block, stable: thread t counts and scatters the t-th
+    // chunk of the input, its points placed after those of earlier chunks
+    const int th = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)hostThreads(), nnz / 65536 + 1,
+                                                                 (int64_t)(32 << 20) / (int64_t)nb}));
     std::vector<int64_t> cnt(nb + 1, 0);
-    for (int64_t p = 0; p < nnz; ++p) cnt[(size_t)rb[I[p]] * d2 + cb[J[p]] + 1]++;
-    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
-    std::vector<int64_t> order(nnz);
-    {
-        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-        for (int64_t p = 0; p < nnz; ++p) order[pos[(size_t)rb[I[p]] * d2 + cb[J[p]]]++] = p;
+    std::vector<std::vector<int64_t>> part(th);
+    auto blk = [&](int64_t p) { return (size_t)rb[I[p]] * d2 + cb[J[p]]; };
+    auto each = [&](auto fn) {
+        std::vector<std::thread> ts;
+        for (int t = 0; t < th; ++t) ts.emplace_back(fn, t, nnz * t / th, nnz * (t + 1) / th);
+        for (auto& x : ts) x.join();
+    };
+    each([&](int t, int64_t lo, int64_t hi) {
+        part[t].assign(nb, 0);
+        for (int64_t p = lo; p < hi; ++p) part[t][blk(p)]++;
+    });
+    for (size_t b = 0; b < nb; ++b) {
+        int64_t at = cnt[b];
+        for (int t = 0; t < th; ++t) {
+            const int64_t k = part[t][b];
+            part[t][b] = at;   // thread t's first position in block b
+            at += k;
+        }
+        cnt[b + 1] = at;
     }
+    std::vector<int64_t> order(nnz);
+    each([&](int t, int64_t lo, int64_t hi) {
+        std::vector<int64_t>& pos = part[t];
+        for (int64_t p = lo; p < hi; ++p) order[pos[blk(p)]++] = p;
+    });
+    part.clear();
     std::vector<int64_t> kept(nb, 0);
     out.resize(nnz);
     parallelFor((int64_t)nb, [&](int64_t lo, int64_t hi) {
@@ -400,12 +423,20 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
     if constexpr (std::is_same<Pt, Pt32>::value) {
         // training points carry 1/nnz of their row (FP32 bits) for the grouped prep kernel
         if (which == DSGDPP_TRAIN) {
+            // a row's points lie in its row block's 2d blocks, which are contiguous
             std::vector<int64_t> rc(c->m, 0);
-            for (const Pt& e : pts) rc[e.i]++;
-            for (Pt& e : pts) {
-                const float f = (float)(1. / (double)rc[e.i]);
-                memcpy(&e.aux, &f, 4);
-            }
+            const uint32_t d2 = 2 * c->d;
+            parallelFor((int64_t)c->d, [&](int64_t lo, int64_t hi) {
+                for (int64_t b1 = lo; b1 < hi; ++b1) {
+                    Pt* const p0 = pts.data() + hb.start[(size_t)b1 * d2];
+                    Pt* const p1 = pts.data() + hb.start[(size_t)(b1 + 1) * d2];
+                    for (Pt* e = p0; e != p1; ++e) rc[e->i]++;
+                    for (Pt* e = p0; e != p1; ++e) {
+                        const float f = (float)(1. / (double)rc[e->i]);
+                        memcpy(&e->aux, &f, 4);
+                    }
+                }
+            });
         }
     }
     dalloc(c->dPts[which], pts.size() * sizeof(Pt));

@@ -415,8 +415,8 @@ static void launch_t(const DevProblem& P, const UpdParams& U, const EpochLaunch&
         k_fill_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d, 0);
         HIPK(hipGetLastError());
         void* args[] = {&A};
-        HIPK(hipLaunchCooperativeKernel((const void*)k_epoch_dataflow_seq<T, K, STRICT>, dim3(P.d),
-                                        dim3(64), args, 0, s));
+        launch_cooperative((const void*)k_epoch_dataflow_seq<T, K, STRICT>, dim3(P.d), dim3(64), args, 0, s,
+                           "k_epoch_dataflow_seq");
     } else {
         for (int st = 0; st < 2 * P.d; ++st) {
             SgdArgs<T> A = make_args<T>(P, U, L.sched + (size_t)st * P.d, L.voff + (size_t)st * P.d,

@@ -1456,7 +1456,7 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(128), args, bytes, s));
+    launch_cooperative(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
 }
 
 template <int KIND, bool TRUNC, int VEC>
@@ -1469,7 +1469,7 @@ static void epoch_wide_t(const DevProblem& P, const UpdParams& U, const GroupedB
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(64), args, bytes, s));
+    launch_cooperative(fn, dim3(P.d), dim3(64), args, bytes, s, "the grouped epoch kernel (H streamed)");
 }
 
 template <typename T, int KIND, bool TRUNC>
@@ -1543,7 +1543,7 @@ static void epoch_grouped64_t(const DevProblem& P, const UpdParams& U, const Gro
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    HIPK(hipLaunchCooperativeKernel(fn, dim3(P.d), dim3(128), args, bytes, s));
+    launch_cooperative(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
 }
 
 template <int KIND>

@@ -46,9 +46,13 @@ def test_cli_matches_oracle_cli(tmp_path):
               "--input-row-file", str(tmp_path / "w.mma"), "--input-col-file", str(tmp_path / "h.mma"),
               "--update", "Nzsl_L2(0.1)", "--regularize", "None", "--loss", "Nzsl_L2(0.1)", "--rank", "4",
               "--decay", "BoldDriver(0.05)", "--epochs", "4", "--tasks-per-rank", "2", "--seed", "11"]
+    # kernel 1 (one launch per stratum): the dataflow kernels launch cooperatively, which the
+    # runtime serves to one process per device, and this test's own process may hold that
+    gpu_only = ["--kernel", "1"]
     outs = {}
     for tag, exe in (("gpu", CLI), ("cpu", ORACLE_CLI)):
-        r = subprocess.run([exe] + common + ["--output-row-file", str(tmp_path / f"w_{tag}.mma"),
+        r = subprocess.run([exe] + common + (gpu_only if tag == "gpu" else []) +
+                           ["--output-row-file", str(tmp_path / f"w_{tag}.mma"),
                                              "--output-col-file", str(tmp_path / f"h_{tag}.mma"),
                                              "--trace", str(tmp_path / f"t_{tag}.R")],
                            capture_output=True, text=True, timeout=120)
