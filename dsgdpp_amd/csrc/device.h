@@ -20,9 +20,9 @@ namespace dsgdpp {
     } while (0)
 
 // A cooperative launch (all workgroups resident at once: the dataflow tasks
-// wait on each other).  The runtime serves cooperative launches of one
-// process per device at a time; another process holding them makes this
-// launch fail with hipErrorIllegalState, which the message names.
+// wait on each other).  A second process's cooperative launch on a device
+// whose cooperative launches another process has used fails with
+// hipErrorIllegalState (observed on MI355X); the message names that case.
 inline void launch_cooperative(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t s,
                                const char* what) {
     const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, lds, s);
