@@ -368,6 +368,23 @@ struct Vec4<float> {
         __builtin_amdgcn_raw_buffer_store_b128(u, r, (int)off, 0, 0);
     }
     static __device__ __forceinline__ float round(float v) { return v; }
+    // a prefetched row as loaded, and its values at the time of use
+    typedef u32x4 Raw;
+    static __device__ __forceinline__ Raw braw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    }
+    static __device__ __forceinline__ float4 cook(const Raw& u) {
+        return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
+    }
+    // the values once stored (raw: their stored words)
+    static __device__ __forceinline__ float4 round4(float4 v, Raw& raw) {
+        raw.x = __float_as_uint(v.x); raw.y = __float_as_uint(v.y);
+        raw.z = __float_as_uint(v.z); raw.w = __float_as_uint(v.w);
+        return v;
+    }
+    static __device__ __forceinline__ void bstore_raw(__amdgpu_buffer_rsrc_t r, uint32_t off, const Raw& raw) {
+        __builtin_amdgcn_raw_buffer_store_b128(raw, r, (int)off, 0, 0);
+    }
 };
 template <>
 struct Vec4<bf16_t> {
@@ -400,6 +417,27 @@ struct Vec4<bf16_t> {
     }
     // value the factor takes once stored
     static __device__ __forceinline__ float round(float v) { return (float)(__bf16)v; }
+    // a prefetched row stays packed until its group uses it: unpacking at the
+    // load would make the load's wait the next instruction's, and every
+    // prefetch in the queue would be waited for at once
+    typedef u32x2 Raw;
+    static __device__ __forceinline__ Raw braw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    }
+    static __device__ __forceinline__ float4 cook(const Raw& u) { return unpack(u.x, u.y); }
+    // round to nearest even in pairs (one v_cvt_pk_bf16_f32 per two values);
+    // raw holds the stored words
+    static __device__ __forceinline__ float4 round4(float4 v, Raw& raw) {
+        typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 lo = {v.x, v.y}, hi = {v.z, v.w};
+        raw.x = __builtin_bit_cast(uint32_t, __builtin_convertvector(lo, bf2));
+        raw.y = __builtin_bit_cast(uint32_t, __builtin_convertvector(hi, bf2));
+        return unpack(raw.x, raw.y);
+    }
+    static __device__ __forceinline__ void bstore_raw(__amdgpu_buffer_rsrc_t r, uint32_t off, const Raw& raw) {
+        __builtin_amdgcn_raw_buffer_store_b64(raw, r, (int)off, 0, 0);
+    }
 };
 
 template <typename T>
@@ -610,11 +648,11 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
         char* const hblk_lane = (char*)(hbp + e0);                     // + column * ROW floats
         const char* const hinv_b = (const char*)(hbp + (A.max_cols + 1) * ROW);   // + column << 2
         uint4 qa[GQ];
-        float4 qb[GPREF][VEC];
-        auto load_w = [&](const uint4& rec, float4* out) {
+        typename Vec4<T>::Raw qb[GPREF][VEC];
+        auto load_w = [&](const uint4& rec, typename Vec4<T>::Raw* out) {
             const uint32_t o = __umul24(rec.x, ldb) + lbase;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) out[v] = Vec4<T>::bload(Wr, o + 4 * v * ES);
+            for (int v = 0; v < VEC; ++v) out[v] = Vec4<T>::braw(Wr, o + 4 * v * ES);
         };
         // prologue (W rows belong to this task: no need to wait for the column block)
 #pragma unroll
@@ -640,7 +678,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                 float4 w[VEC], h[VEC], fwd[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
-                    w[v] = qb[u % GPREF][v];
+                    w[v] = Vec4<T>::cook(qb[u % GPREF][v]);
                     h[v] = hq[v];
                     fwd[v] = wq[v];
                 }
@@ -690,6 +728,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                 if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = ih; }
                 if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
                 float4 wn[VEC], hn[VEC];
+                typename Vec4<T>::Raw wraw[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     wn[v].x = fmaf(-f1, h[v].x, fmaf(-a, w[v].x, w[v].x));
@@ -706,16 +745,15 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                         TRUNC_(hn[v].x); TRUNC_(hn[v].y); TRUNC_(hn[v].z); TRUNC_(hn[v].w);
 #undef TRUNC_
                     }
-                    wn[v].x = Vec4<T>::round(wn[v].x); wn[v].y = Vec4<T>::round(wn[v].y);
-                    wn[v].z = Vec4<T>::round(wn[v].z); wn[v].w = Vec4<T>::round(wn[v].w);
-                    hn[v].x = Vec4<T>::round(hn[v].x); hn[v].y = Vec4<T>::round(hn[v].y);
-                    hn[v].z = Vec4<T>::round(hn[v].z); hn[v].w = Vec4<T>::round(hn[v].w);
+                    typename Vec4<T>::Raw hraw;
+                    wn[v] = Vec4<T>::round4(wn[v], wraw[v]);
+                    hn[v] = Vec4<T>::round4(hn[v], hraw);
                     hprev[v] = hn[v];
                 }
                 const uint32_t so = __umul24(rec.x, ldb) + sbase;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
-                    Vec4<T>::bstore(Wr, so + 4 * v * ES, wn[v]);
+                    Vec4<T>::bstore_raw(Wr, so + 4 * v * ES, wraw[v]);
                     *(float4*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = wn[v];
                     *(float4*)(hblk_lane + (rec.y << RSH) + 16 * v) = hn[v];
                 }
