@@ -50,6 +50,7 @@ struct St<double> {
     static __device__ __forceinline__ C load(const double* p) { return *p; }
     static __device__ __forceinline__ void store(double* p, C v) { *p = v; }
     static __device__ __forceinline__ double wide(const double* p) { return *p; }
+    static __device__ __forceinline__ C rt(C v) { return v; }   // the value once stored and loaded
 };
 template <>
 struct St<float> {
@@ -57,6 +58,7 @@ struct St<float> {
     static __device__ __forceinline__ C load(const float* p) { return *p; }
     static __device__ __forceinline__ void store(float* p, C v) { *p = v; }
     static __device__ __forceinline__ double wide(const float* p) { return (double)*p; }
+    static __device__ __forceinline__ C rt(C v) { return v; }
 };
 __device__ __forceinline__ float bf16_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even
@@ -71,6 +73,7 @@ struct St<bf16_t> {
     static __device__ __forceinline__ C load(const bf16_t* p) { return bf16_to_f32(p->u); }
     static __device__ __forceinline__ void store(bf16_t* p, C v) { p->u = f32_to_bf16(v); }
     static __device__ __forceinline__ double wide(const bf16_t* p) { return (double)bf16_to_f32(p->u); }
+    static __device__ __forceinline__ C rt(C v) { return bf16_to_f32(f32_to_bf16(v)); }
 };
 
 template <typename T>

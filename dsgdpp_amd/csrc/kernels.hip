@@ -110,25 +110,85 @@ struct SgdArgs {
 template <typename T, int K, bool STRICT>
 __device__ __forceinline__ void run_block_seq(const SgdArgs<T>& A, int64_t s0, int64_t n,
                                               const uint32_t* order) {
+    // One point after the other in visit order (sgd_impl.h:293-351), the
+    // memory traffic pipelined around the dependent updates:
+    //  - lane l holds point 64c + l of the current chunk and of the next one
+    //    (its visit position two chunks ahead, its record one chunk ahead,
+    //    its rows' 1/nnz from the middle of the chunk before);
+    //  - the W/H rows of point p + D are loaded after point p's stores; the
+    //    rows that points p .. p + D - 1 update after that load are taken
+    //    from a register ring of the last D updates (the stored values), so
+    //    every update reads what the sequential order leaves there.
     using C = typename St<T>::C;
     using Pt = typename PtOf<T>::type;
+    constexpr int D = 8;
+    static_assert(64 % D == 0, "chunks hold whole prefetch rounds");
     const int lane = threadIdx.x;
     const C eps = A.eps;
     const C m2eps = eps * (C)-2.;
     const C f2 = eps * (C)2. * A.lambda;
     const int r = A.r;
     const int kind = A.kind;
-    for (int64_t p = 0; p < n; ++p) {
-        const uint32_t q = A.use_perm ? order[p] : (uint32_t)p;
-        const Pt pt = A.pts[s0 + q];
-        T* wr = A.W + (int64_t)pt.i * A.ld;
-        T* hr = A.H + (int64_t)pt.j * A.ld;
-        C w[K], h[K];
+    const bool l2 = kind == UPD_NZSL_L2;
+    const int64_t nch = (n + 63) / 64;
+    auto vis = [&](int64_t c) -> uint32_t {   // visit position of lane's point in chunk c
+        const int64_t p = c * 64 + lane;
+        return p < n ? (A.use_perm ? order[p] : (uint32_t)p) : 0u;
+    };
+    auto rec = [&](int64_t c, uint32_t q) -> Pt {
+        Pt e{};
+        if (c * 64 + lane < n) e = A.pts[s0 + q];
+        return e;
+    };
+    // chunk 0 and 1 records, chunk 2 visit positions
+    uint32_t q2 = vis(2);
+    Pt cur = rec(0, vis(0));
+    Pt nxt = rec(1, vis(1));
+    C ca = l2 ? A.inv1[cur.i] : (C)0, cb = l2 ? A.inv2[cur.j] : (C)0;
+    C na = (C)0, nb = (C)0;
+    // the W/H rows of points 0 .. D-1, and the ring of the last D updates
+    C qw[D][K], qh[D][K], rw[D][K], rh[D][K];
+    int ri[D], rj[D];
+    auto load_rows = [&](int i, int j, C* w, C* h) {
+        const T* wr = A.W + (int64_t)i * A.ld;
+        const T* hr = A.H + (int64_t)j * A.ld;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int z = lane + 64 * k;
             w[k] = z < r ? St<T>::load(wr + z) : (C)0;
             h[k] = z < r ? St<T>::load(hr + z) : (C)0;
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+        const int i = (int)readlane_u32((uint32_t)cur.i, u), j = (int)readlane_u32((uint32_t)cur.j, u);
+        load_rows(u < n ? i : 0, u < n ? j : 0, qw[u], qh[u]);
+        ri[u] = -1;
+        rj[u] = -1;
+    }
+    // point p = 64 c + l of the chunk, queue and ring slot u = l % D
+    auto step = [&](const int u, const int l, const int64_t p) __attribute__((always_inline)) {
+        const int i = (int)readlane_u32((uint32_t)cur.i, l), j = (int)readlane_u32((uint32_t)cur.j, l);
+        const C x = (C)readlane(cur.x, l);
+        C w[K], h[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            w[k] = qw[u][k];
+            h[k] = qh[u][k];
+        }
+        int hit = 0;   // a row or column of the last D updates recurs: rare on large blocks
+#pragma unroll
+        for (int s = 0; s < D; ++s) hit |= (ri[s] == i ? 1 : 0) | (rj[s] == j ? 1 : 0);
+        if (hit) {
+#pragma unroll
+            for (int s = 0; s < D; ++s) {   // oldest to newest: slot u holds point p - D
+                const int sl = (u + s) % D;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    w[k] = ri[sl] == i ? rw[sl][k] : w[k];
+                    h[k] = rj[sl] == j ? rh[sl][k] : h[k];
+                }
+            }
         }
         C wh;
         if constexpr (STRICT) {
@@ -140,7 +200,7 @@ __device__ __forceinline__ void run_block_seq(const SgdArgs<T>& A, int64_t s0, i
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int lim = min(64, r - 64 * k);
-                for (int l = 0; l < lim; ++l) wh = wh + readlane(prod[k], l);
+                for (int q = 0; q < lim; ++q) wh = wh + readlane(prod[k], q);
             }
         } else {
             C part = (C)0;
@@ -148,13 +208,11 @@ __device__ __forceinline__ void run_block_seq(const SgdArgs<T>& A, int64_t s0, i
             for (int k = 0; k < K; ++k) part = fma(w[k], h[k], part);
             wh = wave_sum(part);
         }
-        const C x = (C)pt.x;
         const C f1 = m2eps * (x - wh);
-        C a = (C)0, bb = (C)0;
-        if (kind == UPD_NZSL_L2) {
-            a = f2 * A.inv1[pt.i];
-            bb = f2 * A.inv2[pt.j];
-        }
+        const C inv1 = l2 ? (C)readlane(ca, l) : (C)0, inv2 = l2 ? (C)readlane(cb, l) : (C)0;
+        const C a = f2 * inv1, bb = f2 * inv2;
+        T* wr = A.W + (int64_t)i * A.ld;
+        T* hr = A.H + (int64_t)j * A.ld;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const C t = w[k];
@@ -164,8 +222,8 @@ __device__ __forceinline__ void run_block_seq(const SgdArgs<T>& A, int64_t s0, i
                     wn = w[k] - f1 * h[k];
                     hn = h[k] - f1 * t;
                 } else if (kind == UPD_NZSL_L2) {       // update-nzsl-l2.h:49-55
-                    wn = w[k] - (f1 * h[k] + f2 * w[k] * A.inv1[pt.i]);
-                    hn = h[k] - (f1 * t + f2 * h[k] * A.inv2[pt.j]);
+                    wn = w[k] - (f1 * h[k] + f2 * w[k] * inv1);
+                    hn = h[k] - (f1 * t + f2 * h[k] * inv2);
                 } else {                                // update-nzsl-nzl2.h:46-51
                     wn = w[k] - (f1 * h[k] + f2 * w[k]);
                     hn = h[k] - (f1 * t + f2 * h[k]);
@@ -193,7 +251,46 @@ __device__ __forceinline__ void run_block_seq(const SgdArgs<T>& A, int64_t s0, i
                 St<T>::store(wr + z, wn);
                 St<T>::store(hr + z, hn);
             }
+            rw[u][k] = St<T>::rt(wn);
+            rh[u][k] = St<T>::rt(hn);
         }
+        ri[u] = i;
+        rj[u] = j;
+        // the rows of point p + D, loaded after this point's stores
+        const int l2_ = l + D;
+        const int ni = l2_ < 64 ? (int)readlane_u32((uint32_t)cur.i, l2_) : (int)readlane_u32((uint32_t)nxt.i, l2_ - 64);
+        const int nj = l2_ < 64 ? (int)readlane_u32((uint32_t)cur.j, l2_) : (int)readlane_u32((uint32_t)nxt.j, l2_ - 64);
+        const bool live = p + D < n;
+        load_rows(live ? ni : 0, live ? nj : 0, qw[u], qh[u]);
+    };
+    for (int64_t c = 0; c < nch; ++c) {
+        // the next chunk's record (its position arrived a chunk ago), its
+        // rows' 1/nnz (its record arrived a chunk ago), the position two
+        // chunks ahead: issued here, consumed a chunk later
+        const uint32_t q3 = vis(c + 3);   // past the block: 0, never used
+        const Pt nn = rec(c + 2, q2);
+        if (l2) {
+            na = A.inv1[nxt.i];
+            nb = A.inv2[nxt.j];
+        }
+        for (int o = 0; o < 64 / D; ++o) {
+            // whole rounds run without branches around memory operations, which
+            // keeps the compiler's counts of outstanding loads exact (a prefetch is
+            // waited for only when it is used); the block's last round checks each point
+            if (c * 64 + (o + 1) * D <= n) {
+#pragma unroll
+                for (int u = 0; u < D; ++u) step(u, o * D + u, c * 64 + o * D + u);
+            } else {
+#pragma unroll
+                for (int u = 0; u < D; ++u)
+                    if (c * 64 + o * D + u < n) step(u, o * D + u, c * 64 + o * D + u);
+            }
+        }
+        cur = nxt;
+        ca = na;
+        cb = nb;
+        nxt = nn;
+        q2 = q3;
     }
 }
 
