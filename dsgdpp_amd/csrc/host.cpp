@@ -222,6 +222,7 @@ struct dsgdpp_ctx {
     bool lookahead = true;
     bool kprof = false;
     uint64_t dataVersion = 0;
+    int64_t maxBlock = 0;   // largest training block
 
     size_t factorBytes() const {
         return precision <= DSGDPP_FP64 ? 8 : precision == DSGDPP_FP32 ? 4 : 2;
@@ -490,6 +491,7 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
         for (size_t b = 0; b + 1 < c->off2.size(); ++b) maxCols = std::max<int>(maxCols, (int)(c->off2[b + 1] - c->off2[b]));
         c->G.max_rows = maxRows;
         c->G.max_cols = maxCols;
+        c->maxBlock = maxBlock;
         bool wide = false;
         c->groupedOk = grouped_supported(c->P, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
                                          c->kernel == 4);
@@ -805,6 +807,19 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
         return;
     }
     if (P.grouped) {
+        // blocks whose prep state does not fit in LDS keep it in a scratch buffer
+        const size_t spill = std::max(prep_spill_bytes(P.capSmall, G.max_rows, G.max_cols, P.nSmall),
+                                      prep_spill_bytes(maxBlock, G.max_rows, G.max_cols, nchains - P.nSmall));
+        if (spill > c->G.spill_bytes) {
+            HIPC(hipStreamSynchronize(s));   // a prep of the other plan may still use the old one
+            void* q = c->G.spill;
+            dfree(q);
+            c->G.spill = nullptr;
+            dalloc(c->G.spill, spill);
+            c->G.spill_bytes = spill;
+        }
+        G.spill = c->G.spill;
+        G.spill_bytes = c->G.spill_bytes;
         if (P.nSmall > 0)
             launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s);
         launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.nSmall, nchains - P.nSmall, c->sgdOrder, maxBlock, s);
@@ -1304,6 +1319,7 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->prepStream) (void)hipStreamSynchronize(c->prepStream);
     for (int k = 0; k < 2; ++k) { dfree(c->dPts[k]); void* p = c->dBlk[k]; dfree(p); }
+    { void* p = c->G.spill; dfree(p); c->G.spill = nullptr; }
     for (EpochPlan& P : c->plan) {
         void* q;
         q = P.dSched; dfree(q);
@@ -1728,6 +1744,10 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3 || c->kernel == 4))
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " cannot use the grouped kernel: " +
                                                       (c->groupedOk ? std::string("kernel forced") : c->groupedWhy));
+            if (prep_spill_bytes(c->maxBlock, c->G.max_rows, c->G.max_cols, 1) > 0)
+                throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
+                                                      ": its blocks' prep state does not fit in LDS (batched preps "
+                                                      "keep it there; use more tasks)");
             if (k == 0) {
                 b->device = c->device;
                 b->precision = c->precision;

@@ -108,6 +108,8 @@ struct GroupedBuffers {
     unsigned long long* prof = nullptr;   // optional kernel clock counters (5 per task)
     unsigned long long* pprof = nullptr;  // optional prep clock sums {visit order, scheduling, fill, visits}
     bool wide = false;                    // H rows streamed from HBM (blocks too wide for the LDS tile)
+    uint32_t* spill = nullptr;            // prep state of blocks too large for LDS (prep_spill_bytes)
+    size_t spill_bytes = 0;
 };
 // The prep kernel's inputs for one problem (many problems share one launch).
 struct PrepDesc {
@@ -130,6 +132,9 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
                    int order, int64_t max_block);
 bool prep_narrow(int64_t max_block);
 size_t prep_desc_lds(const PrepDesc& D);
+// scratch the prep of `nchains` chains of blocks up to max_block points needs
+// in global memory (0 when its state fits in LDS)
+size_t prep_spill_bytes(int64_t max_block, int max_rows, int max_cols, int nchains);
 void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,
                                hipStream_t s);
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,

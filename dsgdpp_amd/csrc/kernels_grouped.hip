@@ -324,6 +324,24 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                        dummy_row, dummy_col, pprof, sm, wide != 0);
 }
 
+// Blocks whose prep state does not fit in LDS: the same prep with its state
+// in a global scratch slot per workgroup; a fixed grid walks the chains.
+template <typename IX, typename PT>
+__global__ __launch_bounds__(64) void k_prep_grouped_spill(const PermJob* __restrict__ jobs,
+                                                           const int32_t* __restrict__ chain_start, int nchains,
+                                                           int order, const PT* __restrict__ pts,
+                                                           uint32_t* __restrict__ perm, uint4* __restrict__ grec,
+                                                           int32_t* __restrict__ ngroups, int cap, int rows_cap,
+                                                           int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
+                                                           unsigned long long* __restrict__ pprof, int wide,
+                                                           uint32_t* scratch, uint32_t slot_words) {
+    uint32_t* const sm = scratch + (size_t)blockIdx.x * slot_words;
+    for (int ch = blockIdx.x; ch < nchains; ch += gridDim.x)
+        prep_chain<IX, PT>(jobs, chain_start, ch, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
+                           dummy_row, dummy_col, pprof, sm, wide != 0);
+}
+constexpr int PREP_SPILL_GRID = 2048;   // workgroups of a spilled prep launch (at most)
+
 // The chains of many problems in one launch: workgroup -> (problem, chain) by
 // `cmap`, each problem's buffers and bounds from its descriptor.
 template <typename IX>
@@ -1362,7 +1380,6 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     if (P.m + FACTOR_TAIL_ROWS >= (1 << 24)) return no("2^24 rows or more (packed records)");
     int lmax = 0;
     if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return no("no device");
-    if (prep_lds(max_block, max_rows, max_cols, max_block < PREP_NARROW) > (size_t)lmax) return no("prep LDS too large");
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return no("no device");
     auto resident = [&](const void* fn, int threads, size_t bytes) {
@@ -1402,6 +1419,27 @@ static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const Pe
     HIPK(hipGetLastError());
 }
 
+size_t prep_spill_bytes(int64_t max_block, int max_rows, int max_cols, int nchains) {
+    const int64_t cap = std::max<int64_t>(max_block, 1);
+    if (nchains <= 0 || prep_lds(cap, max_rows, max_cols, cap < PREP_NARROW) <= PREP_LDS_PER_CU) return 0;
+    return prep_lds(cap, max_rows, max_cols, false) * (size_t)std::min(nchains, PREP_SPILL_GRID);
+}
+
+template <typename PT>
+static void prep_spill_t(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
+                         const int32_t* d_chain_start, int nchains, int order, int cap, hipStream_t s) {
+    const size_t slot = prep_lds(cap, B.max_rows, B.max_cols, false);
+    const int grid = std::min(nchains, PREP_SPILL_GRID);
+    if (!B.spill || B.spill_bytes < slot * (size_t)grid)
+        throw std::runtime_error("prep scratch smaller than prep_spill_bytes");
+    const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
+    k_prep_grouped_spill<uint32_t, PT><<<grid, 64, 0, s>>>(d_jobs, d_chain_start, nchains, order, (const PT*)P.pts,
+                                                           P.perm, (uint4*)B.grec, B.ngroups, cap, B.max_rows,
+                                                           B.max_cols, drow, dcol, B.pprof, B.wide ? 1 : 0, B.spill,
+                                                           (uint32_t)(slot / 4));
+    HIPK(hipGetLastError());
+}
+
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s) {
     if (nchains <= 0) return;
@@ -1409,6 +1447,11 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     const bool narrow = cap < PREP_NARROW;
     const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow);
     const bool f64 = P.precision == PREC_FP64 || P.precision == PREC_FP64_STRICT;
+    if (bytes > PREP_LDS_PER_CU) {   // the state goes to global scratch
+        if (f64) prep_spill_t<Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, s);
+        else prep_spill_t<Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, s);
+        return;
+    }
     if (f64) {
         if (narrow) prep_launch_t<uint16_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
         else prep_launch_t<uint32_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);

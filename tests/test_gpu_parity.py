@@ -410,6 +410,36 @@ def test_grouped_carry_over_and_many_tasks():
     np.testing.assert_array_equal(Hg, Ho)
 
 
+@pytest.mark.parametrize("m,n,nnz,kernel", [(120_000, 400, 300_000, 3), (20_000, 20_000, 300_000, 4)])
+def test_grouped_prep_state_in_global_memory(m, n, nnz, kernel):
+    """Blocks whose prep state (visit order, row and column tags) does not fit
+    in LDS: the prep keeps it in global scratch, for the tile kernel (tall
+    blocks, narrow column blocks) and the streaming kernel (wide column
+    blocks); factors equal the oracle's FP32 emulation bit for bit."""
+    print(f"case: prep state in global memory {m}x{n} nnz={nnz} -> kernel {kernel}", flush=True)
+    rng = np.random.default_rng(m % 97)
+    cells = np.unique(rng.integers(0, m * n, size=nnz + nnz // 10))[:nnz]
+    rng.shuffle(cells)
+    p = dict(m=m, n=n, train=(cells // n, cells % n, rng.normal(size=len(cells))), rank=16,
+             W0=rng.uniform(-0.5, 0.5, (m, 16)), H0=rng.uniform(-0.5, 0.5, (16, n)))
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=2, seed=17, precision="FP32")
+    ok, why = g.grouped_ok()
+    assert ok, why
+    o.set_arith("FP32")
+    for e in range(2):
+        st = g.epoch(0.02)
+        o.epoch(0.02)
+        assert st["kernel"] == kernel
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+    from dsgdpp_amd import batch as B
+    with pytest.raises(RuntimeError, match="prep state does not fit"):
+        B.DsgdPpBatch([g])
+    g.close()
+
+
 @pytest.mark.parametrize("sgd", ["WOR", "WR"])
 def test_grouped_prep_edge_blocks(sgd):
     """Blocks whose sizes sit on the prep's 64-point chunk boundaries (1, 2,
