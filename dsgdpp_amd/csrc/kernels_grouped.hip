@@ -1329,6 +1329,151 @@ __global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
     wide_task<KIND, TRUNC, VEC>(A, blockIdx.x, lds);
 }
 
+// FP64 column blocks too wide for the LDS tile: the streaming body above with
+// the FP64 layout of the tile kernel (4 doubles per lane, x as a double in the
+// record, the row's 1/nnz from the FP64 array with the W row, the column's
+// with the H row).  The W and H forwarding rings hold 64-double rows.
+static size_t epoch_lds_wide64() { return (size_t)2 * GROUP_RING * GROUP_SLOTS * RB64; }
+
+template <int KIND, bool TRUNC>
+__global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
+    extern __shared__ float lds_f[];
+    char* const wring = (char*)lds_f;
+    char* const hring = wring + GROUP_RING * GROUP_SLOTS * RB64;
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
+    const int e0 = 4 * l16;
+    const bool pad = e0 >= A.ld;   // whole 4-element chunks: ld is a multiple of 4
+    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * 8u;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * 8u : (uint32_t)e0 * 8u;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * 8u : (uint32_t)e0 * 8u;
+    const uint32_t hbase = pad ? OOB_OFF : (uint32_t)e0 * 8u;
+    const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
+    const __amdgpu_buffer_rsrc_t Ir = make_rsrc(A.inv1, A.i_bytes);   // rows past m read 0
+    const uint32_t rec_off = 16u * slot;
+    char* const wring_lane = wring + slot * RB64 + e0 * 8;
+    char* const hring_lane = hring + slot * RB64 + e0 * 8;
+    const double eps = A.eps;
+    const double c1 = eps * -2.;
+    const double f2 = (eps * 2.) * A.lambda;
+    const int d = A.d;
+    for (int e = lane; e < 2 * GROUP_RING * GROUP_SLOTS * RB64 / 16; e += 64)
+        ((uint4*)wring)[e] = make_uint4(0u, 0u, 0u, 0u);
+    unsigned long long c_wait = 0, c_loop = 0, n_groups = 0;
+    const unsigned long long c_begin = A.prof ? clock64() : 0;
+    D4 hprev{0., 0., 0., 0.};
+    for (int s = 0; s < 2 * d; ++s) {
+        const int visit = s * d + t;
+        const int b2 = uniform_i32((int)A.sched[visit]);
+        const int b = t * 2 * d + b2;
+        if (A.blk[b + 1] == A.blk[b]) continue;
+        const int ng = uniform_i32(A.ngroups[visit]);
+        const int ngp = (ng + GQ64 - 1) / GQ64 * GQ64;   // the prep pads to GROUP_PAD, a multiple of GQ64
+        const int64_t gbase = uniform_i64(A.goff[visit]);
+        const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
+        const int col0 = uniform_i32(A.off2[b2]);
+        const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
+        // the dummy column and the pad lanes fall past these: loads read 0, stores are dropped
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * 8u);
+        const __amdgpu_buffer_rsrc_t Jr = make_rsrc(A.inv2 + col0, (uint32_t)ncol * 8u);
+        uint4 qa[GQ64];
+        D4 qw[GPREF64], qh[GPREF64];
+        double qwi[GPREF64], qhi[GPREF64];
+        auto load_w = [&](const uint4& rec, D4& out, double& inv) {
+            out = bload_d4(Wr, __umul24(rec.x, ldb) + lbase);
+            if (KIND == UPD_NZSL_L2) {
+                const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(Ir, (int)__umul24(rec.x, 8u), 0, 0);
+                inv = dbl(u.x, u.y);
+            } else {
+                inv = 0.;
+            }
+        };
+        auto load_h = [&](const uint4& rec, D4& out, double& inv) {
+            out = bload_d4(Hr, __umul24(rec.y & 0xffffu, ldb) + hbase);
+            if (KIND == UPD_NZSL_L2) {
+                const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(Jr, (int)__umul24(rec.y & 0xffffu, 8u), 0, 0);
+                inv = dbl(u.x, u.y);
+            } else {
+                inv = 0.;
+            }
+        };
+#pragma unroll
+        for (int u = 0; u < GQ64; ++u) qa[u] = load_rec(Gr, (int)rec_off + 64 * u);
+#pragma unroll
+        for (int u = 0; u < GPREF64; ++u) load_w(qa[u], qw[u], qwi[u]);
+        const unsigned long long c0 = A.prof ? clock64() : 0;
+        dataflow_wait(A.ready, b2, A.need[visit]);   // H rows of this column block may be read from here on
+        const unsigned long long c2 = A.prof ? clock64() : 0;
+#pragma unroll
+        for (int u = 0; u < GPREF64; ++u) load_h(qa[u], qh[u], qhi[u]);
+        D4 wq = ld_d4(wring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
+        D4 hq = ld_d4(hring + ((qa[0].y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
+        for (int K0 = 0; K0 < ngp; K0 += GQ64) {
+            const int rbase = (int)rec_off + 64 * (K0 + GQ64);
+#pragma unroll
+            for (int u = 0; u < GQ64; ++u) {
+                const int K = K0 + u;
+                const uint4 rec = qa[u];
+                D4 w = qw[u % GPREF64], h = qh[u % GPREF64];
+                const double ai = qwi[u % GPREF64], ci = qhi[u % GPREF64];
+                const D4 fw = wq, fh = hq;
+                // ring operands of group K+1 (written by groups <= K-1)
+                const uint4 nrec = qa[(u + 1) % GQ64];
+                wq = ld_d4(wring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
+                hq = ld_d4(hring + ((nrec.y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
+                const bool wf = (int32_t)rec.x < 0;         // REC_WFWD
+                const bool hf = (rec.y & REC_HFWD) != 0;
+                const bool hr = (int32_t)rec.y < 0;         // REC_HREG
+                w.x = sel(wf, fw.x, w.x); w.y = sel(wf, fw.y, w.y); w.z = sel(wf, fw.z, w.z); w.w = sel(wf, fw.w, w.w);
+                h.x = sel(hr, hprev.x, sel(hf, fh.x, h.x)); h.y = sel(hr, hprev.y, sel(hf, fh.y, h.y));
+                h.z = sel(hr, hprev.z, sel(hf, fh.z, h.z)); h.w = sel(hr, hprev.w, sel(hf, fh.w, h.w));
+                double part = w.x * h.x;
+                part = fma(w.y, h.y, part);
+                part = fma(w.z, h.z, part);
+                part = fma(w.w, h.w, part);
+                const double wh = row16_sum_d(part);
+                const double f1 = c1 * (dbl(rec.z, rec.w) - wh);
+                double a = 0., bb = 0.;
+                if (KIND == UPD_NZSL_L2) { a = f2 * ai; bb = f2 * ci; }
+                if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+                D4 wn, hn;
+                wn.x = fma(-f1, h.x, fma(-a, w.x, w.x)); wn.y = fma(-f1, h.y, fma(-a, w.y, w.y));
+                wn.z = fma(-f1, h.z, fma(-a, w.z, w.z)); wn.w = fma(-f1, h.w, fma(-a, w.w, w.w));
+                hn.x = fma(-f1, w.x, fma(-bb, h.x, h.x)); hn.y = fma(-f1, w.y, fma(-bb, h.y, h.y));
+                hn.z = fma(-f1, w.z, fma(-bb, h.z, h.z)); hn.w = fma(-f1, w.w, fma(-bb, h.w, h.w));
+                if (TRUNC) {   // update-truncate.h:50-64
+#define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
+                    TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(wn.z); TRUNC_(wn.w);
+                    TRUNC_(hn.x); TRUNC_(hn.y); TRUNC_(hn.z); TRUNC_(hn.w);
+#undef TRUNC_
+                }
+                hprev = hn;
+                bstore_d4(Wr, __umul24(rec.x, ldb) + sbase, wn);
+                bstore_d4(Hr, __umul24(rec.y & 0xffffu, ldb) + hbase, hn);
+                st_d4(wring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB64), wn);
+                st_d4(hring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB64), hn);
+                load_w(qa[(u + GPREF64) % GQ64], qw[u % GPREF64], qwi[u % GPREF64]);
+                load_h(qa[(u + GPREF64) % GQ64], qh[u % GPREF64], qhi[u % GPREF64]);
+                qa[u] = load_rec(Gr, rbase + 64 * u);
+            }
+        }
+        if (A.prof) {
+            const unsigned long long c3 = clock64();
+            c_wait += c2 - c0;
+            c_loop += c3 - c2;
+            n_groups += (unsigned long long)ng;
+        }
+        dataflow_signal_wave(A.ready, b2);
+    }
+    if (A.prof && lane == 0) {
+        A.prof[5 * t + 0] = c_wait;
+        A.prof[5 * t + 1] = 0;
+        A.prof[5 * t + 2] = c_loop;
+        A.prof[5 * t + 3] = clock64() - c_begin;
+        A.prof[5 * t + 4] = n_groups;
+    }
+}
+
 // Many problems in one persistent launch: workgroup -> (problem, task) by
 // `tmap`; each problem's arguments are read from device memory, so a captured
 // launch replays with new step sizes and plans.
@@ -1393,6 +1538,15 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
                    : vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
                                        : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
     if (!force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
+    // FP64 blocks too wide for the LDS tile: H rows streamed from HBM
+    if (f64 && wide) {
+        const size_t lw = epoch_lds_wide64();
+        const void* fw = (const void*)k_epoch_wide64<UPD_NZSL_L2, false>;
+        if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
+            *wide = true;
+            return true;
+        }
+    }
     // FP32 blocks too wide for the LDS tile: H rows streamed from HBM
     if (P.precision == PREC_FP32 && wide) {
         const size_t lw = epoch_lds_wide(vec_of(P.ld));
@@ -1618,6 +1772,16 @@ template <int KIND, bool TRUNC>
 static void epoch_grouped64_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                               const EpochLaunch& L, hipStream_t s) {
     GArgs64 A = make_gargs64(P, U, B, L);
+    if (B.wide) {   // H rows streamed from HBM
+        const size_t bytes = epoch_lds_wide64();
+        const void* fn = (const void*)k_epoch_wide64<KIND, TRUNC>;
+        HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+        k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+        HIPK(hipGetLastError());
+        void* args[] = {&A};
+        launch_cooperative(fn, dim3(P.d), dim3(64), args, bytes, s, "the grouped epoch kernel (FP64, H streamed)");
+        return;
+    }
     const size_t bytes = epoch_lds64(B.max_cols);
     const void* fn = (const void*)k_epoch_grouped64<KIND, TRUNC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));

@@ -290,6 +290,43 @@ def test_grouped_fp64_bitexact_vs_emulation(rank, update, truncate):
         assert abs(lg - lr) <= 1e-9 * abs(lr), (e, lg, lr)
 
 
+@pytest.mark.parametrize("rank,update,truncate", [(50, "Nzsl_L2(0.05)", None), (16, "Nzsl", None),
+                                                  (64, "Nzsl_Nzl2(0.02)", None), (37, "Nzsl_L2(0.05)", "(-0.4,0.4)")])
+def test_grouped_fp64_wide_bitexact_vs_emulation(rank, update, truncate):
+    """The FP64 streaming body (H rows from HBM, kernel 4) against the same
+    double association as the tile kernel; a shape whose column blocks are
+    too wide for the FP64 tile takes it by itself."""
+    print(f"case: grouped FP64 wide rank {rank} {update} {truncate}", flush=True)
+    p = skewed_problem(300, 200, 12000, rank, seed=131 + rank)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=47, precision="FP64", kernel=4)
+    o.set_arith("FP64_GROUPED")
+    for e in range(3):
+        st = g.epoch(0.01)
+        assert st["kernel"] == 4
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+    g.close()
+    if rank == 16:   # 2 x 4 blocks of 3,000 columns: the FP64 tile does not fit
+        rng = np.random.default_rng(9)
+        m, n, nnz = 6_000, 12_000, 60_000
+        cells = np.unique(rng.integers(0, m * n, size=nnz + nnz // 10))[:nnz]
+        rng.shuffle(cells)
+        q = dict(m=m, n=n, train=(cells // n, cells % n, rng.normal(size=len(cells))), rank=rank,
+                 W0=rng.uniform(-0.5, 0.5, (m, rank)), H0=rng.uniform(-0.5, 0.5, (rank, n)))
+        g, o = make_pair(q, update=update, tasks=2, seed=21, precision="FP64")
+        o.set_arith("FP64_GROUPED")
+        for e in range(2):
+            st = g.epoch(0.01)
+            o.epoch(0.01)
+            assert st["kernel"] == 4
+            np.testing.assert_array_equal(g.get_factors()[0], o.get_factors()[0], err_msg=f"wide shape, epoch {e}")
+            np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"wide shape, epoch {e}")
+        g.close()
+
+
 @pytest.mark.parametrize("rank,update,truncate", [(50, "Nzsl_L2(0.05)", None), (100, "Nzsl_L2(0.05)", None),
                                                   (16, "Nzsl", None), (72, "Nzsl_Nzl2(0.02)", "(-0.4,0.4)")])
 def test_grouped_wide_bitexact_vs_emulation(rank, update, truncate):
