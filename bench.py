@@ -358,7 +358,7 @@ def report_precision(args):
 def report_time_to_rmse(args):
     """Wall time to reach the plain FP64 GPU run's final test RMSE (+0.5%):
     plain DSGD++ (FP64 factors and math, BoldDriver) for the full run,
-    then FP32 factors with BoldDriver and FP32 factors with the learned step
+    then FP32 and BF16 factors, each with BoldDriver and with the learned step
     controller, each stopping at the target.  Every epoch evaluates the
     training and test loss, as the reference runner does."""
     import torch
@@ -391,10 +391,11 @@ def report_time_to_rmse(args):
     log(f"[time-to-rmse] FP64 plain: {len(h64)} epochs, {h64[-1][0]:.2f}s, test rmse {h64[-1][1]:.5f}; "
         f"target {target:.5f}")
     t64 = next(t for t, r in h64 if r <= target)
-    h32 = run("FP32", D.BoldDriver(decay0), target)
-    hl = run("FP32", controller.LearnedDecay(decay0), target)
+    arms = [(f"{prec.lower()}_{kind}", run(prec, rule(decay0), target))
+            for prec in ("FP32", "BF16")
+            for kind, rule in (("bolddriver", D.BoldDriver), ("learned", controller.LearnedDecay))]
     res = {}
-    for name, h in (("fp32_bolddriver", h32), ("fp32_learned", hl)):
+    for name, h in arms:
         hit = [t for t, r in h if r <= target]
         res[name] = {"epochs": len(h), "seconds": hit[0] if hit else None, "final_test_rmse": h[-1][1],
                      "reached": bool(hit), "speedup_vs_fp64": (t64 / hit[0]) if hit else None}
@@ -419,8 +420,10 @@ def report_controller(args):
     (its best test RMSE + 0.5%: with rank 100 and noise 5 the test RMSE turns up
     after a few epochs); the learned controller, started from the same step,
     runs as many epochs.  Both evaluate training and test loss after every
-    epoch; reported are the times to the target and each run's best RMSE.  The column blocks are too wide for an LDS tile, so
-    epochs run on the dataflow kernel (d = 512 by default)."""
+    epoch; reported are the times to the target and each run's best RMSE.
+    d = 256 by default: the column blocks are too wide for an LDS tile (H rows
+    stream from HBM, kernel 4) and the largest blocks keep their prep state in
+    global scratch; one task per CU."""
     import torch
     import dsgdpp_amd as D
     from dsgdpp_amd import controller, data
@@ -429,7 +432,7 @@ def report_controller(args):
     p = data.controller_250m(device="cuda", shrink=args.shrink)
     torch.cuda.synchronize()
     log(f"[controller] generated {len(p['train'][2])} nonzeros in {time.time() - t0:.1f}s")
-    update, decay0, tasks = "Nzsl_L2(0.05)", 0.001, args.tasks or 512
+    update, decay0, tasks = "Nzsl_L2(0.05)", 0.001, args.tasks or 256
     nt = len(p["test"][2])
 
     def run(name, rule, target=None):
