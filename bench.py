@@ -194,7 +194,8 @@ def main():
     print(json.dumps(out))
 
 
-KERNEL_NAMES = {1: "k_stratum_seq", 2: "k_epoch_dataflow_seq", 3: "k_epoch_grouped"}
+KERNEL_NAMES = {1: "k_stratum_seq", 2: "k_epoch_dataflow_seq", 3: "k_epoch_grouped",
+                4: "k_epoch_wide (FP64: k_epoch_wide64)"}
 
 
 def calibrate_copy():

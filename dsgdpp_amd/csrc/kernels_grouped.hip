@@ -1335,7 +1335,21 @@ __global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
 // with the H row).  The W and H forwarding rings hold 64-double rows.
 static size_t epoch_lds_wide64() { return (size_t)2 * GROUP_RING * GROUP_SLOTS * RB64; }
 
-template <int KIND, bool TRUNC>
+// a double from lane `src` (per lane)
+__device__ __forceinline__ double lane_gather_d(double v, int src) {
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// STRICT: every update in the reference's association (update-nzsl*.h), as
+// the sequential kernel computes it: wh summed left to right over z (a shift
+// chain along the slot's lanes, each adding its 4 products to the running sum
+// of the lane before it), products and sums rounded one by one (no FMA).
+// Grouping only reorders updates that share no row or column, so the result
+// is the sequential one.  Padding elements are +0 products, and the running
+// sum, started at +0, never is -0: adding them changes nothing.
+template <int KIND, bool TRUNC, bool STRICT>
 __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
     extern __shared__ float lds_f[];
     char* const wring = (char*)lds_f;
@@ -1357,6 +1371,7 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
     const double c1 = eps * -2.;
     const double f2 = (eps * 2.) * A.lambda;
     const int d = A.d;
+    const int nl = A.ld / 4;   // lanes of a slot that hold the row
     for (int e = lane; e < 2 * GROUP_RING * GROUP_SLOTS * RB64 / 16; e += 64)
         ((uint4*)wring)[e] = make_uint4(0u, 0u, 0u, 0u);
     unsigned long long c_wait = 0, c_loop = 0, n_groups = 0;
@@ -1427,20 +1442,49 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
                 w.x = sel(wf, fw.x, w.x); w.y = sel(wf, fw.y, w.y); w.z = sel(wf, fw.z, w.z); w.w = sel(wf, fw.w, w.w);
                 h.x = sel(hr, hprev.x, sel(hf, fh.x, h.x)); h.y = sel(hr, hprev.y, sel(hf, fh.y, h.y));
                 h.z = sel(hr, hprev.z, sel(hf, fh.z, h.z)); h.w = sel(hr, hprev.w, sel(hf, fh.w, h.w));
-                double part = w.x * h.x;
-                part = fma(w.y, h.y, part);
-                part = fma(w.z, h.z, part);
-                part = fma(w.w, h.w, part);
-                const double wh = row16_sum_d(part);
-                const double f1 = c1 * (dbl(rec.z, rec.w) - wh);
-                double a = 0., bb = 0.;
-                if (KIND == UPD_NZSL_L2) { a = f2 * ai; bb = f2 * ci; }
-                if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
                 D4 wn, hn;
-                wn.x = fma(-f1, h.x, fma(-a, w.x, w.x)); wn.y = fma(-f1, h.y, fma(-a, w.y, w.y));
-                wn.z = fma(-f1, h.z, fma(-a, w.z, w.z)); wn.w = fma(-f1, h.w, fma(-a, w.w, w.w));
-                hn.x = fma(-f1, w.x, fma(-bb, h.x, h.x)); hn.y = fma(-f1, w.y, fma(-bb, h.y, h.y));
-                hn.z = fma(-f1, w.z, fma(-bb, h.z, h.z)); hn.w = fma(-f1, w.w, fma(-bb, h.w, h.w));
+                if constexpr (STRICT) {
+                    const double px = w.x * h.x, py = w.y * h.y, pz = w.z * h.z, pw = w.w * h.w;
+                    double acc = 0.;
+                    for (int l = 0; l < nl; ++l) {
+                        const double prev = dpp_d<0x111>(acc);   // row_shr:1 (lane 0 of the row: +0)
+                        const double s4 = (((prev + px) + py) + pz) + pw;
+                        acc = l16 == l ? s4 : acc;
+                    }
+                    const double wh = lane_gather_d(acc, 16 * slot + nl - 1);
+                    const double f1 = c1 * (dbl(rec.z, rec.w) - wh);
+                    auto upd = [&](double wv, double hv, double& wo, double& ho) {
+                        const double t = wv;
+                        if (KIND == UPD_NZSL) {                 // update-nzsl.h:44-48
+                            wo = wv - f1 * hv;
+                            ho = hv - f1 * t;
+                        } else if (KIND == UPD_NZSL_L2) {       // update-nzsl-l2.h:49-55
+                            wo = wv - (f1 * hv + f2 * wv * ai);
+                            ho = hv - (f1 * t + f2 * hv * ci);
+                        } else {                                // update-nzsl-nzl2.h:46-51
+                            wo = wv - (f1 * hv + f2 * wv);
+                            ho = hv - (f1 * t + f2 * hv);
+                        }
+                    };
+                    upd(w.x, h.x, wn.x, hn.x);
+                    upd(w.y, h.y, wn.y, hn.y);
+                    upd(w.z, h.z, wn.z, hn.z);
+                    upd(w.w, h.w, wn.w, hn.w);
+                } else {
+                    double part = w.x * h.x;
+                    part = fma(w.y, h.y, part);
+                    part = fma(w.z, h.z, part);
+                    part = fma(w.w, h.w, part);
+                    const double wh = row16_sum_d(part);
+                    const double f1 = c1 * (dbl(rec.z, rec.w) - wh);
+                    double a = 0., bb = 0.;
+                    if (KIND == UPD_NZSL_L2) { a = f2 * ai; bb = f2 * ci; }
+                    if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+                    wn.x = fma(-f1, h.x, fma(-a, w.x, w.x)); wn.y = fma(-f1, h.y, fma(-a, w.y, w.y));
+                    wn.z = fma(-f1, h.z, fma(-a, w.z, w.z)); wn.w = fma(-f1, h.w, fma(-a, w.w, w.w));
+                    hn.x = fma(-f1, w.x, fma(-bb, h.x, h.x)); hn.y = fma(-f1, w.y, fma(-bb, h.y, h.y));
+                    hn.z = fma(-f1, w.z, fma(-bb, h.z, h.z)); hn.w = fma(-f1, w.w, fma(-bb, h.w, h.w));
+                }
                 if (TRUNC) {   // update-truncate.h:50-64
 #define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
                     TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(wn.z); TRUNC_(wn.w);
@@ -1512,8 +1556,8 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
         return false;
     };
     if (wide) *wide = false;
-    const bool f64 = P.precision == PREC_FP64;
-    if (P.precision == PREC_FP64_STRICT) return no("FP64_STRICT keeps the reference association (dataflow kernel)");
+    const bool strict = P.precision == PREC_FP64_STRICT;
+    const bool f64 = P.precision == PREC_FP64 || strict;
     if (f64 && (P.ld > 64 || P.ld % 4 != 0)) return no("FP64 grouped kernel needs rank <= 64, rows padded to 4");
     if (P.ld > 128) return no("grouped kernel needs rank <= 128");
     if (P.ld > 64 && P.ld % 8 != 0) return no("rows above rank 64 must be padded to 8 elements");
@@ -1537,11 +1581,13 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     const void* fn = f64 ? (const void*)k_epoch_grouped64<UPD_NZSL_L2, false>
                    : vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
                                        : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
-    if (!force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
+    // FP64_STRICT (the reference association) runs on the streaming body only
+    if (!strict && !force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
     // FP64 blocks too wide for the LDS tile: H rows streamed from HBM
     if (f64 && wide) {
         const size_t lw = epoch_lds_wide64();
-        const void* fw = (const void*)k_epoch_wide64<UPD_NZSL_L2, false>;
+        const void* fw = strict ? (const void*)k_epoch_wide64<UPD_NZSL_L2, false, true>
+                                : (const void*)k_epoch_wide64<UPD_NZSL_L2, false, false>;
         if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
             *wide = true;
             return true;
@@ -1774,7 +1820,8 @@ static void epoch_grouped64_t(const DevProblem& P, const UpdParams& U, const Gro
     GArgs64 A = make_gargs64(P, U, B, L);
     if (B.wide) {   // H rows streamed from HBM
         const size_t bytes = epoch_lds_wide64();
-        const void* fn = (const void*)k_epoch_wide64<KIND, TRUNC>;
+        const void* fn = P.precision == PREC_FP64_STRICT ? (const void*)k_epoch_wide64<KIND, TRUNC, true>
+                                                          : (const void*)k_epoch_wide64<KIND, TRUNC, false>;
         HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
         k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
         HIPK(hipGetLastError());
@@ -1802,14 +1849,16 @@ void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const Grouped
                           const EpochLaunch& L, hipStream_t s) {
     if (P.precision == PREC_FP32) epoch_grouped_p<float>(P, U, B, L, s);
     else if (P.precision == PREC_BF16) epoch_grouped_p<bf16_t>(P, U, B, L, s);
-    else if (P.precision == PREC_FP64) {
+    else if (P.precision == PREC_FP64 || P.precision == PREC_FP64_STRICT) {
+        if (P.precision == PREC_FP64_STRICT && !B.wide)
+            throw std::invalid_argument("FP64_STRICT grouped epochs run on the H-streaming kernel");
         switch (U.kind) {
             case UPD_NZSL: epoch_grouped64_k<UPD_NZSL>(P, U, B, L, s); break;
             case UPD_NZSL_L2: epoch_grouped64_k<UPD_NZSL_L2>(P, U, B, L, s); break;
             case UPD_NZSL_NZL2: epoch_grouped64_k<UPD_NZSL_NZL2>(P, U, B, L, s); break;
             default: throw std::invalid_argument("unknown update kind");
         }
-    } else throw std::invalid_argument("grouped kernel runs FP64/FP32/BF16 factors (not FP64_STRICT)");
+    } else throw std::invalid_argument("unknown precision");
 }
 
 // ---------------------------------------------------------------------------

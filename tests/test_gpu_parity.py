@@ -384,14 +384,38 @@ def test_wide_column_blocks_choose_the_streaming_kernel():
     bt.close()
 
 
-def test_fp64_strict_stays_on_the_reference_association():
-    p = skewed_problem(100, 80, 2000, 8, seed=3)
-    D = _dsgdpp()
-    g = D.DsgdPp(update="Nzsl_L2(0.05)", loss="Nzsl_L2(0.05)", rank=8, tasks_per_rank=2, precision="FP64_STRICT")
-    g.load_coo(0, p["m"], p["n"], *p["train"])
+@pytest.mark.parametrize("rank,update,truncate,sgd", [(8, "Nzsl_L2(0.05)", None, "WOR"), (10, "Nzsl", None, "WR"),
+                                                      (13, "Nzsl_Nzl2(0.02)", None, "SEQ"),
+                                                      (37, "Nzsl_L2(0.05)", "(-0.4,0.4)", "WOR"),
+                                                      (64, "Nzsl_L2(0.05)", None, "WOR")])
+def test_fp64_strict_grouped_equals_reference_association(rank, update, truncate, sgd):
+    """FP64_STRICT on the grouped streaming kernel (the reference's per-update
+    association; groups only reorder updates that share no row or column):
+    bit-identical to the oracle's reference arithmetic and to the sequential
+    kernel (kernel 2) on the GPU."""
+    print(f"case: FP64_STRICT grouped rank {rank} {update} {truncate} {sgd}", flush=True)
+    p = skewed_problem(300, 200, 12000, rank, seed=57 + rank)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=61, sgd=sgd, precision="FP64_STRICT")
+    s, _ = make_pair(p, update=update, truncate=truncate, tasks=4, seed=61, sgd=sgd, precision="FP64_STRICT",
+                     kernel=2)
     ok, why = g.grouped_ok()
-    assert not ok and "FP64_STRICT" in why
+    assert ok, why
+    for e in range(3):
+        st = g.epoch(0.01)
+        assert st["kernel"] == 4
+        assert s.epoch(0.01)["kernel"] == 2
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        Ws, Hs = s.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Wg, Ws, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Hs, err_msg=f"epoch {e}")
+        lg, lo = g.loss(0), o.loss(0)
+        assert abs(lg - lo) <= 1e-12 * abs(lo), (e, lg, lo)   # the loss kernel's reduction order
     g.close()
+    s.close()
 
 
 @pytest.mark.parametrize("rank", [1, 13, 32, 64])
