@@ -53,7 +53,8 @@ void usage() {
             "  --balance             type of balancing (None, L2, NZL2)\n"
             "  --precision           FP64_STRICT (default), FP64, FP32 or BF16\n"
             "  --device              HIP device ordinal [0]\n"
-            "  --kernel              stratum kernel: 0 auto, 1 sequential wave per block\n");
+            "  --kernel              epoch kernel: 0 auto, 1 one launch per stratum, 2 dataflow,\n"
+            "                        3 grouped, 4 grouped with H streamed from HBM\n");
 }
 
 }  // namespace
