@@ -159,7 +159,7 @@ def block_cap(nnz, d):
 
 def min_tasks(p, wide=True):
     """Smallest power-of-two d whose prep state fits in LDS and whose H tiles
-    fit too, unless the kernel may stream H rows (`wide`, FP32).  Blocks are
+    fit too, unless the kernel may stream H rows (`wide`).  Blocks are
     further kept below PREP_NARROW points while the shape allows (d <= m,
     2d <= n): the next epoch's prep runs beside the epoch launches, and the
     32-bit prep of larger blocks holds a CU's LDS with one wave, at several
@@ -174,8 +174,8 @@ def min_tasks(p, wide=True):
 
 def plan_tasks(problems, budget, rates, wide=True):
     """Tasks per problem.  problems: list of dicts with m, n, nnz, rank.
-    `wide`: the problems are FP32, so column blocks too wide for the LDS tile
-    may run with H rows streamed from HBM (BF16 needs the tile).
+    `wide`: column blocks too wide for the LDS tile may run with H rows
+    streamed from HBM (FP32 and BF16).
     Greedy: start at the smallest d the LDS allows (min_tasks); double d of
     the problem with the largest predicted epoch time nnz / (d * rate(d)) while
     the total stays within `budget` and the problem has at least d rows and

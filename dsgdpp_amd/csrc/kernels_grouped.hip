@@ -1152,18 +1152,19 @@ constexpr uint32_t OOB_OFF = 0x7ffffff0u;   // past any buffer: loads read 0, st
 
 static size_t epoch_lds_wide(int vec) { return (size_t)2 * GROUP_RING * GROUP_SLOTS * 64 * vec * sizeof(float); }
 
-template <int KIND, bool TRUNC, int VEC>
-__device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, float* lds) {
+template <typename T, int KIND, bool TRUNC, int VEC>
+__device__ __forceinline__ void wide_task(const GArgs<T>& A, const int t, float* lds) {
     constexpr int ROW = 64 * VEC;
     float* const wring = lds;
     float* const hring = lds + GROUP_RING * GROUP_SLOTS * ROW;
     const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
     const int e0 = 4 * VEC * l16;
+    constexpr uint32_t ES = Vec4<T>::BYTES;
     const bool pad = e0 >= A.ld;
-    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * 4u;
-    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * 4u : (uint32_t)e0 * 4u;
-    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * 4u : (uint32_t)e0 * 4u;
-    const uint32_t hbase = pad ? OOB_OFF : (uint32_t)e0 * 4u;
+    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * ES;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * ES : (uint32_t)e0 * ES;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * ES : (uint32_t)e0 * ES;
+    const uint32_t hbase = pad ? OOB_OFF : (uint32_t)e0 * ES;
     const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
     const uint32_t rec_off = 16u * slot;
     float* const wring_lane = wring + slot * ROW + e0;
@@ -1189,20 +1190,21 @@ __device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, fl
         const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
         const int col0 = uniform_i32(A.off2[b2]);
         const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
-        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * 4u);
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * ES);
         const __amdgpu_buffer_rsrc_t Ir = make_rsrc(A.inv2 + col0, (uint32_t)ncol * 4u);
         uint4 qa[GQ_W];
-        float4 qw[GPREF_W][VEC], qh[GPREF_W][VEC];
+        typedef typename Vec4<T>::Raw Raw;   // prefetched rows stay as loaded until their group
+        Raw qw[GPREF_W][VEC], qh[GPREF_W][VEC];
         float qi[GPREF_W];
-        auto load_w = [&](const uint4& rec, float4* out) {
+        auto load_w = [&](const uint4& rec, Raw* out) {
             const uint32_t o = __umul24(rec.x, ldb) + lbase;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) out[v] = Vec4<float>::bload(Wr, o + 16 * v);
+            for (int v = 0; v < VEC; ++v) out[v] = Vec4<T>::braw(Wr, o + 4 * v * ES);
         };
-        auto load_h = [&](const uint4& rec, float4* out, float& inv) {
+        auto load_h = [&](const uint4& rec, Raw* out, float& inv) {
             const uint32_t o = __umul24(rec.y, ldb) + hbase;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) out[v] = Vec4<float>::bload(Hr, o + 16 * v);
+            for (int v = 0; v < VEC; ++v) out[v] = Vec4<T>::braw(Hr, o + 4 * v * ES);
             if (KIND == UPD_NZSL_L2)
                 inv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(Ir, (int)__umul24(rec.y, 4u), 0, 0));
             else
@@ -1232,8 +1234,8 @@ __device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, fl
                 float4 w[VEC], h[VEC], fw[VEC], fh[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
-                    w[v] = qw[u % GPREF_W][v];
-                    h[v] = qh[u % GPREF_W][v];
+                    w[v] = Vec4<T>::cook(qw[u % GPREF_W][v]);
+                    h[v] = Vec4<T>::cook(qh[u % GPREF_W][v]);
                     fw[v] = wq[v];
                     fh[v] = hq[v];
                 }
@@ -1274,6 +1276,7 @@ __device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, fl
                 if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = f2 * icol; }
                 if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
                 float4 wn[VEC], hn[VEC];
+                Raw wraw[VEC], hraw[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     wn[v].x = fmaf(-f1, h[v].x, fmaf(-a, w[v].x, w[v].x));
@@ -1290,14 +1293,16 @@ __device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, fl
                         TRUNC_(hn[v].x); TRUNC_(hn[v].y); TRUNC_(hn[v].z); TRUNC_(hn[v].w);
 #undef TRUNC_
                     }
+                    wn[v] = Vec4<T>::round4(wn[v], wraw[v]);   // the values once stored
+                    hn[v] = Vec4<T>::round4(hn[v], hraw[v]);
                     hprev[v] = hn[v];
                 }
                 const uint32_t so = __umul24(rec.x, ldb) + sbase;
                 const uint32_t sh = __umul24(rec.y, ldb) + hbase;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
-                    Vec4<float>::bstore(Wr, so + 16 * v, wn[v]);
-                    Vec4<float>::bstore(Hr, sh + 16 * v, hn[v]);
+                    Vec4<T>::bstore_raw(Wr, so + 4 * v * ES, wraw[v]);
+                    Vec4<T>::bstore_raw(Hr, sh + 4 * v * ES, hraw[v]);
                     *(float4*)(wring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = wn[v];
                     *(float4*)(hring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = hn[v];
                 }
@@ -1323,10 +1328,10 @@ __device__ __forceinline__ void wide_task(const GArgs<float>& A, const int t, fl
     }
 }
 
-template <int KIND, bool TRUNC, int VEC>
-__global__ __launch_bounds__(64) void k_epoch_wide(GArgs<float> A) {
+template <typename T, int KIND, bool TRUNC, int VEC>
+__global__ __launch_bounds__(64) void k_epoch_wide(GArgs<T> A) {
     extern __shared__ float lds[];
-    wide_task<KIND, TRUNC, VEC>(A, blockIdx.x, lds);
+    wide_task<T, KIND, TRUNC, VEC>(A, blockIdx.x, lds);
 }
 
 // FP64 column blocks too wide for the LDS tile: the streaming body above with
@@ -1531,11 +1536,11 @@ __global__ __launch_bounds__(128) void k_epoch_grouped_batch(const GArgs<T>* __r
     uint32_t* dst = (uint32_t*)&A;
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(GArgs<T>) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
-    if constexpr (std::is_same<T, float>::value) {
+    {
         if (A.wide) {   // the whole workgroup takes this branch; one wave does the task
             if (threadIdx.x < 64) {
-                if (A.ld <= 64) wide_task<KIND, TRUNC, 1>(A, t, lds);
-                else wide_task<KIND, TRUNC, 2>(A, t, lds);
+                if (A.ld <= 64) wide_task<T, KIND, TRUNC, 1>(A, t, lds);
+                else wide_task<T, KIND, TRUNC, 2>(A, t, lds);
             }
             return;
         }
@@ -1593,11 +1598,15 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
             return true;
         }
     }
-    // FP32 blocks too wide for the LDS tile: H rows streamed from HBM
-    if (P.precision == PREC_FP32 && wide) {
+    // FP32/BF16 blocks too wide for the LDS tile: H rows streamed from HBM
+    if ((P.precision == PREC_FP32 || P.precision == PREC_BF16) && wide) {
         const size_t lw = epoch_lds_wide(vec_of(P.ld));
-        const void* fw = vec_of(P.ld) == 1 ? (const void*)k_epoch_wide<UPD_NZSL_L2, false, 1>
-                                           : (const void*)k_epoch_wide<UPD_NZSL_L2, false, 2>;
+        const bool bf = P.precision == PREC_BF16;
+        const void* fw = vec_of(P.ld) == 1
+                             ? (bf ? (const void*)k_epoch_wide<bf16_t, UPD_NZSL_L2, false, 1>
+                                   : (const void*)k_epoch_wide<float, UPD_NZSL_L2, false, 1>)
+                             : (bf ? (const void*)k_epoch_wide<bf16_t, UPD_NZSL_L2, false, 2>
+                                   : (const void*)k_epoch_wide<float, UPD_NZSL_L2, false, 2>);
         if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
             *wide = true;
             return true;
@@ -1740,12 +1749,12 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     launch_cooperative(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
 }
 
-template <int KIND, bool TRUNC, int VEC>
+template <typename T, int KIND, bool TRUNC, int VEC>
 static void epoch_wide_t(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
                          hipStream_t s) {
-    GArgs<float> A = make_gargs<float>(P, U, B, L);
+    GArgs<T> A = make_gargs<T>(P, U, B, L);
     const size_t bytes = epoch_lds_wide(VEC);
-    const void* fn = (const void*)k_epoch_wide<KIND, TRUNC, VEC>;
+    const void* fn = (const void*)k_epoch_wide<T, KIND, TRUNC, VEC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
     HIPK(hipGetLastError());
@@ -1756,12 +1765,10 @@ static void epoch_wide_t(const DevProblem& P, const UpdParams& U, const GroupedB
 template <typename T, int KIND, bool TRUNC>
 static void epoch_grouped_v(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                             const EpochLaunch& L, hipStream_t s) {
-    if constexpr (std::is_same<T, float>::value) {
-        if (B.wide) {
-            if (vec_of(P.ld) == 1) epoch_wide_t<KIND, TRUNC, 1>(P, U, B, L, s);
-            else epoch_wide_t<KIND, TRUNC, 2>(P, U, B, L, s);
-            return;
-        }
+    if (B.wide) {
+        if (vec_of(P.ld) == 1) epoch_wide_t<T, KIND, TRUNC, 1>(P, U, B, L, s);
+        else epoch_wide_t<T, KIND, TRUNC, 2>(P, U, B, L, s);
+        return;
     }
     if (vec_of(P.ld) == 1) epoch_grouped_t<T, KIND, TRUNC, 1>(P, U, B, L, s);
     else epoch_grouped_t<T, KIND, TRUNC, 2>(P, U, B, L, s);

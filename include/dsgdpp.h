@@ -78,7 +78,7 @@ typedef struct dsgdpp_cfg {
                                 1 = one launch per stratum, one wave per block, sequential;
                                 2 = dataflow (one persistent launch per epoch), sequential blocks;
                                 3 = grouped dataflow (FP32/BF16 up to rank 128, FP64 up to rank 64);
-                                4 = grouped dataflow with H rows streamed from HBM (FP32, FP64;
+                                4 = grouped dataflow with H rows streamed from HBM (all precisions;
                                     chosen by 0 and 3 when a column block is too wide for the LDS
                                     tile).  Blocks whose prep state exceeds LDS keep it in global
                                     scratch (any grouped kernel). */
@@ -164,8 +164,8 @@ int dsgdpp_get_task_seeds(dsgdpp_ctx* ctx, uint64_t* out);
 int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint32_t* out,
                            int64_t cap, int64_t* n);
 /* Whether epochs of this context can use the grouped kernel (FP32/BF16 up to
- * rank 128, FP64 up to rank 64 with rows padded to 4; the H tile in LDS or,
- * for FP32/FP64, H streamed from HBM); `why` (optional) gets the reason when
+ * rank 128, FP64 and FP64_STRICT up to rank 64 with rows padded to 4; the H
+ * tile in LDS or H streamed from HBM); `why` (optional) gets the reason when
  * not.  No counterpart in the reference (a kernel-choice query). */
 int dsgdpp_grouped_ok(dsgdpp_ctx* ctx, int32_t* ok, char* why, int64_t cap);
 /* nnz per block (d x 2d, row-major) and dims {m, n, r, d, nnz_train, nnz_test} */
@@ -177,8 +177,8 @@ int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
  * factorization at a time: dsgdpp_impl.h:282-349).  `count` contexts --
  * independent problems of any shape, rank <= 128, own lambda -- sharing the
  * device, precision (FP32 or BF16), update kind and truncation advance one
- * epoch each in ONE persistent launch of the grouped kernel (an FP32 context
- * whose column blocks are too wide for the LDS tile runs the H-streaming body,
+ * epoch each in ONE persistent launch of the grouped kernel (a context whose
+ * column blocks are too wide for the LDS tile runs the H-streaming body,
  * kernel 4, inside the same launch); every context's result is bit-identical
  * to dsgdpp_epoch on it alone.  With use_graph the
  * launch is captured once and replayed each epoch.  The sum of the contexts'

@@ -327,16 +327,18 @@ def test_grouped_fp64_wide_bitexact_vs_emulation(rank, update, truncate):
         g.close()
 
 
+@pytest.mark.parametrize("precision", ["FP32", "BF16"])
 @pytest.mark.parametrize("rank,update,truncate", [(50, "Nzsl_L2(0.05)", None), (100, "Nzsl_L2(0.05)", None),
                                                   (16, "Nzsl", None), (72, "Nzsl_Nzl2(0.02)", "(-0.4,0.4)")])
-def test_grouped_wide_bitexact_vs_emulation(rank, update, truncate):
+def test_grouped_wide_bitexact_vs_emulation(rank, update, truncate, precision):
     """The grouped kernel with H rows streamed from HBM (kernel 4: H
-    prefetch, H forwarding ring, no LDS tile) against the oracle's FP32
-    emulation, bitwise: a missed H forwarding shows up as a bit difference."""
-    print(f"case: grouped wide rank {rank} {update} {truncate}", flush=True)
+    prefetch, H forwarding ring, no LDS tile) against the oracle's FP32 or
+    BF16 emulation, bitwise: a missed H forwarding (or a BF16 value used
+    before its rounding) shows up as a bit difference."""
+    print(f"case: grouped wide {precision} rank {rank} {update} {truncate}", flush=True)
     p = skewed_problem(300, 200, 12000, rank, seed=77 + rank)
-    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=45, precision="FP32", kernel=4)
-    o.set_arith("FP32")
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=45, precision=precision, kernel=4)
+    o.set_arith(precision)
     for e in range(3):
         st = g.epoch(0.01)
         assert st["kernel"] == 4
