@@ -1338,7 +1338,7 @@ __global__ __launch_bounds__(64) void k_epoch_wide(GArgs<T> A) {
 // the FP64 layout of the tile kernel (4 doubles per lane, x as a double in the
 // record, the row's 1/nnz from the FP64 array with the W row, the column's
 // with the H row).  The W and H forwarding rings hold 64-double rows.
-static size_t epoch_lds_wide64() { return (size_t)2 * GROUP_RING * GROUP_SLOTS * RB64; }
+static size_t epoch_lds_wide64(int v) { return (size_t)2 * GROUP_RING * GROUP_SLOTS * RB64 * v; }
 
 // a double from lane `src` (per lane)
 __device__ __forceinline__ double lane_gather_d(double v, int src) {
@@ -1354,15 +1354,16 @@ __device__ __forceinline__ double lane_gather_d(double v, int src) {
 // Grouping only reorders updates that share no row or column, so the result
 // is the sequential one.  Padding elements are +0 products, and the running
 // sum, started at +0, never is -0: adding them changes nothing.
-template <int KIND, bool TRUNC, bool STRICT>
+template <int KIND, bool TRUNC, bool STRICT, int V>
 __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
     extern __shared__ float lds_f[];
     char* const wring = (char*)lds_f;
-    char* const hring = wring + GROUP_RING * GROUP_SLOTS * RB64;
+    constexpr int RB = RB64 * V;   // ring row: 64 V doubles
+    char* const hring = wring + GROUP_RING * GROUP_SLOTS * RB;
     const int t = blockIdx.x;
     const int lane = threadIdx.x, slot = lane >> 4, l16 = lane & 15;
-    const int e0 = 4 * l16;
-    const bool pad = e0 >= A.ld;   // whole 4-element chunks: ld is a multiple of 4
+    const int e0 = 4 * V * l16;
+    const bool pad = e0 >= A.ld;   // whole 4V-element chunks: ld is a multiple of 4V
     const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * 8u;
     const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * 8u : (uint32_t)e0 * 8u;
     const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * 8u : (uint32_t)e0 * 8u;
@@ -1370,18 +1371,20 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
     const __amdgpu_buffer_rsrc_t Wr = make_rsrc(A.W, A.w_bytes);
     const __amdgpu_buffer_rsrc_t Ir = make_rsrc(A.inv1, A.i_bytes);   // rows past m read 0
     const uint32_t rec_off = 16u * slot;
-    char* const wring_lane = wring + slot * RB64 + e0 * 8;
-    char* const hring_lane = hring + slot * RB64 + e0 * 8;
+    char* const wring_lane = wring + slot * RB + e0 * 8;
+    char* const hring_lane = hring + slot * RB + e0 * 8;
     const double eps = A.eps;
     const double c1 = eps * -2.;
     const double f2 = (eps * 2.) * A.lambda;
     const int d = A.d;
-    const int nl = A.ld / 4;   // lanes of a slot that hold the row
-    for (int e = lane; e < 2 * GROUP_RING * GROUP_SLOTS * RB64 / 16; e += 64)
+    const int nl = A.ld / (4 * V);   // lanes of a slot that hold the row
+    for (int e = lane; e < 2 * GROUP_RING * GROUP_SLOTS * RB / 16; e += 64)
         ((uint4*)wring)[e] = make_uint4(0u, 0u, 0u, 0u);
     unsigned long long c_wait = 0, c_loop = 0, n_groups = 0;
     const unsigned long long c_begin = A.prof ? clock64() : 0;
-    D4 hprev{0., 0., 0., 0.};
+    D4 hprev[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) hprev[v] = D4{0., 0., 0., 0.};
     for (int s = 0; s < 2 * d; ++s) {
         const int visit = s * d + t;
         const int b2 = uniform_i32((int)A.sched[visit]);
@@ -1397,10 +1400,11 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
         const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * 8u);
         const __amdgpu_buffer_rsrc_t Jr = make_rsrc(A.inv2 + col0, (uint32_t)ncol * 8u);
         uint4 qa[GQ64];
-        D4 qw[GPREF64], qh[GPREF64];
+        D4 qw[GPREF64][V], qh[GPREF64][V];
         double qwi[GPREF64], qhi[GPREF64];
-        auto load_w = [&](const uint4& rec, D4& out, double& inv) {
-            out = bload_d4(Wr, __umul24(rec.x, ldb) + lbase);
+        auto load_w = [&](const uint4& rec, D4* out, double& inv) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) out[v] = bload_d4(Wr, __umul24(rec.x, ldb) + lbase + 32 * v);
             if (KIND == UPD_NZSL_L2) {
                 const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(Ir, (int)__umul24(rec.x, 8u), 0, 0);
                 inv = dbl(u.x, u.y);
@@ -1408,8 +1412,9 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
                 inv = 0.;
             }
         };
-        auto load_h = [&](const uint4& rec, D4& out, double& inv) {
-            out = bload_d4(Hr, __umul24(rec.y & 0xffffu, ldb) + hbase);
+        auto load_h = [&](const uint4& rec, D4* out, double& inv) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) out[v] = bload_d4(Hr, __umul24(rec.y & 0xffffu, ldb) + hbase + 32 * v);
             if (KIND == UPD_NZSL_L2) {
                 const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(Jr, (int)__umul24(rec.y & 0xffffu, 8u), 0, 0);
                 inv = dbl(u.x, u.y);
@@ -1426,34 +1431,61 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
         const unsigned long long c2 = A.prof ? clock64() : 0;
 #pragma unroll
         for (int u = 0; u < GPREF64; ++u) load_h(qa[u], qh[u], qhi[u]);
-        D4 wq = ld_d4(wring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
-        D4 hq = ld_d4(hring + ((qa[0].y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
+        D4 wq[V], hq[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            wq[v] = ld_d4(wring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * RB + e0 * 8 + 32 * v);
+            hq[v] = ld_d4(hring + ((qa[0].y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB + e0 * 8 + 32 * v);
+        }
         for (int K0 = 0; K0 < ngp; K0 += GQ64) {
             const int rbase = (int)rec_off + 64 * (K0 + GQ64);
 #pragma unroll
             for (int u = 0; u < GQ64; ++u) {
                 const int K = K0 + u;
                 const uint4 rec = qa[u];
-                D4 w = qw[u % GPREF64], h = qh[u % GPREF64];
+                D4 w[V], h[V], fw[V], fh[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    w[v] = qw[u % GPREF64][v];
+                    h[v] = qh[u % GPREF64][v];
+                    fw[v] = wq[v];
+                    fh[v] = hq[v];
+                }
                 const double ai = qwi[u % GPREF64], ci = qhi[u % GPREF64];
-                const D4 fw = wq, fh = hq;
                 // ring operands of group K+1 (written by groups <= K-1)
                 const uint4 nrec = qa[(u + 1) % GQ64];
-                wq = ld_d4(wring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
-                hq = ld_d4(hring + ((nrec.y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB64 + e0 * 8);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    wq[v] = ld_d4(wring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * RB + e0 * 8 + 32 * v);
+                    hq[v] = ld_d4(hring + ((nrec.y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB + e0 * 8 + 32 * v);
+                }
                 const bool wf = (int32_t)rec.x < 0;         // REC_WFWD
                 const bool hf = (rec.y & REC_HFWD) != 0;
                 const bool hr = (int32_t)rec.y < 0;         // REC_HREG
-                w.x = sel(wf, fw.x, w.x); w.y = sel(wf, fw.y, w.y); w.z = sel(wf, fw.z, w.z); w.w = sel(wf, fw.w, w.w);
-                h.x = sel(hr, hprev.x, sel(hf, fh.x, h.x)); h.y = sel(hr, hprev.y, sel(hf, fh.y, h.y));
-                h.z = sel(hr, hprev.z, sel(hf, fh.z, h.z)); h.w = sel(hr, hprev.w, sel(hf, fh.w, h.w));
-                D4 wn, hn;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    w[v].x = sel(wf, fw[v].x, w[v].x); w[v].y = sel(wf, fw[v].y, w[v].y);
+                    w[v].z = sel(wf, fw[v].z, w[v].z); w[v].w = sel(wf, fw[v].w, w[v].w);
+                    h[v].x = sel(hr, hprev[v].x, sel(hf, fh[v].x, h[v].x));
+                    h[v].y = sel(hr, hprev[v].y, sel(hf, fh[v].y, h[v].y));
+                    h[v].z = sel(hr, hprev[v].z, sel(hf, fh[v].z, h[v].z));
+                    h[v].w = sel(hr, hprev[v].w, sel(hf, fh[v].w, h[v].w));
+                }
+                D4 wn[V], hn[V];
                 if constexpr (STRICT) {
-                    const double px = w.x * h.x, py = w.y * h.y, pz = w.z * h.z, pw = w.w * h.w;
+                    double pr[4 * V];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        pr[4 * v] = w[v].x * h[v].x;
+                        pr[4 * v + 1] = w[v].y * h[v].y;
+                        pr[4 * v + 2] = w[v].z * h[v].z;
+                        pr[4 * v + 3] = w[v].w * h[v].w;
+                    }
                     double acc = 0.;
                     for (int l = 0; l < nl; ++l) {
-                        const double prev = dpp_d<0x111>(acc);   // row_shr:1 (lane 0 of the row: +0)
-                        const double s4 = (((prev + px) + py) + pz) + pw;
+                        double s4 = dpp_d<0x111>(acc);   // row_shr:1 (lane 0 of the row: +0)
+#pragma unroll
+                        for (int e = 0; e < 4 * V; ++e) s4 = s4 + pr[e];
                         acc = l16 == l ? s4 : acc;
                     }
                     const double wh = lane_gather_d(acc, 16 * slot + nl - 1);
@@ -1471,36 +1503,56 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
                             ho = hv - (f1 * t + f2 * hv);
                         }
                     };
-                    upd(w.x, h.x, wn.x, hn.x);
-                    upd(w.y, h.y, wn.y, hn.y);
-                    upd(w.z, h.z, wn.z, hn.z);
-                    upd(w.w, h.w, wn.w, hn.w);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        upd(w[v].x, h[v].x, wn[v].x, hn[v].x);
+                        upd(w[v].y, h[v].y, wn[v].y, hn[v].y);
+                        upd(w[v].z, h[v].z, wn[v].z, hn[v].z);
+                        upd(w[v].w, h[v].w, wn[v].w, hn[v].w);
+                    }
                 } else {
-                    double part = w.x * h.x;
-                    part = fma(w.y, h.y, part);
-                    part = fma(w.z, h.z, part);
-                    part = fma(w.w, h.w, part);
+                    double part = w[0].x * h[0].x;
+                    part = fma(w[0].y, h[0].y, part);
+                    part = fma(w[0].z, h[0].z, part);
+                    part = fma(w[0].w, h[0].w, part);
+#pragma unroll
+                    for (int v = 1; v < V; ++v) {
+                        part = fma(w[v].x, h[v].x, part);
+                        part = fma(w[v].y, h[v].y, part);
+                        part = fma(w[v].z, h[v].z, part);
+                        part = fma(w[v].w, h[v].w, part);
+                    }
                     const double wh = row16_sum_d(part);
                     const double f1 = c1 * (dbl(rec.z, rec.w) - wh);
                     double a = 0., bb = 0.;
                     if (KIND == UPD_NZSL_L2) { a = f2 * ai; bb = f2 * ci; }
                     if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
-                    wn.x = fma(-f1, h.x, fma(-a, w.x, w.x)); wn.y = fma(-f1, h.y, fma(-a, w.y, w.y));
-                    wn.z = fma(-f1, h.z, fma(-a, w.z, w.z)); wn.w = fma(-f1, h.w, fma(-a, w.w, w.w));
-                    hn.x = fma(-f1, w.x, fma(-bb, h.x, h.x)); hn.y = fma(-f1, w.y, fma(-bb, h.y, h.y));
-                    hn.z = fma(-f1, w.z, fma(-bb, h.z, h.z)); hn.w = fma(-f1, w.w, fma(-bb, h.w, h.w));
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        wn[v].x = fma(-f1, h[v].x, fma(-a, w[v].x, w[v].x));
+                        wn[v].y = fma(-f1, h[v].y, fma(-a, w[v].y, w[v].y));
+                        wn[v].z = fma(-f1, h[v].z, fma(-a, w[v].z, w[v].z));
+                        wn[v].w = fma(-f1, h[v].w, fma(-a, w[v].w, w[v].w));
+                        hn[v].x = fma(-f1, w[v].x, fma(-bb, h[v].x, h[v].x));
+                        hn[v].y = fma(-f1, w[v].y, fma(-bb, h[v].y, h[v].y));
+                        hn[v].z = fma(-f1, w[v].z, fma(-bb, h[v].z, h[v].z));
+                        hn[v].w = fma(-f1, w[v].w, fma(-bb, h[v].w, h[v].w));
+                    }
                 }
-                if (TRUNC) {   // update-truncate.h:50-64
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    if (TRUNC) {   // update-truncate.h:50-64
 #define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
-                    TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(wn.z); TRUNC_(wn.w);
-                    TRUNC_(hn.x); TRUNC_(hn.y); TRUNC_(hn.z); TRUNC_(hn.w);
+                        TRUNC_(wn[v].x); TRUNC_(wn[v].y); TRUNC_(wn[v].z); TRUNC_(wn[v].w);
+                        TRUNC_(hn[v].x); TRUNC_(hn[v].y); TRUNC_(hn[v].z); TRUNC_(hn[v].w);
 #undef TRUNC_
+                    }
+                    hprev[v] = hn[v];
+                    bstore_d4(Wr, __umul24(rec.x, ldb) + sbase + 32 * v, wn[v]);
+                    bstore_d4(Hr, __umul24(rec.y & 0xffffu, ldb) + hbase + 32 * v, hn[v]);
+                    st_d4(wring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB) + 32 * v, wn[v]);
+                    st_d4(hring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB) + 32 * v, hn[v]);
                 }
-                hprev = hn;
-                bstore_d4(Wr, __umul24(rec.x, ldb) + sbase, wn);
-                bstore_d4(Hr, __umul24(rec.y & 0xffffu, ldb) + hbase, hn);
-                st_d4(wring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB64), wn);
-                st_d4(hring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * RB64), hn);
                 load_w(qa[(u + GPREF64) % GQ64], qw[u % GPREF64], qwi[u % GPREF64]);
                 load_h(qa[(u + GPREF64) % GQ64], qh[u % GPREF64], qhi[u % GPREF64]);
                 qa[u] = load_rec(Gr, rbase + 64 * u);
@@ -1563,7 +1615,9 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     if (wide) *wide = false;
     const bool strict = P.precision == PREC_FP64_STRICT;
     const bool f64 = P.precision == PREC_FP64 || strict;
-    if (f64 && (P.ld > 64 || P.ld % 4 != 0)) return no("FP64 grouped kernel needs rank <= 64, rows padded to 4");
+    if (f64 && (P.ld > 128 || P.ld % (P.ld > 64 ? 8 : 4) != 0))
+        return no("FP64 grouped kernel needs rank <= 128, rows padded to 4 (to 8 above 64)");
+    const bool f64_wide_only = f64 && P.ld > 64;   // the FP64 tile kernel holds 4 doubles per lane
     if (P.ld > 128) return no("grouped kernel needs rank <= 128");
     if (P.ld > 64 && P.ld % 8 != 0) return no("rows above rank 64 must be padded to 8 elements");
     if (max_rows >= 65536 || max_cols >= 65536) return no("block dimensions above 65535");
@@ -1586,13 +1640,16 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     const void* fn = f64 ? (const void*)k_epoch_grouped64<UPD_NZSL_L2, false>
                    : vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
                                        : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
-    // FP64_STRICT (the reference association) runs on the streaming body only
-    if (!strict && !force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
+    // FP64_STRICT (the reference association) and FP64 above rank 64 run on the streaming body only
+    if (!strict && !f64_wide_only && !force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
     // FP64 blocks too wide for the LDS tile: H rows streamed from HBM
     if (f64 && wide) {
-        const size_t lw = epoch_lds_wide64();
-        const void* fw = strict ? (const void*)k_epoch_wide64<UPD_NZSL_L2, false, true>
-                                : (const void*)k_epoch_wide64<UPD_NZSL_L2, false, false>;
+        const int v = P.ld > 64 ? 2 : 1;
+        const size_t lw = epoch_lds_wide64(v);
+        const void* fw = strict ? (v == 1 ? (const void*)k_epoch_wide64<UPD_NZSL_L2, false, true, 1>
+                                          : (const void*)k_epoch_wide64<UPD_NZSL_L2, false, true, 2>)
+                                : (v == 1 ? (const void*)k_epoch_wide64<UPD_NZSL_L2, false, false, 1>
+                                          : (const void*)k_epoch_wide64<UPD_NZSL_L2, false, false, 2>);
         if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
             *wide = true;
             return true;
@@ -1826,9 +1883,13 @@ static void epoch_grouped64_t(const DevProblem& P, const UpdParams& U, const Gro
                               const EpochLaunch& L, hipStream_t s) {
     GArgs64 A = make_gargs64(P, U, B, L);
     if (B.wide) {   // H rows streamed from HBM
-        const size_t bytes = epoch_lds_wide64();
-        const void* fn = P.precision == PREC_FP64_STRICT ? (const void*)k_epoch_wide64<KIND, TRUNC, true>
-                                                          : (const void*)k_epoch_wide64<KIND, TRUNC, false>;
+        const int v = P.ld > 64 ? 2 : 1;
+        const bool strict = P.precision == PREC_FP64_STRICT;
+        const size_t bytes = epoch_lds_wide64(v);
+        const void* fn = strict ? (v == 1 ? (const void*)k_epoch_wide64<KIND, TRUNC, true, 1>
+                                          : (const void*)k_epoch_wide64<KIND, TRUNC, true, 2>)
+                                : (v == 1 ? (const void*)k_epoch_wide64<KIND, TRUNC, false, 1>
+                                          : (const void*)k_epoch_wide64<KIND, TRUNC, false, 2>);
         HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
         k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
         HIPK(hipGetLastError());

@@ -77,7 +77,7 @@ typedef struct dsgdpp_cfg {
     int32_t kernel;          /* 0 = auto (grouped dataflow when eligible, else dataflow);
                                 1 = one launch per stratum, one wave per block, sequential;
                                 2 = dataflow (one persistent launch per epoch), sequential blocks;
-                                3 = grouped dataflow (FP32/BF16 up to rank 128, FP64 up to rank 64);
+                                3 = grouped dataflow (up to rank 128; the FP64 LDS tile up to 64);
                                 4 = grouped dataflow with H rows streamed from HBM (all precisions;
                                     chosen by 0 and 3 when a column block is too wide for the LDS
                                     tile).  Blocks whose prep state exceeds LDS keep it in global
@@ -164,9 +164,10 @@ int dsgdpp_get_task_seeds(dsgdpp_ctx* ctx, uint64_t* out);
 int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint32_t* out,
                            int64_t cap, int64_t* n);
 /* Whether epochs of this context can use the grouped kernel (FP32/BF16 up to
- * rank 128, FP64 and FP64_STRICT up to rank 64 with rows padded to 4; the H
- * tile in LDS or H streamed from HBM); `why` (optional) gets the reason when
- * not.  No counterpart in the reference (a kernel-choice query). */
+ * rank 128, FP64 and FP64_STRICT up to rank 128 with rows padded to 4 (to 8
+ * above 64); the H tile in LDS or H streamed from HBM); `why` (optional) gets
+ * the reason when not.  No counterpart in the reference (a kernel-choice
+ * query). */
 int dsgdpp_grouped_ok(dsgdpp_ctx* ctx, int32_t* ok, char* why, int64_t cap);
 /* nnz per block (d x 2d, row-major) and dims {m, n, r, d, nnz_train, nnz_test} */
 int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);

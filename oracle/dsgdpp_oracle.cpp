@@ -509,15 +509,16 @@ static inline void updatePointF(const UpdateArgs& a, int mode, double* w, double
 static inline void updatePointD(const UpdateArgs& a, double* w, double* h, double x, double eps, double n1,
                                 double n2) {
     const int64_t r = a.r;
-    double wd[64], hd[64];
-    for (int64_t z = 0; z < 64; ++z) {
+    const int L = r > 64 ? 8 : 4;   // elements per lane: 4 up to rank 64, 8 up to 128
+    double wd[128], hd[128];
+    for (int64_t z = 0; z < 128; ++z) {
         wd[z] = z < r ? w[z] : 0.;
         hd[z] = z < r ? h[z] : 0.;
     }
     double dl[16];
     for (int l = 0; l < 16; ++l) {
-        double p = wd[4 * l] * hd[4 * l];
-        for (int e = 1; e < 4; ++e) p = std::fma(wd[4 * l + e], hd[4 * l + e], p);
+        double p = wd[L * l] * hd[L * l];
+        for (int e = 1; e < L; ++e) p = std::fma(wd[L * l + e], hd[L * l + e], p);
         dl[l] = p;
     }
     double q[4];
@@ -1030,7 +1031,7 @@ int orc_set_record(orc_ctx* c, int on) { c->record = on != 0; return 0; }
 int orc_set_arith(orc_ctx* c, int mode) {
     if (mode < 0 || mode > 3) { setErr(c, "invalid arithmetic mode"); return -1; }
     if ((mode == 1 || mode == 2) && c->r > 128) { setErr(c, "float emulation needs rank <= 128"); return -1; }
-    if (mode == 3 && c->r > 64) { setErr(c, "FP64 grouped emulation needs rank <= 64"); return -1; }
+    if (mode == 3 && c->r > 128) { setErr(c, "FP64 grouped emulation needs rank <= 128"); return -1; }
     c->arith = mode;
     // values as the GPU stores them: x as float, factors rounded to the storage type
     if (mode == 1 || mode == 2) {

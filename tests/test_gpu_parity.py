@@ -291,7 +291,8 @@ def test_grouped_fp64_bitexact_vs_emulation(rank, update, truncate):
 
 
 @pytest.mark.parametrize("rank,update,truncate", [(50, "Nzsl_L2(0.05)", None), (16, "Nzsl", None),
-                                                  (64, "Nzsl_Nzl2(0.02)", None), (37, "Nzsl_L2(0.05)", "(-0.4,0.4)")])
+                                                  (64, "Nzsl_Nzl2(0.02)", None), (37, "Nzsl_L2(0.05)", "(-0.4,0.4)"),
+                                                  (100, "Nzsl_L2(0.05)", None), (72, "Nzsl_Nzl2(0.02)", "(-0.4,0.4)")])
 def test_grouped_fp64_wide_bitexact_vs_emulation(rank, update, truncate):
     """The FP64 streaming body (H rows from HBM, kernel 4) against the same
     double association as the tile kernel; a shape whose column blocks are
@@ -389,7 +390,8 @@ def test_wide_column_blocks_choose_the_streaming_kernel():
 @pytest.mark.parametrize("rank,update,truncate,sgd", [(8, "Nzsl_L2(0.05)", None, "WOR"), (10, "Nzsl", None, "WR"),
                                                       (13, "Nzsl_Nzl2(0.02)", None, "SEQ"),
                                                       (37, "Nzsl_L2(0.05)", "(-0.4,0.4)", "WOR"),
-                                                      (64, "Nzsl_L2(0.05)", None, "WOR")])
+                                                      (64, "Nzsl_L2(0.05)", None, "WOR"),
+                                                      (100, "Nzsl_L2(0.05)", None, "WOR")])
 def test_fp64_strict_grouped_equals_reference_association(rank, update, truncate, sgd):
     """FP64_STRICT on the grouped streaming kernel (the reference's per-update
     association; groups only reorder updates that share no row or column):

@@ -348,8 +348,9 @@ def test_quickstart_statistics():
     assert nzsl == pytest.approx(nnz * (10 + 0.01 + 10 / 144), rel=0.03)
 
 
+@pytest.mark.parametrize("rank", [37, 100])
 @pytest.mark.parametrize("update", ["Nzsl", "Nzsl_L2(0.3)", "Nzsl_Nzl2(0.2)", "Nzsl_L2(0.3)|(-0.4,0.4)"])
-def test_fp64_grouped_emulation_tracks_reference(update):
+def test_fp64_grouped_emulation_tracks_reference(update, rank):
     """The FP64 grouped emulation (the association of the GPU's FP64 grouped
     kernel) differs from the reference arithmetic only in rounding: the dot
     product's order and fused multiply-adds.  Its per-epoch losses stay within
@@ -357,7 +358,7 @@ def test_fp64_grouped_emulation_tracks_reference(update):
     wrong sign or a wrong factor moves them by orders of magnitude more."""
     upd, _, trunc = update.partition("|")
     kw = dict(truncate=trunc) if trunc else {}
-    p = random_problem(60, 80, 1500, 37, seed=21)
+    p = random_problem(60, 80, 1500, rank, seed=21)
     a = _oracle(p, upd, tasks=3, seed=5, **kw)
     b = _oracle(p, upd, tasks=3, seed=5, **kw)
     b.set_arith("FP64_GROUPED")
@@ -372,8 +373,8 @@ def test_fp64_grouped_emulation_tracks_reference(update):
     np.testing.assert_allclose(Wb, Wa, rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(Hb, Ha, rtol=1e-10, atol=1e-13)
     with pytest.raises(Exception):
-        c = _oracle(random_problem(10, 10, 30, 65, seed=1), upd)
-        c.set_arith("FP64_GROUPED")   # rank above 64
+        c = _oracle(random_problem(10, 10, 30, 129, seed=1), upd)
+        c.set_arith("FP64_GROUPED")   # rank above 128
 
 
 def test_threads_do_not_change_results():
