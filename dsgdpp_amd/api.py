@@ -192,7 +192,8 @@ class DsgdPp:
             raise DsgdPpError(rc, self._lib.dsgdpp_last_error(self._h).decode())
 
     def grouped_ok(self):
-        """(True, "") when epochs use the grouped kernel, else (False, reason)."""
+        """(True, note) when epochs use the grouped kernel, else (False, reason);
+        the note says how many blocks run two-slot groups (empty if none)."""
         ok = ctypes.c_int32()
         buf = ctypes.create_string_buffer(256)
         self._check(self._lib.dsgdpp_grouped_ok(self._h, ctypes.byref(ok), buf, 256))

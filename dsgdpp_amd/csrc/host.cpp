@@ -223,6 +223,11 @@ struct dsgdpp_ctx {
     bool kprof = false;
     uint64_t dataVersion = 0;
     int64_t maxBlock = 0;   // largest training block
+    // two-slot groups (FP32 tile kernel, rank <= 64): per block, set at load
+    int pairsMode = -1;                 // DSGDPP_PAIRS: 0 never, 1 every eligible block, else by the chain rule
+    std::vector<uint8_t> pairBlk;       // d x 2d, empty when no block qualifies
+    uint8_t* dPairs = nullptr;
+    int64_t pairBlocks = 0;
 
     size_t factorBytes() const {
         return precision <= DSGDPP_FP64 ? 8 : precision == DSGDPP_FP32 ? 4 : 2;
@@ -414,6 +419,46 @@ void buildBlocks(dsgdpp_ctx* c, int64_t nnz, const int32_t* I, const int32_t* J,
     }
 }
 
+// Blocks whose visits run two-slot groups (FP32 on the tile kernel, rank <= 64).
+// A visit's groups number at least its busiest column's point count c, and at
+// least n/4 (four slots) or n/2 (two slots) for n points; a two-slot group
+// costs about 45 instructions against 54, so two slots win when
+// 45 max(c, n/2) < 54 max(c, n/4), i.e. c > 5n/12.  The prep and the epoch
+// kernel read the same flags, and a four-slot loop would also run a two-slot
+// visit correctly (its slots 2 and 3 hold dummy records).
+template <typename Pt>
+void choosePairs(dsgdpp_ctx* c, const std::vector<Pt>& pts, const std::vector<int64_t>& start) {
+    c->pairBlk.clear();
+    c->pairBlocks = 0;
+    c->G.pairs = nullptr;
+    if (!std::is_same<Pt, Pt32>::value || c->precision != DSGDPP_FP32 || !c->groupedOk || c->G.wide ||
+        c->P.ld > 64 || c->pairsMode == 0)
+        return;
+    const uint32_t d = c->d, d2 = 2 * d;
+    std::vector<uint8_t> flag((size_t)d * d2, 0);
+    parallelFor((int64_t)d, [&](int64_t lo, int64_t hi) {
+        std::vector<int32_t> cnt;
+        for (int64_t b1 = lo; b1 < hi; ++b1)
+            for (uint32_t b2 = 0; b2 < d2; ++b2) {
+                const size_t b = (size_t)b1 * d2 + b2;
+                const int64_t n = start[b + 1] - start[b];
+                if (n == 0) continue;
+                if (c->pairsMode == 1) { flag[b] = 1; continue; }
+                const int64_t col0 = c->off2[b2];
+                cnt.assign((size_t)(c->off2[b2 + 1] - col0), 0);
+                int32_t top = 0;
+                for (int64_t p = start[b]; p < start[b + 1]; ++p) top = std::max(top, ++cnt[(size_t)(pts[p].j - col0)]);
+                flag[b] = 12 * (int64_t)top > 5 * n ? 1 : 0;
+            }
+    });
+    for (uint8_t f : flag) c->pairBlocks += f;
+    if (c->pairBlocks == 0) return;
+    c->pairBlk.swap(flag);
+    dalloc(c->dPairs, c->pairBlk.size());
+    HIPC(hipMemcpy(c->dPairs, c->pairBlk.data(), c->pairBlk.size(), hipMemcpyHostToDevice));
+    c->G.pairs = c->dPairs;
+}
+
 template <typename Pt>
 void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const int32_t* J,
                   const double* X) {
@@ -497,6 +542,7 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
                                          c->kernel == 4);
         c->G.wide = wide;
         if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
+        choosePairs<Pt>(c, pts, hb.start);
         if (c->groupedOk) {
             std::vector<int32_t> o2(c->off2.begin(), c->off2.end());
             int32_t* d2p = const_cast<int32_t*>(c->G.off2);
@@ -684,8 +730,9 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
                     const int64_t nb = bs[b + 1] - bs[b];
                     const int64_t base = P.voff[(size_t)sub * d + id];
                     const uint32_t b2 = P.schedule[(size_t)sub * d + id];
+                    const int32_t pairs = P.grouped && !c->pairBlk.empty() && c->pairBlk[b] ? 1 : 0;
                     PermJob j{base, nb, -1, draw, k0, k1, bs[b], (int32_t)c->off1[id], (int32_t)c->off2[b2],
-                              (int32_t)(sub * d + id), 0, P.goff[(size_t)sub * d + id]};
+                              (int32_t)(sub * d + id), pairs, P.goff[(size_t)sub * d + id]};
                     if (c->sgdOrder == DSGDPP_SGD_WR) {
                         if (nb > 0) perTask[id].push_back(j);
                         draw += (uint64_t)nb + 2;
@@ -1301,6 +1348,8 @@ int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out) {
         c->lookahead = !(la && la[0] == '0');
         const char* kp = getenv("DSGDPP_KPROF");
         c->kprof = kp && kp[0] == '1';
+        const char* pm = getenv("DSGDPP_PAIRS");
+        c->pairsMode = pm && (pm[0] == '0' || pm[0] == '1') ? pm[0] - '0' : -1;
         dalloc(c->dPartials, (size_t)partials_capacity() * sizeof(double));
         HIPC(hipHostMalloc((void**)&c->hPinned, sizeof(double) * 8, hipHostMallocDefault));
     } catch (const std::exception& e) {
@@ -1346,6 +1395,7 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     v = c->G.prof; dfree(v);
     v = c->G.pprof; dfree(v);
     v = c->dReady; dfree(v);
+    { void* q = c->dPairs; dfree(q); c->dPairs = nullptr; }
     if (c->hPinned) (void)hipHostFree(c->hPinned);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1431,7 +1481,10 @@ int dsgdpp_grouped_ok(dsgdpp_ctx* c, int32_t* ok, char* why, int64_t cap) {
         if (!c->blocks[0].loaded) throw Error(DSGDPP_E_STATE, "load the training matrix first");
         *ok = c->groupedOk ? 1 : 0;
         if (why && cap > 0) {
-            const std::string w = c->groupedOk ? std::string() : c->groupedWhy;
+            const std::string w = !c->groupedOk ? c->groupedWhy
+                                  : c->pairBlocks ? "two-slot groups in " + std::to_string(c->pairBlocks) + " of " +
+                                                        std::to_string((size_t)c->d * 2 * c->d) + " blocks"
+                                                  : std::string();
             const size_t k = std::min<size_t>(w.size(), (size_t)cap - 1);
             memcpy(why, w.data(), k);
             why[k] = 0;

@@ -62,7 +62,7 @@ struct PermJob {
     int64_t s0;          // first point of the block in the blocked point array
     int32_t row0, col0;  // first global row / column of the block
     int32_t visit;       // s * d + t
-    int32_t pad;
+    int32_t pairs;       // 1: groups of at most two points (slots 2 and 3 get the dummy record)
     int64_t gbase;       // first group of this visit in the grouped record buffer
 };
 
@@ -109,6 +109,7 @@ struct GroupedBuffers {
     unsigned long long* pprof = nullptr;  // optional prep clock sums {visit order, scheduling, fill, visits}
     bool wide = false;                    // H rows streamed from HBM (blocks too wide for the LDS tile)
     uint32_t* spill = nullptr;            // prep state of blocks too large for LDS (prep_spill_bytes)
+    const uint8_t* pairs = nullptr;       // per block (t * 2d + b2): its visits run two-slot groups
     size_t spill_bytes = 0;
 };
 // The prep kernel's inputs for one problem (many problems share one launch).

@@ -195,7 +195,10 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         //    groups of the window, so the search is one rotate and find-first-zero.
         //    A group that left the window is never searched: a point starts at
         //    maxG - 61 or later.
+        //    A two-slot visit (J.pairs) starts every group with slots 2 and 3
+        //    taken, so its points go to slots 0 and 1 only.
         int maxG = 1;
+        const uint32_t empty = J.pairs ? 0xCu : 0u;   // slot mask of a group with no point yet
         uint32_t win = 0;    // lane b: G << 4 | slot mask of the window's group G in lane b
         uint64_t full = 0;   // bit b: that group has no free slot
         // the next chunk's points are fetched while the current chunk is scanned
@@ -235,7 +238,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 const uint32_t sc = LC & 3u;
                 const int g0 = max(max((int)(LR >> 2) + 2, gH), maxG - 61);
                 const uint32_t w0 = readlane_u32(win, g0 & 63);
-                const uint32_t m0 = (int)(w0 >> 4) == g0 ? w0 : 0u;
+                const uint32_t m0 = (int)(w0 >> 4) == g0 ? w0 : empty;
                 const int at_h = 1 - min(g0 - gH, 1);   // g0 == gH (g0 >= gH)
                 const int hit = at_h & (int)(~(m0 >> sc) & 1u);
                 // otherwise the first group past gH, or from g0, that is not full;
@@ -244,7 +247,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 const int gs = k + (int)__builtin_ctzll(~rotr64(full, k & 63));
                 const int g = gs + hit * (g0 - gs);
                 const uint32_t w = readlane_u32(win, g & 63);
-                const uint32_t mask = (int)(w >> 4) == g ? w & 15u : 0u;   // a lane left by an older group reads empty
+                const uint32_t mask = (int)(w >> 4) == g ? w & 15u : empty;   // a lane left by an older group reads empty
                 const uint32_t ss = (uint32_t)__builtin_ctz(~mask & 15u);
                 const uint32_t sl = ss + (uint32_t)hit * (sc - ss);
                 const uint32_t nm = mask | (1u << sl);
@@ -471,6 +474,7 @@ struct GArgs {
     T* W;
     T* H;
     const float* __restrict__ inv2;
+    const uint8_t* __restrict__ pairs;   // optional, per block: two-slot groups (FP32, rank <= 64)
     unsigned long long* prof;   // optional: per task {wait, stage, loop, total} clocks + group count
     uint32_t w_bytes;          // W buffer size (rows m + FACTOR_TAIL_ROWS)
     uint32_t ztail_b;          // byte offset of the zero rows after the last row
@@ -620,6 +624,126 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
     if (A.prof && lane == 0) A.prof[5 * t + 1] = c_df;
 }
 
+// Sum of a two-slot member's dot product from the chunk sums in its odd
+// lanes (lane 2c + 1 holds chunk c, elements [4c, 4c + 4)): the pairwise order
+// of row16_sum over the 16 chunks, its last level joining the member's two DPP
+// rows.  Every lane of the member gets the bitwise identical total.
+__device__ __forceinline__ float pair_sum(float v) {
+    v = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xF5, 0xF, 0xF, true));   // quad_perm [1,1,3,3]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));  // quad_perm [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true)); // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, true)); // row_mirror
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// One visit in two-slot groups (FP32, rank <= 64; the prep left slots 2 and 3
+// of every group empty).  Slot s = lane / 32 holds member s and lane l32 =
+// lane % 32 holds elements [2 l32, 2 l32 + 2) of W_i and H_j, so a group costs
+// half the update instructions: the form for visits whose group count is set
+// by one column's chain of updates rather than by their point count.  The
+// dot product keeps the four-slot association: the even lane of a 4-element
+// chunk starts the chunk's FMA chain, the odd lane ends it (a DPP move in
+// between), and pair_sum adds the chunks in the same tree.  Data movement is
+// the four-slot loop's.  Returns the clocks spent waiting for the visit's H
+// columns (0 without profiling).
+template <int KIND, bool TRUNC>
+__device__ __forceinline__ unsigned long long pair_groups(const GArgs<float>& A, const __amdgpu_buffer_rsrc_t Wr,
+                                                          const __amdgpu_buffer_rsrc_t Gr, const int ngp,
+                                                          float* const ring, const float* const hbp,
+                                                          int32_t* const flags, const int k) {
+    constexpr int ROW = 64;    // floats per staged row
+    constexpr int GPR = 16;    // W prefetch distance (groups), <= GROUP_RING
+    constexpr int RSH = 8;     // log2(ROW * 4 bytes)
+    static_assert(GPR <= GROUP_RING && GQ % GPR == 0, "W queue");
+    const int lane = threadIdx.x, slot = lane >> 5, l32 = lane & 31;
+    const int e0 = 2 * l32;
+    // ld is a multiple of 4: a lane's pair is in the row or past it
+    const bool pad = e0 >= A.ld;
+    const uint32_t ldb = pad ? 0u : (uint32_t)A.ld * 4u;
+    const uint32_t lbase = pad ? A.ztail_b + (uint32_t)(e0 - A.ld) * 4u : (uint32_t)e0 * 4u;
+    const uint32_t sbase = pad ? A.dummy_b + (uint32_t)(e0 - A.ld) * 4u : (uint32_t)e0 * 4u;
+    const uint32_t rec_off = 16u * slot;
+    float* const ring_lane = ring + slot * ROW + e0;
+    char* const hblk_lane = (char*)(hbp + e0);                                  // + column * ROW floats
+    const char* const hinv_b = (const char*)(hbp + (A.max_cols + 1) * ROW);    // + column << 2
+    const float eps = A.eps;
+    const float c1 = eps * -2.f;
+    const float f2 = (eps * 2.f) * A.lambda;
+    uint4 qa[GQ];
+    u32x2 qb[GPR];
+    auto load_w = [&](const uint4& rec, u32x2& out) {
+        out = __builtin_amdgcn_raw_buffer_load_b64(Wr, (int)(__umul24(rec.x, ldb) + lbase), 0, 0);
+    };
+#pragma unroll
+    for (int u = 0; u < GQ; ++u) qa[u] = load_rec(Gr, (int)rec_off + 64 * u);
+#pragma unroll
+    for (int u = 0; u < GPR; ++u) load_w(qa[u], qb[u]);
+    const unsigned long long c0 = A.prof ? clock64() : 0;
+    lds_await(flags + 0, k);   // the mover has staged this visit's columns
+    const unsigned long long waited = A.prof ? clock64() - c0 : 0;
+    float2 hq = *(const float2*)(hblk_lane + (qa[0].y << RSH));
+    float2 wq = *(const float2*)(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0);
+    float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
+    float2 hprev = make_float2(0.f, 0.f);
+    for (int K0 = 0; K0 < ngp; K0 += GQ) {
+        const int rbase = (int)rec_off + 64 * (K0 + GQ);
+#pragma unroll
+        for (int u = 0; u < GQ; ++u) {
+            const int K = K0 + u;
+            const uint4 rec = qa[u];
+            float2 w = make_float2(__uint_as_float(qb[u % GPR].x), __uint_as_float(qb[u % GPR].y));
+            float2 h = hq;
+            const float2 fwd = wq;
+            const float ih = ihq;
+            // operands of group K+1, read before group K writes (as in the four-slot loop)
+            const uint4 nrec = qa[(u + 1) % GQ];
+            hq = *(const float2*)(hblk_lane + (nrec.y << RSH));
+            wq = *(const float2*)(ring + ((nrec.x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0);
+            if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (nrec.y << 2));
+            {   // W_i from the ring where the prefetch may predate its last update
+                const uint32_t msk = (uint32_t)((int32_t)rec.x >> 31);   // REC_WFWD
+                w.x = __uint_as_float((__float_as_uint(fwd.x) & msk) | (__float_as_uint(w.x) & ~msk));
+                w.y = __uint_as_float((__float_as_uint(fwd.y) & msk) | (__float_as_uint(w.y) & ~msk));
+            }
+            const bool hr = (int32_t)rec.y < 0;   // REC_HREG
+            h.x = hr ? hprev.x : h.x;
+            h.y = hr ? hprev.y : h.y;
+            // chunk c: lane 2c forms w0 h0 + w1 h1, lane 2c + 1 adds w2 h2 and w3 h3
+            float lo = w.x * h.x;
+            lo = fmaf(w.y, h.y, lo);
+            float part = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(lo), 0x111, 0xF, 0xF, true));   // row_shr:1
+            part = fmaf(w.x, h.x, part);
+            part = fmaf(w.y, h.y, part);
+            const float wh = pair_sum(part);
+            const float f1 = c1 * (__uint_as_float(rec.z) - wh);
+            float a = 0.f, bb = 0.f;
+            if (KIND == UPD_NZSL_L2) { a = f2 * __uint_as_float(rec.w); bb = ih; }
+            if (KIND == UPD_NZSL_NZL2) { a = f2; bb = f2; }
+            float2 wn, hn;
+            wn.x = fmaf(-f1, h.x, fmaf(-a, w.x, w.x));
+            wn.y = fmaf(-f1, h.y, fmaf(-a, w.y, w.y));
+            hn.x = fmaf(-f1, w.x, fmaf(-bb, h.x, h.x));
+            hn.y = fmaf(-f1, w.y, fmaf(-bb, h.y, h.y));
+            if (TRUNC) {   // update-truncate.h:50-64
+#define TRUNC_(x) x = (x <= A.tmin) ? A.tmin + eps : ((x >= A.tmax) ? A.tmax - eps : x)
+                TRUNC_(wn.x); TRUNC_(wn.y); TRUNC_(hn.x); TRUNC_(hn.y);
+#undef TRUNC_
+            }
+            hprev = hn;
+            u32x2 wraw;
+            wraw.x = __float_as_uint(wn.x);
+            wraw.y = __float_as_uint(wn.y);
+            __builtin_amdgcn_raw_buffer_store_b64(wraw, Wr, (int)(__umul24(rec.x, ldb) + sbase), 0, 0);
+            *(float2*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW)) = wn;
+            *(float2*)(hblk_lane + (rec.y << RSH)) = hn;
+            load_w(qa[(u + GPR) % GQ], qb[u % GPR]);
+            qa[u] = load_rec(Gr, rbase + 64 * u);
+        }
+    }
+    return waited;
+}
+
 // The compute wave of a task: its visits' groups, in order.
 template <typename T, int KIND, bool TRUNC, int VEC>
 __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, float* lds) {
@@ -663,6 +787,20 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
         const int64_t gbase = uniform_i64(A.goff[visit]);
         const __amdgpu_buffer_rsrc_t Gr = make_rsrc(A.grec + gbase * GROUP_SLOTS, (uint32_t)ngp * 64u);
         const float* const hbp = hbuf0 + (k & 1) * hb;
+        if constexpr (VEC == 1 && std::is_same<T, float>::value) {
+            if (A.pairs && uniform_i32((int)A.pairs[b])) {   // a two-slot visit
+                const unsigned long long cv = A.prof ? clock64() : 0;
+                const unsigned long long waited = pair_groups<KIND, TRUNC>(A, Wr, Gr, ngp, ring, hbp, flags, k);
+                lds_publish(flags + 1, k);   // the buffer goes back to the mover
+                ++k;
+                if (A.prof) {
+                    c_wait += waited;
+                    c_loop += clock64() - cv - waited;
+                    n_groups += (unsigned long long)ng;
+                }
+                continue;
+            }
+        }
         char* const hblk_lane = (char*)(hbp + e0);                     // + column * ROW floats
         const char* const hinv_b = (const char*)(hbp + (A.max_cols + 1) * ROW);   // + column << 2
         uint4 qa[GQ];
@@ -1775,6 +1913,7 @@ static GArgs<T> make_gargs(const DevProblem& P, const UpdParams& U, const Groupe
     A.W = (T*)P.W;
     A.H = (T*)P.H;
     A.inv2 = (const float*)P.inv2;
+    A.pairs = std::is_same<T, float>::value ? B.pairs : nullptr;
     const uint32_t es = Vec4<T>::BYTES;
     A.w_bytes = (uint32_t)((P.m + FACTOR_TAIL_ROWS) * P.ld * es);
     A.ztail_b = (uint32_t)(P.m * P.ld * es);

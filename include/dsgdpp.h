@@ -166,8 +166,9 @@ int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint
 /* Whether epochs of this context can use the grouped kernel (FP32/BF16 up to
  * rank 128, FP64 and FP64_STRICT up to rank 128 with rows padded to 4 (to 8
  * above 64); the H tile in LDS or H streamed from HBM); `why` (optional) gets
- * the reason when not.  No counterpart in the reference (a kernel-choice
- * query). */
+ * the reason when not, and when it can, how many blocks run two-slot groups
+ * ("two-slot groups in K of N blocks", or empty).  No counterpart in the
+ * reference (a kernel-choice query). */
 int dsgdpp_grouped_ok(dsgdpp_ctx* ctx, int32_t* ok, char* why, int64_t cap);
 /* nnz per block (d x 2d, row-major) and dims {m, n, r, d, nnz_train, nnz_test} */
 int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);

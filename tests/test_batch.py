@@ -75,9 +75,12 @@ def _problems():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_bitexact_vs_individual(use_graph):
-    print(f"case: batch of 4 (ranks 16/64/96/128), graph={use_graph}", flush=True)
+@pytest.mark.parametrize("use_graph,pairs", [(True, None), (False, None), (True, "1")])
+def test_batch_bitexact_vs_individual(monkeypatch, use_graph, pairs):
+    """pairs "1": every eligible block (FP32, rank <= 64) in two-slot groups."""
+    print(f"case: batch of 4 (ranks 16/64/96/128), graph={use_graph}, pairs={pairs}", flush=True)
+    if pairs is not None:
+        monkeypatch.setenv("DSGDPP_PAIRS", pairs)
     probs = _problems()
     batch_ctx = [_ctx(p, d, seed=10 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
     solo_ctx = [_ctx(p, d, seed=10 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]

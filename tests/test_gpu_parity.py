@@ -439,6 +439,60 @@ def test_grouped_orders_ranks(rank, sgd, stratum):
         np.testing.assert_array_equal(Hg, Ho)
 
 
+@pytest.mark.parametrize("rank,update,truncate,sgd", [(50, "Nzsl_L2(0.05)", None, "WOR"), (13, "Nzsl", None, "WR"),
+                                                      (64, "Nzsl_Nzl2(0.02)", None, "WOR"),
+                                                      (1, "Nzsl_L2(0.05)", "(-0.4,0.4)", "SEQ"),
+                                                      (32, "Nzsl_L2(0.05)", "(-0.4,0.4)", "WOR")])
+def test_two_slot_groups_bitexact_vs_emulation(monkeypatch, rank, update, truncate, sgd):
+    """Every block in two-slot groups (DSGDPP_PAIRS=1): the two-slot loop keeps
+    the four-slot dot-product association, so the factors equal the oracle's
+    FP32 emulation bit for bit."""
+    print(f"case: two-slot groups rank {rank} {update} {truncate} {sgd}", flush=True)
+    monkeypatch.setenv("DSGDPP_PAIRS", "1")
+    p = skewed_problem(300, 200, 12000, rank, seed=37)
+    g, o = make_pair(p, update=update, truncate=truncate, tasks=4, seed=41, precision="FP32", kernel=3, sgd=sgd)
+    ok, note = g.grouped_ok()
+    assert ok and note.startswith("two-slot groups in ") and int(note.split()[3]) > 24, note
+    o.set_arith("FP32")
+    for e in range(3):
+        assert g.epoch(0.01)["kernel"] == 3
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+
+
+def test_two_slot_groups_chosen_for_chain_bound_blocks():
+    """Blocks dominated by one column (here a column present in every row)
+    take two-slot groups by default; the rest keep four slots; the result is
+    the oracle's FP32 emulation bit for bit either way."""
+    print("case: two-slot groups chosen by the chain rule", flush=True)
+    rng = np.random.default_rng(3)
+    m, n, r = 400, 160, 24
+    dense = [(i, 5) for i in range(m)] + [(i, 97) for i in range(m)]   # two columns in every row
+    other = rng.choice(m * n, size=3000, replace=False)
+    cells = sorted(set(dense) | {(int(c) // n, int(c) % n) for c in other})
+    i = np.array([a for a, _ in cells], dtype=np.int32)
+    j = np.array([b for _, b in cells], dtype=np.int32)
+    x = rng.normal(size=len(cells))
+    p = dict(m=m, n=n, rank=r, train=(i, j, x), W0=rng.uniform(-0.5, 0.5, (m, r)),
+             H0=rng.uniform(-0.5, 0.5, (r, n)))
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=4, seed=9, precision="FP32", kernel=3)
+    ok, note = g.grouped_ok()
+    assert ok and note.startswith("two-slot groups in "), note
+    k = int(note.split()[3])
+    assert 0 < k < 32, note
+    o.set_arith("FP32")
+    for e in range(3):
+        g.epoch(0.01)
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+
+
 def test_grouped_carry_over_and_many_tasks():
     print("case: grouped, equal-size blocks + d=16", flush=True)
     m, n, d = 32, 64, 4
