@@ -4,14 +4,17 @@
 Workload (default): 480,189 x 17,770 matrix, 100M training + 1.4M test
 nonzeros, Zipf(1.0) row/column degrees (capped), integer values 1..5 from a
 rank-20 ground truth, rank 50, FP32 factors, Nzsl_L2(0.05) / None /
-Nzsl_L2(0.05), BoldDriver(0.01), WOR points, COWOR strata, 128 x 256 blocks.
-One step = one DSGD++ epoch over all training points (schedule + visit orders
-+ 256 stratum launches).  Inputs are resident in HBM before timing starts.
+Nzsl_L2(0.05), BoldDriver(0.01), WOR points, COWOR strata, d = 256: 256 row
+blocks x 512 column blocks, one task per CU.  One step = one DSGD++ epoch
+over all training points (one persistent launch; the next epoch's schedule,
+visit orders and groups are prepared beside it).  Inputs are resident in HBM
+before timing starts.
 
-Prints ONE JSON line: metric/value (nnz updates/s), roofline of the stratum
-kernel (algorithmic bytes per update = (8 + s_x) + 4 r s_f, HIP-event time),
-and the CPU baseline (the oracle, an FP64 C++ restatement of the reference's
-mfdsgdpp run with d = 128 on the host cores, one epoch on the same data).
+Prints ONE JSON line: metric/value (nnz updates/s), roofline of the epoch
+kernel (algorithmic bytes per update = (8 + s_x) + 4 r s_f, HIP-event time on
+its stream), and the CPU baseline (the oracle, an FP64 C++ restatement of the
+reference's mfdsgdpp run, with d = --tasks-per-rank = the host cores, whole
+epochs of the same data).
 """
 import argparse
 import json
@@ -83,7 +86,7 @@ def main():
     name = args.workload
     if name == "zipf100m":
         p = data.zipf_100m(device="cuda", shrink=args.shrink)
-        update, decay0, tasks = "Nzsl_L2(0.05)", 0.01, args.tasks or 128
+        update, decay0, tasks = "Nzsl_L2(0.05)", 0.01, args.tasks or 256
         precision = args.precision or "FP32"
     else:
         p = data.quickstart_1m(device="cuda", shrink=args.shrink)
@@ -177,7 +180,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(name, precision, r, stats[-1]["kernel"]),
+            "traffic": measured_traffic(name, precision, r, stats[-1]["kernel"], tasks),
             "kernel": KERNEL_NAMES.get(stats[-1]["kernel"], "?"),
             "bytes_per_update": b_upd,
             "avg_launch_us": avg_launch_s * 1e6,
@@ -210,14 +213,17 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": a.numel() * 4, "launches": 3}))
 
 
-def measured_traffic(name, precision, rank, kernel):
+def measured_traffic(name, precision, rank, kernel, tasks):
     """HBM bytes per epoch launch from the committed rocprofv3 --pmc summary
-    for this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
-    calibration), or None when no summary matches."""
+    for this workload and blocking (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy calibration), or None when no summary matches."""
     path = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES.get(kernel, '?')}_{name}_{precision}_r{rank}.txt")
     if not os.path.exists(path):
         return None
-    for line in open(path):
+    lines = open(path).read().splitlines()
+    if not lines or f"d = {tasks})" not in lines[0]:
+        return None
+    for line in lines:
         if line.startswith("bytes_per_launch:"):
             return float(line.split(":")[1])
     return None
