@@ -588,6 +588,13 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (A.prof) c_df += clock64() - c0;
         float* const hbp = hbuf0 + (k & 1) * hb;
+        // the first 128 columns' 1/nnz are loaded before the H rows, so the two
+        // round trips overlap (a staging is on the handoff's critical path)
+        float inv_a = 0.f, inv_b = 0.f;
+        if (KIND == UPD_NZSL_L2) {
+            if (lane < ncol) inv_a = A.inv2[col0 + lane];
+            if (lane + 64 < ncol) inv_b = A.inv2[col0 + lane + 64];
+        }
         const int total = ncol * 16 * VEC;
         for (int base = 0; base < total; base += 64 * CH) {
             float4 v[CH];
@@ -605,7 +612,9 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
         }
         if (KIND == UPD_NZSL_L2) {   // the column's factor f2 / nnz_j, the same product the update forms
             float* const hinv = hbp + (A.max_cols + 1) * ROW;
-            for (int c = lane; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
+            if (lane < ncol) hinv[lane] = f2 * inv_a;
+            if (lane + 64 < ncol) hinv[lane + 64] = f2 * inv_b;
+            for (int c = lane + 128; c < ncol; c += 64) hinv[c] = f2 * A.inv2[col0 + c];
         }
         lds_publish(flags + 0, k);
         if (pend_b2 >= 0) {   // the compute wave is on visit k - 1 (or done with it)
