@@ -18,6 +18,7 @@ the reference's defaultRunner (mf/sgd/sgd_impl.h:34-145).
 """
 import ctypes
 import math
+import sys
 import time
 
 import numpy as np
@@ -176,9 +177,13 @@ class DsgdPp:
             self._h = None
 
     def __del__(self):
+        # At interpreter exit the HIP runtime and profiler hooks may already be
+        # torn down: release nothing then (the process is ending anyway).
+        if sys.is_finalizing():
+            return
         try:
             self.close()
-        except Exception:
+        except Exception:   # noqa: BLE001
             pass
 
     def __enter__(self):

@@ -20,6 +20,7 @@ import ctypes
 import json
 import math
 import os
+import sys
 import time
 
 import numpy as np
@@ -53,6 +54,10 @@ class DsgdPpBatch:
             self._h = None
 
     def __del__(self):
+        # At interpreter exit the HIP runtime and profiler hooks may already be
+        # torn down: release nothing then (the process is ending anyway).
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:   # noqa: BLE001

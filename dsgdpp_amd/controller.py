@@ -26,6 +26,7 @@ choice, so the states BoldDriver visits are covered too).
 import ctypes
 import math
 import os
+import sys
 
 import numpy as np
 
@@ -94,6 +95,10 @@ class Mlp:
             self._h = None
 
     def __del__(self):
+        # At interpreter exit the HIP runtime and profiler hooks may already be
+        # torn down: release nothing then (the process is ending anyway).
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:   # noqa: BLE001

@@ -320,6 +320,7 @@ struct Ctx {
     bool record = false;
     std::vector<std::vector<uint32_t>> recPerm;   // [task*2d + subepoch]
     int arith = 0;   // 0 FP64 reference; 1 FP32 / 2 BF16 / 3 FP64 grouped-GPU emulation
+    int mutant = 0;  // planted mistake of an emulation (UpdateArgs::mutant), pin tests only
 };
 
 static void setErr(Ctx* c, const std::string& e) { c->err = e; }
@@ -400,6 +401,10 @@ static void computeNnz12(Ctx* c) {
 struct UpdateArgs {
     int kind; double lambda; int64_t r;
     bool truncate; double tmin, tmax;
+    // Planted mistakes for the tests that show the emulation's pins catch
+    // them (0 = none): 1 drops the regularization term, 2 updates h with the
+    // new w, 3 swaps the row and column counts.  Emulations only.
+    int mutant = 0;
 };
 
 // UpdateNzsl (update-nzsl.h:34-49), UpdateNzslL2 (update-nzsl-l2.h:34-57),
@@ -487,12 +492,14 @@ static inline void updatePointF(const UpdateArgs& a, int mode, double* w, double
     const float f1 = (eps * -2.f) * (x - wh);
     const float f2 = (eps * 2.f) * lam;
     float aa = 0.f, bb = 0.f;
+    if (a.mutant == 3) std::swap(n1, n2);
     if (a.kind == U_NZSL_L2) { aa = f2 * (float)(1. / n1); bb = f2 * (float)(1. / n2); }
     else if (a.kind == U_NZSL_NZL2) { aa = f2; bb = f2; }
+    if (a.mutant == 1) aa = bb = 0.f;
     const float tmin = (float)a.tmin, tmax = (float)a.tmax;
     for (int64_t z = 0; z < r; ++z) {
         float wn = fmaf(-f1, hf[z], fmaf(-aa, wf[z], wf[z]));
-        float hn = fmaf(-f1, wf[z], fmaf(-bb, hf[z], hf[z]));
+        float hn = fmaf(-f1, a.mutant == 2 ? wn : wf[z], fmaf(-bb, hf[z], hf[z]));
         if (a.truncate) {
             wn = (wn <= tmin) ? tmin + eps : ((wn >= tmax) ? tmax - eps : wn);
             hn = (hn <= tmin) ? tmin + eps : ((hn >= tmax) ? tmax - eps : hn);
@@ -527,11 +534,13 @@ static inline void updatePointD(const UpdateArgs& a, double* w, double* h, doubl
     const double f1 = (eps * -2.) * (x - wh);
     const double f2 = (eps * 2.) * a.lambda;
     double aa = 0., bb = 0.;
+    if (a.mutant == 3) std::swap(n1, n2);
     if (a.kind == U_NZSL_L2) { aa = f2 * (1. / n1); bb = f2 * (1. / n2); }
     else if (a.kind == U_NZSL_NZL2) { aa = f2; bb = f2; }
+    if (a.mutant == 1) aa = bb = 0.;
     for (int64_t z = 0; z < r; ++z) {
         double wn = std::fma(-f1, hd[z], std::fma(-aa, wd[z], wd[z]));
-        double hn = std::fma(-f1, wd[z], std::fma(-bb, hd[z], hd[z]));
+        double hn = std::fma(-f1, a.mutant == 2 ? wn : wd[z], std::fma(-bb, hd[z], hd[z]));
         if (a.truncate) {
             wn = (wn <= a.tmin) ? a.tmin + eps : ((wn >= a.tmax) ? a.tmax - eps : wn);
             hn = (hn <= a.tmin) ? a.tmin + eps : ((hn >= a.tmax) ? a.tmax - eps : hn);
@@ -555,7 +564,7 @@ static void runBlock(Ctx* c, TaskRunner& tr, uint32_t b1, uint32_t b2, double ep
     const uint64_t n = B.i.size();
     const int64_t r = c->r;
     const int64_t o1 = c->off1[b1], o2 = c->off2[b2];
-    UpdateArgs ua{c->updKind, c->updLambda, r, c->truncate, c->tmin, c->tmax};
+    UpdateArgs ua{c->updKind, c->updLambda, r, c->truncate, c->tmin, c->tmax, c->mutant};
     double* W = c->W.data();
     double* H = c->H.data();
     const int mode = c->arith;
@@ -1029,7 +1038,12 @@ int orc_get_schedule(orc_ctx* c, uint32_t* out) {
 int orc_set_record(orc_ctx* c, int on) { c->record = on != 0; return 0; }
 
 int orc_set_arith(orc_ctx* c, int mode) {
-    if (mode < 0 || mode > 3) { setErr(c, "invalid arithmetic mode"); return -1; }
+    const int mutant = mode >> 4;
+    mode &= 15;
+    if (mode < 0 || mode > 3 || mutant < 0 || mutant > 3 || (mutant && mode == 0)) {
+        setErr(c, "invalid arithmetic mode"); return -1;
+    }
+    c->mutant = mutant;
     if ((mode == 1 || mode == 2) && c->r > 128) { setErr(c, "float emulation needs rank <= 128"); return -1; }
     if (mode == 3 && c->r > 128) { setErr(c, "FP64 grouped emulation needs rank <= 128"); return -1; }
     c->arith = mode;
@@ -1213,6 +1227,18 @@ void orc_update_point(int kind, double lambda, int64_t r, double* w, double* h, 
                       int truncate) {
     UpdateArgs a{kind, lambda, r, truncate != 0, tmin, tmax};
     updatePoint(a, w, h, x, eps, nnz1, nnz2);
+}
+
+int orc_update_point_arith(int arith, int mutant, int kind, double lambda, int64_t r, double* w, double* h,
+                           double x, double eps, double nnz1, double nnz2, double tmin, double tmax,
+                           int truncate) {
+    if (arith < 0 || arith > 3 || (arith > 0 && r > 128) || mutant < 0 || mutant > 3 || (mutant && !arith))
+        return -1;
+    UpdateArgs a{kind, lambda, r, truncate != 0, tmin, tmax, mutant};
+    if (arith == 0) updatePoint(a, w, h, x, eps, nnz1, nnz2);
+    else if (arith == 3) updatePointD(a, w, h, x, eps, nnz1, nnz2);
+    else updatePointF(a, arith, w, h, x, eps, nnz1, nnz2);
+    return 0;
 }
 
 void orc_permute(uint64_t key64, uint64_t first_draw, int64_t n, uint32_t* perm, int carry) {

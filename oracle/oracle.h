@@ -55,7 +55,8 @@ int orc_balance(orc_ctx* c, int type);
 int orc_get_schedule(orc_ctx* c, uint32_t* out /* 2d x d row-major */);
 int orc_set_record(orc_ctx* c, int on);
 /* 0: FP64 reference arithmetic; 1/2/3: emulate the GPU grouped FP32/BF16/FP64
- * update (call after loading data and factors) */
+ * update (call after loading data and factors).  mode | (k << 4) plants
+ * mistake k of orc_update_point_arith in the emulation (pin tests only). */
 int orc_set_arith(orc_ctx* c, int mode);
 int orc_get_permutation(orc_ctx* c, int task, int subepoch, uint32_t* out,
                         int64_t cap, int64_t* n);
@@ -86,6 +87,14 @@ void orc_block_offsets(int64_t size, int64_t blocks, int64_t* out /* blocks+1 */
 void orc_update_point(int kind, double lambda, int64_t r, double* w, double* h, double x,
                       double eps, double nnz1, double nnz2, double tmin, double tmax,
                       int truncate);
+/* the same step in another arithmetic: 0 the reference's FP64 association,
+ * 1 FP32 / 2 BF16 / 3 FP64 emulation of the GPU grouped update (rank <= 128).
+ * mutant (tests of the emulation's pins only; 0 = none): 1 drops the
+ * regularization term, 2 updates h with the new w, 3 swaps nnz1 and nnz2.
+ * Returns -1 on invalid arguments. */
+int orc_update_point_arith(int arith, int mutant, int kind, double lambda, int64_t r, double* w, double* h,
+                           double x, double eps, double nnz1, double nnz2, double tmin, double tmax,
+                           int truncate);
 void orc_permute(uint64_t key64, uint64_t first_draw, int64_t n, uint32_t* perm, int carry);
 int orc_format_trace(int n, const double* loss, const double* test_loss, int has_test,
                      const double* eps, const double* t_eps, const double* t_epoch,

@@ -14,6 +14,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 CLI = os.path.join(HERE, "mfdsgdpp_cpu")
 SGD = {"SEQ": 0, "WR": 1, "WOR": 2}
 STRATUM = {"SEQ": 0, "RSEQ": 1, "WR": 2, "WOR": 3, "COWOR": 4}
+ARITH = {"FP64": 0, "FP32": 1, "BF16": 2, "FP64_GROUPED": 3}
 
 _lib = None
 
@@ -59,6 +60,7 @@ def lib():
             "orc_dsgdpp_schedule": (i32, [i32, i32, i32, u64, pu32, pu64]),
             "orc_block_offsets": (None, [i64, i64, pi64]),
             "orc_update_point": (None, [i32, dbl, i64, pd, pd, dbl, dbl, dbl, dbl, dbl, dbl, i32]),
+            "orc_update_point_arith": (i32, [i32, i32, i32, dbl, i64, pd, pd, dbl, dbl, dbl, dbl, dbl, dbl, i32]),
             "orc_permute": (None, [u64, u64, i64, pu32, i32]),
             "orc_format_trace": (i32, [i32, pd, pd, i32, pd, pd, pd, pd, pd, cp, cp, ctypes.c_char_p, i64]),
         }
@@ -138,10 +140,12 @@ class Oracle:
     def balance(self, kind):
         self._check(lib().orc_balance(self._h, {"None": 0, "L2": 1, "NZL2": 2}[kind]))
 
-    def set_arith(self, mode):
-        """0 FP64 reference; 'FP32'/'BF16'/'FP64_GROUPED': emulate the GPU grouped update."""
-        m = {"FP64": 0, "FP32": 1, "BF16": 2, "FP64_GROUPED": 3}.get(mode, mode)
-        self._check(lib().orc_set_arith(self._h, int(m)))
+    def set_arith(self, mode, mutant=0):
+        """0 FP64 reference; 'FP32'/'BF16'/'FP64_GROUPED': emulate the GPU grouped
+        update.  mutant k > 0 plants mistake k in the emulation (see
+        update_point_arith); only the tests of the emulation's pins use it."""
+        m = ARITH.get(mode, mode)
+        self._check(lib().orc_set_arith(self._h, int(m) | (int(mutant) << 4)))
 
     def record(self, on=True):
         self._check(lib().orc_set_record(self._h, 1 if on else 0))
@@ -233,6 +237,22 @@ def update_point(kind, lam, w, h, x, eps, n1, n2, tmin=0.0, tmax=0.0, truncate=F
     k = {"Nzsl": 0, "Nzsl_L2": 1, "Nzsl_Nzl2": 2}[kind] if isinstance(kind, str) else kind
     lib().orc_update_point(k, float(lam), w.size, _p(w, ctypes.c_double), _p(h, ctypes.c_double), float(x),
                            float(eps), float(n1), float(n2), float(tmin), float(tmax), 1 if truncate else 0)
+    return w, h
+
+
+def update_point_arith(arith, kind, lam, w, h, x, eps, n1, n2, tmin=0.0, tmax=0.0, truncate=False, mutant=0):
+    """One update step in arithmetic `arith` ('FP64' = the reference association,
+    'FP32' / 'BF16' / 'FP64_GROUPED' = the GPU grouped kernel's); mutant k > 0
+    plants mistake k (1 drops the regularization term, 2 updates h with the new
+    w, 3 swaps nnz1 and nnz2) for the tests that show the pins catch it."""
+    w = np.array(w, dtype=np.float64)
+    h = np.array(h, dtype=np.float64)
+    k = {"Nzsl": 0, "Nzsl_L2": 1, "Nzsl_Nzl2": 2}[kind] if isinstance(kind, str) else kind
+    rc = lib().orc_update_point_arith(ARITH.get(arith, arith), int(mutant), k, float(lam), w.size,
+                                      _p(w, ctypes.c_double), _p(h, ctypes.c_double), float(x), float(eps),
+                                      float(n1), float(n2), float(tmin), float(tmax), 1 if truncate else 0)
+    if rc != 0:
+        raise ValueError("invalid arguments to update_point_arith")
     return w, h
 
 

@@ -1,8 +1,9 @@
 """Batched epochs (one persistent launch over several independent problems)
 and the task-count autotuner.  GPU tests: every problem of a batch ends each
-epoch bit-identical to the same problem run alone, with and without the
-captured graph; the rank-128 lane layout of the grouped kernel matches the
-oracle's float emulation bit for bit."""
+epoch bit-identical to the oracle's emulation of its precision (the grouped
+kernel's association on the same schedules and visit orders), with and
+without the captured graph; the rank-128 lane layout of the grouped kernel
+matches the oracle's float emulation bit for bit."""
 import numpy as np
 import pytest
 
@@ -74,34 +75,50 @@ def _problems():
     return out
 
 
+def _oracle(p, tasks, seed, lam, precision="FP32"):
+    from oracle import pyoracle as po
+    upd = f"Nzsl_L2({lam})"
+    o = po.Oracle(update=upd, loss=upd, rank=p["rank"], tasks_per_rank=tasks, seed=seed, threads=4)
+    o.load_coo(0, p["m"], p["n"], *p["train"])
+    o.set_factors(p["W0"], p["H0"])
+    o.set_arith(precision)
+    return o
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_graph,pairs", [(True, None), (False, None), (True, "1")])
-def test_batch_bitexact_vs_individual(monkeypatch, use_graph, pairs):
-    """pairs "1": every eligible block (FP32, rank <= 64) in two-slot groups."""
-    print(f"case: batch of 4 (ranks 16/64/96/128), graph={use_graph}, pairs={pairs}", flush=True)
+@pytest.mark.parametrize("use_graph,pairs,precision", [(True, None, "FP32"), (False, None, "FP32"),
+                                                       (True, "1", "FP32"), (True, None, "BF16")])
+def test_batch_bitexact_vs_oracle(monkeypatch, use_graph, pairs, precision):
+    """Every problem of a batch ends each epoch bit-identical to the oracle's
+    emulation of the same precision (same schedules and visit orders, the
+    grouped kernel's association), with and without the captured graph.
+    pairs "1": every eligible block (FP32, rank <= 64) in two-slot groups."""
+    print(f"case: batch of 4 (ranks 16/64/96/128) {precision}, graph={use_graph}, pairs={pairs}", flush=True)
     if pairs is not None:
         monkeypatch.setenv("DSGDPP_PAIRS", pairs)
     probs = _problems()
-    batch_ctx = [_ctx(p, d, seed=10 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
-    solo_ctx = [_ctx(p, d, seed=10 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    batch_ctx = [_ctx(p, d, seed=10 + k, lam=lam, precision=precision) for k, (p, d, lam) in enumerate(probs)]
+    orcs = [_oracle(p, d, seed=10 + k, lam=lam, precision=precision) for k, (p, d, lam) in enumerate(probs)]
     bt = B.DsgdPpBatch(batch_ctx, use_graph=use_graph)
     eps = np.array([0.01, 0.02, 0.005, 0.015])
     for e in range(3):
         st = bt.epoch(eps * (1 + 0.1 * e))
         assert st["launches"] == 1 and st["updates"] == sum(len(p[0]["train"][2]) for p in probs)
-        for k, g in enumerate(solo_ctx):
-            g.epoch(eps[k] * (1 + 0.1 * e))
+        for k, o in enumerate(orcs):
+            o.epoch(eps[k] * (1 + 0.1 * e))
         for k in range(len(probs)):
             Wb, Hb = batch_ctx[k].get_factors()
-            Ws, Hs = solo_ctx[k].get_factors()
-            np.testing.assert_array_equal(Wb, Ws, err_msg=f"problem {k} epoch {e}")
-            np.testing.assert_array_equal(Hb, Hs, err_msg=f"problem {k} epoch {e}")
+            Wo, Ho = orcs[k].get_factors()
+            np.testing.assert_array_equal(Wb, Wo, err_msg=f"problem {k} epoch {e}")
+            np.testing.assert_array_equal(Hb, Ho, err_msg=f"problem {k} epoch {e}")
+            lb, lo = batch_ctx[k].loss(0), orcs[k].loss(0)
+            assert abs(lb - lo) <= 1e-10 * abs(lo), (k, e, lb, lo)
     # the contexts stay usable on their own after batched epochs
     batch_ctx[0].epoch(0.01)
-    solo_ctx[0].epoch(0.01)
-    np.testing.assert_array_equal(batch_ctx[0].get_factors()[0], solo_ctx[0].get_factors()[0])
+    orcs[0].epoch(0.01)
+    np.testing.assert_array_equal(batch_ctx[0].get_factors()[0], orcs[0].get_factors()[0])
     bt.close()
-    for g in batch_ctx + solo_ctx:
+    for g in batch_ctx:
         g.close()
 
 

@@ -417,3 +417,70 @@ def test_trace_python_mirror_matches():
     for k, v in (("rank", 10.0), ("nodes", 1.0), ("threads", 4.0), ("update", "Nzsl_L2(1)"), ("barrier", "false")):
         t.add_field(k, v)
     assert t.to_r("tr") == GOLDEN_TRACE
+
+
+# ---------------------------------------------------------------------------
+# balanceSimple (balance_impl.cc:102-127): W *= f, H *= 1/f with
+# f = (reg(H) / reg(W))^(1/4), reg = L2 or count-weighted NZL2.  Pinned by what
+# the mathematics fixes: afterwards reg(W) = reg(H) = sqrt(reg(W) reg(H)), the
+# product W H (hence the NZSL term) is unchanged, a second balance is the
+# identity, and 0/0 replaces both factors by zero matrices.
+# ---------------------------------------------------------------------------
+def _np_losses(p, W, H, cnt1=None, cnt2=None):
+    i, j, x = p["train"]
+    pred = np.einsum("kz,zk->k", W[i], H[:, j])
+    nzsl = float(((x - pred) ** 2).sum())
+    if cnt1 is None:
+        return nzsl, float((W ** 2).sum()), float((H ** 2).sum())
+    return nzsl, float((cnt1[:, None] * W ** 2).sum()), float((cnt2[None, :] * H ** 2).sum())
+
+
+@pytest.mark.parametrize("kind", ["L2", "NZL2"])
+def test_balance_equalizes_regularizers(kind):
+    p = random_problem(50, 70, 900, 5, seed=31)
+    W0 = p["W0"] * 3.0          # unbalanced on purpose: reg(W) ~ 9x, reg(H) as drawn
+    q = dict(p, W0=W0)
+    o = _oracle(q, "Nzsl_L2(0.1)" if kind == "L2" else "Nzsl_Nzl2(0.1)", tasks=3)
+    i, j, _ = p["train"]
+    c1 = np.bincount(i, minlength=p["m"]).astype(float) if kind == "NZL2" else None
+    c2 = np.bincount(j, minlength=p["n"]).astype(float) if kind == "NZL2" else None
+    nz0, rw0, rh0 = _np_losses(p, W0, p["H0"], c1, c2)
+    o.balance(kind)
+    W1, H1 = o.get_factors()
+    nz1, rw1, rh1 = _np_losses(p, W1, H1, c1, c2)
+    f = (rh0 / rw0) ** 0.25
+    np.testing.assert_allclose(W1, W0 * f, rtol=1e-14)
+    np.testing.assert_allclose(H1, p["H0"] / f, rtol=1e-14)
+    assert rw1 == pytest.approx(rh1, rel=1e-13)
+    assert rw1 == pytest.approx(math.sqrt(rw0 * rh0), rel=1e-13)
+    assert nz1 == pytest.approx(nz0, rel=1e-13)
+    o.balance(kind)                                   # idempotent
+    W2, H2 = o.get_factors()
+    np.testing.assert_allclose(W2, W1, rtol=1e-14)
+    np.testing.assert_allclose(H2, H1, rtol=1e-14)
+
+
+def test_balance_none_and_zero_factors():
+    p = random_problem(20, 30, 200, 3, seed=32)
+    o = _oracle(p, "Nzsl_L2(0.1)", tasks=2)
+    o.balance("None")
+    W, H = o.get_factors()
+    assert np.array_equal(W, p["W0"]) and np.array_equal(H, p["H0"])
+    z = dict(p, W0=np.zeros_like(p["W0"]), H0=np.zeros_like(p["H0"]))
+    o = _oracle(z, "Nzsl_L2(0.1)", tasks=2)
+    o.balance("L2")                                   # 0/0: NaN multiplier -> zero matrices
+    W, H = o.get_factors()
+    assert not np.isnan(W).any() and not np.isnan(H).any()
+    assert (W == 0).all() and (H == 0).all()
+
+
+def test_balance_in_run_loop_after_every_epoch():
+    """defaultRunner balances after each epoch, before the loss (sgd_impl.h:110-122):
+    with balance L2 every traced loss has reg(W) = reg(H)."""
+    p = random_problem(60, 80, 1500, 4, seed=33)
+    o = _oracle(dict(p, W0=p["W0"] * 2), "Nzsl_L2(0.05)", tasks=3)
+    r = o.run(3, "BoldDriver(0.01)", balance=1)
+    W, H = o.get_factors()
+    assert (W ** 2).sum() == pytest.approx((H ** 2).sum(), rel=1e-12)
+    nz, rw, rh = _np_losses(p, W, H)
+    assert r["loss"][-1] == pytest.approx(nz + 0.05 * (rw + rh), rel=1e-12)
