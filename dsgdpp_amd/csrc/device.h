@@ -19,20 +19,18 @@ namespace dsgdpp {
                                      " at " #call);                                 \
     } while (0)
 
-// A cooperative launch (all workgroups resident at once: the dataflow tasks
-// wait on each other).  A second process's cooperative launch on a device
-// whose cooperative launches another process has used fails with
-// hipErrorIllegalState (observed on MI355X); the message names that case.
-inline void launch_cooperative(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t s,
-                               const char* what) {
-    const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, lds, s);
+// A dataflow launch: the workgroups wait on each other's completion counters,
+// so all of them must be resident at once.  The caller has checked that the
+// grid fits with the occupancy API, and every wait carries a WaitLimit, so a
+// workgroup that never starts (another kernel spinning on the device) ends the
+// epoch with an error instead of a hang.  An ordinary launch, not a
+// cooperative one: these kernels need no grid barrier, and the runtime serves
+// cooperative launches to one process per device at a time.
+inline void launch_dataflow(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t s,
+                            const char* what) {
+    const hipError_t e = hipLaunchKernel(fn, grid, block, args, lds, s);
     if (e != hipSuccess)
-        throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " at the cooperative launch of " +
-                                 what +
-                                 (e == hipErrorIllegalState
-                                      ? " (cooperative launches serve one process per device at a time: another "
-                                        "process on this GPU may be running dataflow epochs; kernel 1 needs none)"
-                                      : ""));
+        throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " at the launch of " + what);
 }
 
 // ---------------------------------------------------------------------------
@@ -117,6 +115,43 @@ __device__ __forceinline__ void dataflow_wait(int32_t* ready, int b2, int32_t ne
     while (__hip_atomic_load(ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
         __builtin_amdgcn_s_sleep(1);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// A time limit on dataflow waits, for launches whose workgroups are not all
+// guaranteed to be resident (batched epochs are ordinary launches): after
+// `limit` ticks of the constant-rate wall clock, or once another wave of the
+// launch has given up (*status != 0), the wait gives up and sets *status.
+// Every wave then leaves its task, so the grid drains and the host reports
+// the error instead of the GPU waiting on a workgroup that never started.
+// limit 0: no limit (cooperative launches).  Checked every 64th poll.
+struct WaitLimit {
+    unsigned long long t0;
+    uint32_t limit, polls;
+    int32_t* status;
+    __device__ __forceinline__ WaitLimit(uint32_t lim, int32_t* st) : t0(0), limit(lim), polls(0), status(st) {
+        if (limit) t0 = (unsigned long long)wall_clock64();
+    }
+    __device__ __forceinline__ bool expired() {
+        if (limit == 0 || (polls++ & 63u) != 0) return false;
+        if ((unsigned long long)wall_clock64() - t0 <= limit &&
+            __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            return false;
+        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+};
+
+// dataflow_wait with a WaitLimit; false when it gave up.
+__device__ __forceinline__ bool dataflow_wait_limited(int32_t* ready, int b2, int32_t need, uint32_t limit,
+                                                      int32_t* status) {
+    if (need <= 0) return true;
+    WaitLimit wl(limit, status);
+    while (__hip_atomic_load(ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (wl.expired()) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
 }
 
 // The same from a single wave (its own stores; no workgroup barrier).

@@ -212,6 +212,11 @@ struct dsgdpp_ctx {
     double* dPartials = nullptr;
     double* hPinned = nullptr;
     int32_t* dReady = nullptr;
+    // dataflow launches are ordinary launches: their waits give up after a
+    // time limit and set *dStatus, which the epoch turns into an error
+    int32_t* dStatus = nullptr;
+    int32_t* hStatus = nullptr;   // pinned
+    int wallKhz = 0;              // wall-clock ticks per millisecond
     bool dataflowOk = false;
     bool groupedOk = false;
     std::string groupedWhy;
@@ -260,6 +265,20 @@ int guarded(dsgdpp_ctx* c, F f) {
         return DSGDPP_E_DEVICE;
     }
 }
+
+// Time limit of the dataflow waits, in wall-clock ticks: DSGDPP_WAIT_LIMIT_MS
+// (default 30,000 ms; a legitimate wait lasts at most one epoch), at least
+// one tick.
+uint32_t dataflowWaitTicks(int wallKhz) {
+    double ms = 30000.0;
+    if (const char* e = getenv("DSGDPP_WAIT_LIMIT_MS")) ms = strtod(e, nullptr);
+    const double ticks = ms * (double)wallKhz;
+    return (uint32_t)std::min(4.0e9, std::max(1.0, ticks));
+}
+
+const char* kGaveUp =
+    "a dataflow wait gave up (a workgroup of the launch was not resident, e.g. another kernel that waits on its "
+    "own workgroups held the device)";
 
 void dfree(void*& p) {
     if (p) (void)hipFree(p);
@@ -538,7 +557,7 @@ void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const
         c->G.max_cols = maxCols;
         c->maxBlock = maxBlock;
         bool wide = false;
-        c->groupedOk = grouped_supported(c->P, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
+        c->groupedOk = grouped_supported(c->P, c->upd, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
                                          c->kernel == 4);
         c->G.wide = wide;
         if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
@@ -901,6 +920,12 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     L.use_perm = c->sgdOrder == DSGDPP_SGD_SEQ ? 0 : 1;
     L.dataflow = P.dataflow ? 1 : 0;
     L.eps = eps;
+    const uint32_t waitTicks = dataflowWaitTicks(c->wallKhz);
+    if (P.dataflow) {
+        L.status = c->dStatus;
+        L.wait_ticks = waitTicks;
+        *c->hStatus = 0;
+    }
     DevProblem Dp = c->P;
     Dp.perm = P.perm;
     if (P.grouped) {
@@ -917,6 +942,8 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     } else {
         launch_epoch(Dp, c->upd, L, c->stream);
     }
+    if (P.dataflow)
+        HIPC(hipMemcpyAsync(c->hStatus, c->dStatus, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipEventRecord(c->ev[2], c->stream));
     const int used = c->cur;
     c->lastPlan = used;
@@ -930,6 +957,10 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     }
     HIPC(hipEventSynchronize(c->ev[2]));
     const auto t1 = std::chrono::steady_clock::now();
+    if (P.dataflow && *c->hStatus != 0)
+        throw Error(DSGDPP_E_DEVICE, std::string("epoch failed: ") + kGaveUp + " after " +
+                                         std::to_string(waitTicks / std::max(1, c->wallKhz)) +
+                                         " ms; the factors are undefined until set again");
     if (c->kprof && P.grouped && c->G.pprof) {
         unsigned long long h[4];
         HIPC(hipMemcpy(h, c->G.pprof + (used == 0 ? 0 : 4), sizeof(h), hipMemcpyDeviceToHost));
@@ -1352,6 +1383,11 @@ int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out) {
         c->pairsMode = pm && (pm[0] == '0' || pm[0] == '1') ? pm[0] - '0' : -1;
         dalloc(c->dPartials, (size_t)partials_capacity() * sizeof(double));
         HIPC(hipHostMalloc((void**)&c->hPinned, sizeof(double) * 8, hipHostMallocDefault));
+        HIPC(hipHostMalloc((void**)&c->hStatus, sizeof(int32_t), hipHostMallocDefault));
+        *c->hStatus = 0;
+        dalloc(c->dStatus, sizeof(int32_t));
+        HIPC(hipDeviceGetAttribute(&c->wallKhz, hipDeviceAttributeWallClockRate, c->device));
+        if (c->wallKhz <= 0) c->wallKhz = 100000;
     } catch (const std::exception& e) {
         g_createError = e.what();
         const int code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : DSGDPP_E_DEVICE;
@@ -1395,8 +1431,10 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     v = c->G.prof; dfree(v);
     v = c->G.pprof; dfree(v);
     v = c->dReady; dfree(v);
+    v = c->dStatus; dfree(v);
     { void* q = c->dPairs; dfree(q); c->dPairs = nullptr; }
     if (c->hPinned) (void)hipHostFree(c->hPinned);
+    if (c->hStatus) (void)hipHostFree(c->hStatus);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1604,6 +1642,12 @@ struct dsgdpp_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool useGraph = true;
+    // The launch is an ordinary one (a replayed graph node), so residency of
+    // all its workgroups rests on the occupancy check at create time; its
+    // dataflow waits give up after waitMs and set *dStatus instead of hanging.
+    int32_t* dStatus = nullptr;
+    int32_t* hStatus = nullptr;   // pinned
+    int wallKhz = 0;              // wall-clock ticks per millisecond
     std::string err;
 };
 
@@ -1618,7 +1662,10 @@ void batchDestroy(dsgdpp_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->hArgs) (void)hipHostFree(b->hArgs);
-    void* q = b->dArgs;
+    if (b->hStatus) (void)hipHostFree(b->hStatus);
+    void* q = b->dStatus;
+    dfree(q);
+    q = b->dArgs;
     dfree(q);
     q = b->dTmap;
     dfree(q);
@@ -1634,9 +1681,11 @@ void batchDestroy(dsgdpp_batch* b) {
     delete b;
 }
 
+
 void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t K = b->ctx.size();
+    const uint32_t waitTicks = dataflowWaitTicks(b->wallKhz);
     int64_t updates = 0;
     for (size_t k = 0; k < K; ++k) {
         dsgdpp_ctx* c = b->ctx[k];
@@ -1664,7 +1713,7 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         GroupedBuffers G = c->G;
         G.grec = P.grec;
         G.ngroups = P.ngroups;
-        fill_grouped_args(Dp, c->upd, G, L, b->hArgs + k * b->argBytes);
+        fill_grouped_args(Dp, c->upd, G, L, b->dStatus, waitTicks, b->hArgs + k * b->argBytes);
         updates += c->blocks[0].nnz;
     }
     const auto t_prep = std::chrono::steady_clock::now();
@@ -1674,13 +1723,14 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         // one graph: the counters' reset and the launch; the table it reads is refreshed above
         HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
-                           b->ready.data(), b->dsz.data(), (int)K, b->stream);
+                           b->ready.data(), b->dsz.data(), (int)K, b->dStatus, b->stream);
         HIPC(hipStreamEndCapture(b->stream, &b->graph));
         HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     if (b->useGraph) HIPC(hipGraphLaunch(b->exec, b->stream));
     else launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
-                            b->ready.data(), b->dsz.data(), (int)K, b->stream);
+                            b->ready.data(), b->dsz.data(), (int)K, b->dStatus, b->stream);
+    HIPC(hipMemcpyAsync(b->hStatus, b->dStatus, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
     HIPC(hipEventRecord(b->ev1, b->stream));
     const auto t_launch = std::chrono::steady_clock::now();
     // every context's next plan is prepared while the batch runs: the host
@@ -1754,6 +1804,11 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     HIPC(hipEventSynchronize(b->ev1));
     // the contexts' own streams see the batch's writes before their next work
     for (dsgdpp_ctx* c : b->ctx) HIPC(hipStreamWaitEvent(c->stream, b->ev1, 0));
+    if (*b->hStatus != 0)
+        throw Error(DSGDPP_E_DEVICE, std::string("batched epoch failed: ") + kGaveUp + " after " +
+                                         std::to_string(waitTicks / std::max(1, b->wallKhz)) +
+                                         " ms; the factors of every context in this batch are undefined until set "
+                                         "again");
     const auto t1 = std::chrono::steady_clock::now();
     if (b->ctx[0]->kprof) {
         auto ms = [](auto a, auto z) { return std::chrono::duration<double, std::milli>(z - a).count(); };
@@ -1835,6 +1890,11 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
         HIPC(hipEventCreate(&b->ev0));
         HIPC(hipEventCreate(&b->ev1));
         HIPC(hipHostMalloc((void**)&b->hArgs, (size_t)count * b->argBytes, hipHostMallocDefault));
+        HIPC(hipHostMalloc((void**)&b->hStatus, sizeof(int32_t), hipHostMallocDefault));
+        *b->hStatus = 0;
+        dalloc(b->dStatus, sizeof(int32_t));
+        HIPC(hipDeviceGetAttribute(&b->wallKhz, hipDeviceAttributeWallClockRate, b->device));
+        if (b->wallKhz <= 0) b->wallKhz = 100000;
         dalloc(b->dArgs, (size_t)count * b->argBytes);
         dalloc(b->dTmap, tmap.size() * sizeof(int2));
         HIPC(hipMemcpy(b->dTmap, tmap.data(), tmap.size() * sizeof(int2), hipMemcpyHostToDevice));

@@ -79,6 +79,8 @@ struct EpochLaunch {
     const int64_t* goff = nullptr;     // 2d x d: first group of each visit (grouped epoch)
     const int32_t* need = nullptr;     // 2d x d (dataflow)
     int32_t* ready = nullptr;          // 2d (dataflow)
+    int32_t* status = nullptr;         // dataflow: set when a wait gave up (WaitLimit)
+    uint32_t wait_ticks = 0;           // dataflow: time limit of the waits, wall-clock ticks
     int use_perm = 1;
     int dataflow = 0;
     double eps = 0;
@@ -127,7 +129,8 @@ struct PrepDesc {
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 // grouped epoch eligibility; *wide (optional) is set when only the variant
 // that streams H rows from HBM fits (FP32 blocks too wide for the LDS tile)
-bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols,
+// (residency is checked on the variant U selects)
+bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_block, int max_rows, int max_cols,
                        int device, std::string* why, bool* wide = nullptr, bool force_wide = false);
 PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs, const int32_t* d_chain_start,
                    int order, int64_t max_block);
@@ -146,12 +149,15 @@ void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const Grouped
 // batched grouped epoch: one launch over several problems (same precision,
 // update kind and truncation); per-problem arguments live in device memory
 size_t grouped_args_bytes(int precision);
+// status / wait_ticks: the launch's give-up word and the time limit of its
+// dataflow waits in wall-clock ticks (the launch is not cooperative)
 void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
-                       void* out);
+                       int32_t* status, uint32_t wait_ticks, void* out);
 size_t batch_epoch_lds(int max_cols, int64_t max_ld, bool wide);
 int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device);
 void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
-                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, hipStream_t s);
+                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, int32_t* status,
+                        hipStream_t s);
 double device_nzsl(const DevProblem& P, int which, double* d_partials, double* h_pinned,
                    hipStream_t s);
 double device_sumsq(const DevProblem& P, int which_factor, const double* d_weights,

@@ -97,6 +97,7 @@ struct SgdArgs {
     const int64_t* __restrict__ voff;     // same shape as sched
     const int32_t* __restrict__ need;     // 2d x d: earlier non-empty visits of the column block
     int32_t* ready;                       // 2d: completed visits per column block
+    int32_t* status;                      // set when a dataflow wait gave up (WaitLimit)
     T* W;
     T* H;
     const C* __restrict__ inv1;
@@ -105,6 +106,7 @@ struct SgdArgs {
     int64_t ld;
     C eps, lambda, tmin, tmax;
     int kind, truncate, use_perm;
+    uint32_t wait_ticks;                  // WaitLimit of the dataflow waits
 };
 
 template <typename T, int K, bool STRICT>
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(64) void k_epoch_dataflow_seq(SgdArgs<T> A) {
         const int64_t s0 = A.blk[b];
         const int64_t n = A.blk[b + 1] - s0;
         if (n == 0) continue;
-        dataflow_wait(A.ready, b2, A.need[s * d + t]);
+        if (!dataflow_wait_limited(A.ready, b2, A.need[s * d + t], A.wait_ticks, A.status)) return;
         run_block_seq<T, K, STRICT>(A, s0, n, A.perm + (A.use_perm ? A.voff[s * d + t] : 0));
         dataflow_signal(A.ready, b2);
     }
@@ -502,6 +504,8 @@ static SgdArgs<T> make_args(const DevProblem& P, const UpdParams& U, const uint3
     A.kind = U.kind;
     A.truncate = U.truncate ? 1 : 0;
     A.use_perm = use_perm;
+    A.status = nullptr;
+    A.wait_ticks = 0;
     return A;
 }
 
@@ -509,11 +513,14 @@ template <typename T, int K, bool STRICT>
 static void launch_t(const DevProblem& P, const UpdParams& U, const EpochLaunch& L, hipStream_t s) {
     if (L.dataflow) {
         SgdArgs<T> A = make_args<T>(P, U, L.sched, L.voff, L.need, L.ready, L.use_perm, L.eps);
+        A.status = L.status;
+        A.wait_ticks = L.wait_ticks;
         k_fill_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d, 0);
+        if (L.status) k_fill_i32<<<1, 64, 0, s>>>(L.status, 1, 0);
         HIPK(hipGetLastError());
         void* args[] = {&A};
-        launch_cooperative((const void*)k_epoch_dataflow_seq<T, K, STRICT>, dim3(P.d), dim3(64), args, 0, s,
-                           "k_epoch_dataflow_seq");
+        launch_dataflow((const void*)k_epoch_dataflow_seq<T, K, STRICT>, dim3(P.d), dim3(64), args, 0, s,
+                        "k_epoch_dataflow_seq");
     } else {
         for (int st = 0; st < 2 * P.d; ++st) {
             SgdArgs<T> A = make_args<T>(P, U, L.sched + (size_t)st * P.d, L.voff + (size_t)st * P.d,
@@ -542,9 +549,21 @@ void launch_epoch(const DevProblem& P, const UpdParams& U, const EpochLaunch& L,
     }
 }
 
+// The dataflow kernel an epoch of this problem launches (storage type, rank
+// class, strict association): its occupancy decides residency.
+template <typename T, bool STRICT>
+static const void* dataflow_fn_t(int64_t r) {
+    return r <= 64 ? (const void*)k_epoch_dataflow_seq<T, 1, STRICT>
+           : r <= 128 ? (const void*)k_epoch_dataflow_seq<T, 2, STRICT> : (const void*)k_epoch_dataflow_seq<T, 4, STRICT>;
+}
+
 bool dataflow_fits(const DevProblem& P, int device) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_epoch_dataflow_seq<float, 1, false>, 64, 0) != hipSuccess)
+    const void* fn = P.precision == PREC_FP64_STRICT ? dataflow_fn_t<double, true>(P.r)
+                     : P.precision == PREC_FP64      ? dataflow_fn_t<double, false>(P.r)
+                     : P.precision == PREC_FP32      ? dataflow_fn_t<float, false>(P.r)
+                                                     : dataflow_fn_t<bf16_t, false>(P.r);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0) != hipSuccess)
         return false;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return false;
     return (int64_t)per_cu * cus >= P.d;

@@ -476,11 +476,13 @@ struct GArgs {
     const float* __restrict__ inv2;
     const uint8_t* __restrict__ pairs;   // optional, per block: two-slot groups (FP32, rank <= 64)
     unsigned long long* prof;   // optional: per task {wait, stage, loop, total} clocks + group count
+    int32_t* status;           // batched launches: set when a dataflow wait gave up (WaitLimit)
     uint32_t w_bytes;          // W buffer size (rows m + FACTOR_TAIL_ROWS)
     uint32_t ztail_b;          // byte offset of the zero rows after the last row
     uint32_t dummy_b;          // byte offset of the dummy row
     int d, ld, max_cols;
-    int wide, unused;          // wide: H rows streamed from HBM (batched launches)
+    int wide;                  // H rows streamed from HBM (batched launches)
+    uint32_t wait_ticks;       // WaitLimit of the dataflow waits (0: none, cooperative launches)
     float eps, lambda, tmin, tmax;
 };
 
@@ -518,7 +520,9 @@ static size_t epoch_lds(int max_cols, int vec) {
 // The two waves of a task's workgroup hand H buffers over through two LDS
 // counters: `staged` (the mover has filled the buffer of the task's k-th
 // non-empty visit) and `done` (the compute wave has finished visit k).  Both
-// only order LDS accesses.
+// only order LDS accesses.  A mover whose dataflow wait gave up sets `staged`
+// to GAVE_UP: the compute wave stops at its next visit.
+constexpr int32_t GAVE_UP = 0x7fffffff;
 __device__ __forceinline__ void lds_publish(int32_t* flag, int k) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __hip_atomic_store(flag, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -577,11 +581,16 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
         const int32_t need = uniform_i32(A.need[visit]);
         // buffer k & 1 held visit k - 2, written back before this loop began
         const unsigned long long c0 = A.prof ? clock64() : 0;
+        WaitLimit wl(A.wait_ticks, A.status);
         while (need > 0 && __hip_atomic_load(A.ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
             if (pend_b2 >= 0 && lds_reached(flags + 1, k - 1)) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 write_back();
             } else {
+                if (wl.expired()) {   // no write-back, no signal: the epoch is reported failed
+                    lds_publish(flags + 0, GAVE_UP);
+                    return;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
@@ -791,6 +800,8 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
         const int b2 = uniform_i32((int)A.sched[visit]);
         const int b = t * 2 * d + b2;
         if (A.blk[b + 1] == A.blk[b]) continue;
+        // the mover gave up a dataflow wait (batched launches): stop here
+        if (A.wait_ticks && __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == GAVE_UP) break;
         const int ng = uniform_i32(A.ngroups[visit]);
         const int ngp = (ng + GQ - 1) / GQ * GQ;   // the prep kernel pads with empty groups
         const int64_t gbase = uniform_i64(A.goff[visit]);
@@ -986,8 +997,10 @@ struct GArgs64 {
     const double* __restrict__ inv1;
     const double* __restrict__ inv2;
     unsigned long long* prof;
+    int32_t* status;           // set when a dataflow wait gave up (WaitLimit)
     uint32_t w_bytes, ztail_b, dummy_b, i_bytes;
-    int d, ld, max_cols, unused;
+    int d, ld, max_cols;
+    uint32_t wait_ticks;       // WaitLimit of the dataflow waits
     double eps, lambda, tmin, tmax;
 };
 constexpr int GQ64 = 16;      // record prefetch distance = unroll of the group loop
@@ -1092,11 +1105,16 @@ __device__ __forceinline__ void grouped_mover_f64(const GArgs64& A, const int t,
         const int ncol = uniform_i32(A.off2[b2 + 1]) - col0;
         const int32_t need = uniform_i32(A.need[visit]);
         const unsigned long long c0 = A.prof ? clock64() : 0;
+        WaitLimit wl(A.wait_ticks, A.status);
         while (need > 0 && __hip_atomic_load(A.ready + b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
             if (pend_b2 >= 0 && lds_reached(flags + 1, k - 1)) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 write_back();
             } else {
+                if (wl.expired()) {   // no write-back, no signal: the epoch is reported failed
+                    lds_publish(flags + 0, GAVE_UP);
+                    return;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
@@ -1173,6 +1191,8 @@ __device__ __forceinline__ void grouped_compute_f64(const GArgs64& A, const int 
         const int b2 = uniform_i32((int)A.sched[visit]);
         const int b = t * 2 * d + b2;
         if (A.blk[b + 1] == A.blk[b]) continue;
+        // the mover gave up a dataflow wait: stop here
+        if (A.wait_ticks && __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == GAVE_UP) break;
         const int ng = uniform_i32(A.ngroups[visit]);
         const int ngp = (ng + GQ64 - 1) / GQ64 * GQ64;   // the prep pads to GROUP_PAD, a multiple of GQ64
         const int64_t gbase = uniform_i64(A.goff[visit]);
@@ -1362,7 +1382,9 @@ __device__ __forceinline__ void wide_task(const GArgs<T>& A, const int t, float*
 #pragma unroll
         for (int u = 0; u < GPREF_W; ++u) load_w(qa[u], qw[u]);
         const unsigned long long c0 = A.prof ? clock64() : 0;
-        dataflow_wait(A.ready, b2, A.need[visit]);   // H rows of this column block may be read from here on
+        // H rows of this column block may be read from here on (a wait that
+        // gave up ends the task: the epoch is reported failed)
+        if (!dataflow_wait_limited(A.ready, b2, A.need[visit], A.wait_ticks, A.status)) return;
         const unsigned long long c2 = A.prof ? clock64() : 0;
 #pragma unroll
         for (int u = 0; u < GPREF_W; ++u) load_h(qa[u], qh[u], qi[u]);
@@ -1574,7 +1596,9 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
 #pragma unroll
         for (int u = 0; u < GPREF64; ++u) load_w(qa[u], qw[u], qwi[u]);
         const unsigned long long c0 = A.prof ? clock64() : 0;
-        dataflow_wait(A.ready, b2, A.need[visit]);   // H rows of this column block may be read from here on
+        // H rows of this column block may be read from here on (a wait that
+        // gave up ends the task: the epoch is reported failed)
+        if (!dataflow_wait_limited(A.ready, b2, A.need[visit], A.wait_ticks, A.status)) return;
         const unsigned long long c2 = A.prof ? clock64() : 0;
 #pragma unroll
         for (int u = 0; u < GPREF64; ++u) load_h(qa[u], qh[u], qhi[u]);
@@ -1753,8 +1777,46 @@ __global__ __launch_bounds__(128) void k_epoch_grouped_batch(const GArgs<T>* __r
 // ---------------------------------------------------------------------------
 static int vec_of(int64_t ld) { return ld <= 64 ? 1 : 2; }
 
-bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int max_cols, int device,
-                       std::string* why, bool* wide, bool force_wide) {
+// The kernel variant that an epoch of the problem launches (storage type,
+// update kind, truncation, lane layout): its occupancy decides residency.
+template <typename T, int KIND, int VEC>
+static const void* tile_fn_k(bool tr) {
+    return tr ? (const void*)k_epoch_grouped<T, KIND, true, VEC> : (const void*)k_epoch_grouped<T, KIND, false, VEC>;
+}
+template <typename T, int KIND, int VEC>
+static const void* wide_fn_k(bool tr) {
+    return tr ? (const void*)k_epoch_wide<T, KIND, true, VEC> : (const void*)k_epoch_wide<T, KIND, false, VEC>;
+}
+template <int KIND>
+static const void* tile64_fn_k(bool tr) {
+    return tr ? (const void*)k_epoch_grouped64<KIND, true> : (const void*)k_epoch_grouped64<KIND, false>;
+}
+template <int KIND, bool STRICT, int V>
+static const void* wide64_fn_k(bool tr) {
+    return tr ? (const void*)k_epoch_wide64<KIND, true, STRICT, V> : (const void*)k_epoch_wide64<KIND, false, STRICT, V>;
+}
+template <typename T, int VEC>
+static const void* tile_fn(int kind, bool tr) {
+    return kind == UPD_NZSL ? tile_fn_k<T, UPD_NZSL, VEC>(tr)
+           : kind == UPD_NZSL_NZL2 ? tile_fn_k<T, UPD_NZSL_NZL2, VEC>(tr) : tile_fn_k<T, UPD_NZSL_L2, VEC>(tr);
+}
+template <typename T, int VEC>
+static const void* wide_fn(int kind, bool tr) {
+    return kind == UPD_NZSL ? wide_fn_k<T, UPD_NZSL, VEC>(tr)
+           : kind == UPD_NZSL_NZL2 ? wide_fn_k<T, UPD_NZSL_NZL2, VEC>(tr) : wide_fn_k<T, UPD_NZSL_L2, VEC>(tr);
+}
+static const void* tile64_fn(int kind, bool tr) {
+    return kind == UPD_NZSL ? tile64_fn_k<UPD_NZSL>(tr)
+           : kind == UPD_NZSL_NZL2 ? tile64_fn_k<UPD_NZSL_NZL2>(tr) : tile64_fn_k<UPD_NZSL_L2>(tr);
+}
+template <bool STRICT, int V>
+static const void* wide64_fn(int kind, bool tr) {
+    return kind == UPD_NZSL ? wide64_fn_k<UPD_NZSL, STRICT, V>(tr)
+           : kind == UPD_NZSL_NZL2 ? wide64_fn_k<UPD_NZSL_NZL2, STRICT, V>(tr) : wide64_fn_k<UPD_NZSL_L2, STRICT, V>(tr);
+}
+
+bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_block, int max_rows, int max_cols,
+                       int device, std::string* why, bool* wide, bool force_wide) {
     auto no = [&](const char* m) {
         if (why) *why = m;
         return false;
@@ -1784,19 +1846,18 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
         return (int64_t)per_cu * cus >= P.d;
     };
     const size_t lds = f64 ? epoch_lds64(max_cols) : epoch_lds(max_cols, vec_of(P.ld));
-    const void* fn = f64 ? (const void*)k_epoch_grouped64<UPD_NZSL_L2, false>
-                   : vec_of(P.ld) == 1 ? (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 1>
-                                       : (const void*)k_epoch_grouped<float, UPD_NZSL_L2, false, 2>;
+    const bool bf = P.precision == PREC_BF16, tr = U.truncate;
+    const void* fn = f64 ? tile64_fn(U.kind, tr)
+                   : vec_of(P.ld) == 1 ? (bf ? tile_fn<bf16_t, 1>(U.kind, tr) : tile_fn<float, 1>(U.kind, tr))
+                                       : (bf ? tile_fn<bf16_t, 2>(U.kind, tr) : tile_fn<float, 2>(U.kind, tr));
     // FP64_STRICT (the reference association) and FP64 above rank 64 run on the streaming body only
     if (!strict && !f64_wide_only && !force_wide && lds <= (size_t)lmax && resident(fn, 128, lds)) return true;
     // FP64 blocks too wide for the LDS tile: H rows streamed from HBM
     if (f64 && wide) {
         const int v = P.ld > 64 ? 2 : 1;
         const size_t lw = epoch_lds_wide64(v);
-        const void* fw = strict ? (v == 1 ? (const void*)k_epoch_wide64<UPD_NZSL_L2, false, true, 1>
-                                          : (const void*)k_epoch_wide64<UPD_NZSL_L2, false, true, 2>)
-                                : (v == 1 ? (const void*)k_epoch_wide64<UPD_NZSL_L2, false, false, 1>
-                                          : (const void*)k_epoch_wide64<UPD_NZSL_L2, false, false, 2>);
+        const void* fw = strict ? (v == 1 ? wide64_fn<true, 1>(U.kind, tr) : wide64_fn<true, 2>(U.kind, tr))
+                                : (v == 1 ? wide64_fn<false, 1>(U.kind, tr) : wide64_fn<false, 2>(U.kind, tr));
         if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
             *wide = true;
             return true;
@@ -1805,12 +1866,8 @@ bool grouped_supported(const DevProblem& P, int64_t max_block, int max_rows, int
     // FP32/BF16 blocks too wide for the LDS tile: H rows streamed from HBM
     if ((P.precision == PREC_FP32 || P.precision == PREC_BF16) && wide) {
         const size_t lw = epoch_lds_wide(vec_of(P.ld));
-        const bool bf = P.precision == PREC_BF16;
-        const void* fw = vec_of(P.ld) == 1
-                             ? (bf ? (const void*)k_epoch_wide<bf16_t, UPD_NZSL_L2, false, 1>
-                                   : (const void*)k_epoch_wide<float, UPD_NZSL_L2, false, 1>)
-                             : (bf ? (const void*)k_epoch_wide<bf16_t, UPD_NZSL_L2, false, 2>
-                                   : (const void*)k_epoch_wide<float, UPD_NZSL_L2, false, 2>);
+        const void* fw = vec_of(P.ld) == 1 ? (bf ? wide_fn<bf16_t, 1>(U.kind, tr) : wide_fn<float, 1>(U.kind, tr))
+                                           : (bf ? wide_fn<bf16_t, 2>(U.kind, tr) : wide_fn<float, 2>(U.kind, tr));
         if (lw <= (size_t)lmax && resident(fw, 64, lw)) {
             *wide = true;
             return true;
@@ -1931,7 +1988,8 @@ static GArgs<T> make_gargs(const DevProblem& P, const UpdParams& U, const Groupe
     A.ld = (int)P.ld;
     A.max_cols = B.max_cols;
     A.wide = B.wide ? 1 : 0;
-    A.unused = 0;
+    A.wait_ticks = L.wait_ticks;
+    A.status = L.status;
     A.eps = (float)L.eps;
     A.lambda = (float)U.lambda;
     A.tmin = (float)U.tmin;
@@ -1949,9 +2007,10 @@ static void epoch_grouped_t(const DevProblem& P, const UpdParams& U, const Group
     const void* fn = (const void*)k_epoch_grouped<T, KIND, TRUNC, VEC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+    if (L.status) k_zero_i32<<<1, 64, 0, s>>>(L.status, 1);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    launch_cooperative(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
+    launch_dataflow(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
 }
 
 template <typename T, int KIND, bool TRUNC, int VEC>
@@ -1962,9 +2021,10 @@ static void epoch_wide_t(const DevProblem& P, const UpdParams& U, const GroupedB
     const void* fn = (const void*)k_epoch_wide<T, KIND, TRUNC, VEC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+    if (L.status) k_zero_i32<<<1, 64, 0, s>>>(L.status, 1);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    launch_cooperative(fn, dim3(P.d), dim3(64), args, bytes, s, "the grouped epoch kernel (H streamed)");
+    launch_dataflow(fn, dim3(P.d), dim3(64), args, bytes, s, "the grouped epoch kernel (H streamed)");
 }
 
 template <typename T, int KIND, bool TRUNC>
@@ -2023,6 +2083,8 @@ static GArgs64 make_gargs64(const DevProblem& P, const UpdParams& U, const Group
     A.tmin = U.tmin;
     A.tmax = U.tmax;
     A.prof = B.prof;
+    A.status = L.status;
+    A.wait_ticks = L.wait_ticks;
     return A;
 }
 
@@ -2040,18 +2102,20 @@ static void epoch_grouped64_t(const DevProblem& P, const UpdParams& U, const Gro
                                           : (const void*)k_epoch_wide64<KIND, TRUNC, false, 2>);
         HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
         k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+        if (L.status) k_zero_i32<<<1, 64, 0, s>>>(L.status, 1);
         HIPK(hipGetLastError());
         void* args[] = {&A};
-        launch_cooperative(fn, dim3(P.d), dim3(64), args, bytes, s, "the grouped epoch kernel (FP64, H streamed)");
+        launch_dataflow(fn, dim3(P.d), dim3(64), args, bytes, s, "the grouped epoch kernel (FP64, H streamed)");
         return;
     }
     const size_t bytes = epoch_lds64(B.max_cols);
     const void* fn = (const void*)k_epoch_grouped64<KIND, TRUNC>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     k_zero_i32<<<1, 256, 0, s>>>(L.ready, 2 * P.d);
+    if (L.status) k_zero_i32<<<1, 64, 0, s>>>(L.status, 1);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    launch_cooperative(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
+    launch_dataflow(fn, dim3(P.d), dim3(128), args, bytes, s, "the grouped epoch kernel");
 }
 
 template <int KIND>
@@ -2087,14 +2151,18 @@ size_t grouped_args_bytes(int precision) {
 }
 
 void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B, const EpochLaunch& L,
-                       void* out) {
+                       int32_t* status, uint32_t wait_ticks, void* out) {
     if (P.precision == PREC_FP32) {
         GArgs<float> A = make_gargs<float>(P, U, B, L);
         A.prof = nullptr;
+        A.status = status;
+        A.wait_ticks = wait_ticks;
         memcpy(out, &A, sizeof(A));
     } else {
         GArgs<bf16_t> A = make_gargs<bf16_t>(P, U, B, L);
         A.prof = nullptr;
+        A.status = status;
+        A.wait_ticks = wait_ticks;
         memcpy(out, &A, sizeof(A));
     }
 }
@@ -2128,10 +2196,12 @@ int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device) 
 }
 
 void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
-                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, hipStream_t s) {
+                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, int32_t* status,
+                        hipStream_t s) {
     // (batch_capacity has set the function's dynamic LDS limit; nothing here but
     // stream work, so the sequence can be captured into a graph)
     for (int k = 0; k < nprob; ++k) k_zero_i32<<<1, 256, 0, s>>>(ready[k], 2 * dsz[k]);
+    k_zero_i32<<<1, 64, 0, s>>>(status, 1);
     const void* fn = precision == PREC_FP32 ? batch_fn_p<float>(kind, trunc) : batch_fn_p<bf16_t>(kind, trunc);
     void* args[] = {const_cast<void**>(&d_args), const_cast<int2**>(&d_tmap)};
     HIPK(hipLaunchKernel(fn, dim3(nblocks), dim3(128), args, lds, s));

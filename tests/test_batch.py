@@ -157,3 +157,44 @@ def test_batch_rejects_mixed_contexts():
         B.DsgdPpBatch([a, a])
     for g in (a, b, c):
         g.close()
+
+
+@pytest.mark.gpu
+def test_batch_wait_limit_reports_instead_of_hanging(monkeypatch):
+    """A batched launch is an ordinary launch (a replayed graph node), so its
+    dataflow waits carry a time limit.  Forced from the host to one tick, a
+    wait gives up on its first unmet poll: every wave leaves its task (the
+    tile kernel's mover and compute wave, the H-streaming task), the epoch
+    reports the error, and the next epoch under the default limit is again
+    bit-identical to the oracle from freshly set factors."""
+    from test_gpu_parity import imbalanced_problem
+    print("case: batch wait limit forced to one tick (tile + streaming problems)", flush=True)
+    # 1,000-column blocks at rank 32: two H tiles exceed the LDS, so H is streamed
+    wide = imbalanced_problem(32, d=4, rows=100, cols=1000, heavy=3000, light=12, seed=1)
+    probs = [(wide, 4, 0.05), (imbalanced_problem(64, d=4, seed=2), 4, 0.1),
+             (imbalanced_problem(16, d=4, seed=3), 4, 0.07)]
+    ctxs = [_ctx(p, d, seed=30 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    orcs = [_oracle(p, d, seed=30 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    assert ctxs[0].grouped_ok()[0]
+    bt = B.DsgdPpBatch(ctxs, use_graph=True)
+    eps = [0.01, 0.02, 0.015]
+    monkeypatch.setenv("DSGDPP_WAIT_LIMIT_MS", "0.00001")
+    with pytest.raises(RuntimeError, match="dataflow wait gave up"):
+        bt.epoch(eps)
+    monkeypatch.delenv("DSGDPP_WAIT_LIMIT_MS")
+    for k, (p, _, _) in enumerate(probs):
+        orcs[k].epoch(eps[k])                    # the oracle's streams pass the failed epoch too
+        ctxs[k].set_factors(p["W0"], p["H0"])
+        orcs[k].set_factors(p["W0"], p["H0"])
+        orcs[k].set_arith("FP32")                # factors as the GPU stores them
+    for e in range(2):
+        bt.epoch(eps)
+        for k, o in enumerate(orcs):
+            o.epoch(eps[k])
+            Wb, Hb = ctxs[k].get_factors()
+            Wo, Ho = o.get_factors()
+            np.testing.assert_array_equal(Wb, Wo, err_msg=f"problem {k} epoch {e}")
+            np.testing.assert_array_equal(Hb, Ho, err_msg=f"problem {k} epoch {e}")
+    bt.close()
+    for g in ctxs:
+        g.close()

@@ -39,16 +39,18 @@ def _losses(trace):
     return [(float(a), float(b)) for a, b in re.findall(r"it=\d+, epoch=\d+, loss=([^,]+), loss\.test=([^,]+),", trace)]
 
 
-def test_cli_matches_oracle_cli(tmp_path):
-    print("case: mfdsgdpp CLI vs oracle CLI, FP64_STRICT", flush=True)
+@pytest.mark.parametrize("kernel", [None, "1"])
+def test_cli_matches_oracle_cli(tmp_path, kernel):
+    """The CLI in a child process while this process holds contexts that ran
+    dataflow epochs: the default kernel (dataflow epochs are ordinary
+    launches, so processes share the device) and kernel 1."""
+    print(f"case: mfdsgdpp CLI vs oracle CLI, FP64_STRICT, kernel {kernel or 'default'}", flush=True)
     _write(tmp_path, 60, 80, 4, 2000, 300, seed=3)
     common = ["--input-file", str(tmp_path / "v.mmc"), "--input-test-file", str(tmp_path / "t.mmc"),
               "--input-row-file", str(tmp_path / "w.mma"), "--input-col-file", str(tmp_path / "h.mma"),
               "--update", "Nzsl_L2(0.1)", "--regularize", "None", "--loss", "Nzsl_L2(0.1)", "--rank", "4",
               "--decay", "BoldDriver(0.05)", "--epochs", "4", "--tasks-per-rank", "2", "--seed", "11"]
-    # kernel 1 (one launch per stratum): the dataflow kernels launch cooperatively, which the
-    # runtime serves to one process per device, and this test's own process may hold that
-    gpu_only = ["--kernel", "1"]
+    gpu_only = ["--kernel", kernel] if kernel else []
     outs = {}
     for tag, exe in (("gpu", CLI), ("cpu", ORACLE_CLI)):
         r = subprocess.run([exe] + common + (gpu_only if tag == "gpu" else []) +

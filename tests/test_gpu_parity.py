@@ -668,3 +668,53 @@ def test_const_decay_run_loop():
     np.testing.assert_array_equal(Wg, Wo)
     np.testing.assert_array_equal(Hg, Ho)
     g.close()
+
+
+def imbalanced_problem(rank, d=4, rows=100, cols=100, heavy=3000, light=12, seed=0):
+    """Row block 0's blocks hold `heavy` points each, the others `light`: the
+    other tasks reach their later sub-epochs while task 0 is still in its
+    first visit, so some of their dataflow waits are certainly unmet."""
+    rng = np.random.default_rng(seed)
+    i, j = [], []
+    for b1 in range(d):
+        for b2 in range(2 * d):
+            cells = rng.choice(rows * cols, size=heavy if b1 == 0 else light, replace=False)
+            i += list(b1 * rows + cells // cols)
+            j += list(b2 * cols + cells % cols)
+    m, n = d * rows, 2 * d * cols
+    return dict(m=m, n=n, rank=rank, train=(np.array(i, dtype=np.int32), np.array(j, dtype=np.int32),
+                                            rng.normal(size=len(i))),
+                W0=rng.uniform(-0.5, 0.5, (m, rank)), H0=rng.uniform(-0.5, 0.5, (rank, n)))
+
+
+@pytest.mark.parametrize("precision,kernel,arith", [("FP32", 3, "FP32"), ("BF16", 4, "BF16"),
+                                                    ("FP64", 3, "FP64_GROUPED"), ("FP64_STRICT", 4, None),
+                                                    ("FP64_STRICT", 2, None)])
+def test_dataflow_wait_limit_reports_instead_of_hanging(monkeypatch, precision, kernel, arith):
+    """Dataflow epochs are ordinary launches whose waits carry a time limit.
+    Forced from the host to one tick, an unmet wait gives up: every wave of
+    the launch leaves its task (tile mover and compute wave, FP64 tile,
+    streaming task, sequential task), the epoch reports the error, and the
+    next epoch is again bit-identical to the oracle from freshly set factors."""
+    print(f"case: wait limit one tick, {precision} kernel {kernel}", flush=True)
+    p = imbalanced_problem(16, seed=kernel)
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=4, seed=5, precision=precision, kernel=kernel)
+    if arith:
+        o.set_arith(arith)
+    monkeypatch.setenv("DSGDPP_WAIT_LIMIT_MS", "0.00001")
+    with pytest.raises(RuntimeError, match="dataflow wait gave up"):
+        g.epoch(0.01)
+    monkeypatch.delenv("DSGDPP_WAIT_LIMIT_MS")
+    o.epoch(0.01)                                 # the oracle's streams pass the failed epoch too
+    g.set_factors(p["W0"], p["H0"])
+    o.set_factors(p["W0"], p["H0"])
+    if arith in ("FP32", "BF16"):
+        o.set_arith(arith)                        # factors as the GPU stores them
+    for e in range(2):
+        assert g.epoch(0.01)["kernel"] == kernel
+        o.epoch(0.01)
+        Wg, Hg = g.get_factors()
+        Wo, Ho = o.get_factors()
+        np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
+    g.close()
