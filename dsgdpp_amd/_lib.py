@@ -26,7 +26,7 @@ EXPORTS = (
     "dsgdpp_create", "dsgdpp_destroy", "dsgdpp_last_error", "dsgdpp_create_error",
     "dsgdpp_load_coo", "dsgdpp_load_mmc", "dsgdpp_set_factors", "dsgdpp_get_factors",
     "dsgdpp_load_factors_mma", "dsgdpp_write_factors_mma", "dsgdpp_epoch", "dsgdpp_loss",
-    "dsgdpp_balance", "dsgdpp_run", "dsgdpp_get_schedule", "dsgdpp_get_task_seeds",
+    "dsgdpp_loss_noalloc", "dsgdpp_balance", "dsgdpp_run", "dsgdpp_get_schedule", "dsgdpp_get_task_seeds",
     "dsgdpp_get_permutation", "dsgdpp_block_nnz", "dsgdpp_dims", "dsgdpp_grouped_ok",
     "dsgdpp_mlp_num_params", "dsgdpp_mlp_create", "dsgdpp_mlp_create_error", "dsgdpp_mlp_last_error",
     "dsgdpp_mlp_destroy", "dsgdpp_mlp_set_params", "dsgdpp_mlp_get_params", "dsgdpp_mlp_forward",
@@ -119,6 +119,7 @@ def load():
         "dsgdpp_write_factors_mma": (i32, [P, ctypes.c_char_p, ctypes.c_char_p]),
         "dsgdpp_epoch": (i32, [P, dbl, ctypes.POINTER(EpochStats)]),
         "dsgdpp_loss": (i32, [P, i32, pd]),
+        "dsgdpp_loss_noalloc": (i32, [P, i32, P, P]),
         "dsgdpp_balance": (i32, [P, i32]),
         "dsgdpp_run": (i32, [P, ctypes.POINTER(RunCfg)]),
         "dsgdpp_get_schedule": (i32, [P, pu32]),

@@ -250,6 +250,13 @@ class DsgdPp:
         self._check(self._lib.dsgdpp_loss(self._h, which, ctypes.byref(out)))
         return out.value
 
+    def loss_noalloc(self, which, d_out, stream=0):
+        """dsgdpp_loss_noalloc: enqueue the loss on `stream` (a hipStream_t as
+        an integer) and write it to the device double at address `d_out`,
+        without allocating or waiting."""
+        self._check(self._lib.dsgdpp_loss_noalloc(self._h, which, ctypes.c_void_p(int(d_out)),
+                                                  ctypes.c_void_p(int(stream))))
+
     def balance(self, kind="None"):
         self._check(self._lib.dsgdpp_balance(self._h, L.BALANCE[kind] if isinstance(kind, str) else kind))
 

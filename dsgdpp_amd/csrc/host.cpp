@@ -1388,6 +1388,7 @@ int dsgdpp_create(const dsgdpp_cfg* cfg, dsgdpp_ctx** out) {
         dalloc(c->dStatus, sizeof(int32_t));
         HIPC(hipDeviceGetAttribute(&c->wallKhz, hipDeviceAttributeWallClockRate, c->device));
         if (c->wallKhz <= 0) c->wallKhz = 100000;
+        noalloc_scratch(c->device);   // the device's address of the dsgdpp_loss_noalloc scratch
     } catch (const std::exception& e) {
         g_createError = e.what();
         const int code = dynamic_cast<const Error*>(&e) ? static_cast<const Error&>(e).code : DSGDPP_E_DEVICE;
@@ -1540,6 +1541,23 @@ int dsgdpp_loss(dsgdpp_ctx* c, int32_t which, double* out) {
         if (which == DSGDPP_TRAIN) *out = trainLoss(c);
         else if (which == DSGDPP_TEST) *out = testLoss(c);
         else throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+    });
+}
+
+int dsgdpp_loss_noalloc(dsgdpp_ctx* c, int32_t which, double* d_out, void* stream) {
+    return guarded(c, [&] {
+        if (which != DSGDPP_TRAIN && which != DSGDPP_TEST) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+        if (!d_out) throw Error(DSGDPP_E_ARG, "null output pointer");
+        requireReady(c);
+        if (which == DSGDPP_TEST && !c->blocks[1].loaded) throw Error(DSGDPP_E_STATE, "no test matrix loaded");
+        double* sp = noalloc_scratch_lookup(c->device);
+        if (!sp) throw Error(DSGDPP_E_STATE, "the no-allocation scratch of this device is not set up");
+        const int64_t need = device_loss_noalloc(c->P, which, c->upd.kind, c->lossLambda, sp, d_out,
+                                                 (hipStream_t)stream);
+        if (need > 0)
+            throw Error(DSGDPP_E_UNSUPPORTED, "the loss reduction needs " + std::to_string(need) +
+                                                  " partial sums; the no-allocation scratch holds " +
+                                                  std::to_string(noalloc_partials()));
     });
 }
 

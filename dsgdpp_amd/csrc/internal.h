@@ -164,5 +164,16 @@ double device_sumsq(const DevProblem& P, int which_factor, const double* d_weigh
                     int64_t lo, int64_t hi, double* d_partials, double* h_pinned, hipStream_t s);
 void device_scale(const DevProblem& P, int which_factor, double f, hipStream_t s);
 int partials_capacity();
+// The loss without allocating: noalloc_scratch (called once per device, with
+// it current) resolves the device address of the fixed scratch array;
+// device_loss_noalloc enqueues the loss of device_nzsl / device_sumsq on `s`
+// with that scratch and writes the value to *out (device).  Returns 0, or the
+// partial sums needed when more than the array holds (nothing enqueued).
+size_t noalloc_scratch_bytes();
+int noalloc_partials();
+double* noalloc_scratch(int device);
+double* noalloc_scratch_lookup(int device);
+int64_t device_loss_noalloc(const DevProblem& P, int which, int kind, double lambda, double* sp, double* out,
+                            hipStream_t s);
 
 }  // namespace dsgdpp

@@ -9,9 +9,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
+#include "../../include/dsgdpp.h"
 #include "internal.h"
 #include "philox.h"
 #include "device.h"
@@ -441,6 +443,12 @@ __global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ 
     }
 }
 
+// NZSL + lambda (a + b) from the three totals, in the association of the
+// host's SumLoss (loss.h:99): one rounding per operation, no contraction.
+__global__ void k_loss_combine(const double* __restrict__ totals, double lambda, double* __restrict__ out) {
+    if (threadIdx.x == 0) *out = totals[0] + lambda * (totals[1] + totals[2]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_scale(T* A, int64_t rows, int r, int64_t ld,
                                                        double f) {
@@ -617,6 +625,87 @@ double device_sumsq(const DevProblem& P, int which_factor, const double* d_weigh
     case PREC_FP64: return sumsq_t<double>(P, which_factor, d_weights, lo, hi, d_partials, h_pinned, s);
     case PREC_FP32: return sumsq_t<float>(P, which_factor, d_weights, lo, hi, d_partials, h_pinned, s);
     default: return sumsq_t<bf16_t>(P, which_factor, d_weights, lo, hi, d_partials, h_pinned, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The loss without allocating (dsgdpp_loss_noalloc): the same kernels and
+// grids as device_nzsl / device_sumsq, with the partial sums and the totals in
+// `scratch`, a fixed array that every device holds for the life of the
+// process.  Host code never passes the array's name to a kernel (on the host
+// it is a stand-in, not a device address): noalloc_scratch resolves the
+// device's own address once, with that device current, and the launches take
+// that pointer.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_TOTALS = 4;
+static __device__ __align__(256) double scratch[RED_BLOCKS + NOALLOC_TOTALS];
+static_assert(RED_BLOCKS == DSGDPP_NOALLOC_PARTIALS && sizeof(scratch) == DSGDPP_NOALLOC_SCRATCH_BYTES,
+              "the no-allocation scratch is the size include/dsgdpp.h states");
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratchMu;
+static double* g_scratchPtr[MAX_DEVICES] = {};
+
+size_t noalloc_scratch_bytes() { return sizeof(scratch); }
+int noalloc_partials() { return RED_BLOCKS; }
+
+double* noalloc_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw std::invalid_argument("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        HIPK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchPtr[device] = (double*)p;
+    }
+    return g_scratchPtr[device];
+}
+
+double* noalloc_scratch_lookup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    return g_scratchPtr[device];
+}
+
+static int red_blocks(int64_t items) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (items + RED_THREADS - 1) / RED_THREADS));
+}
+
+template <typename T>
+static int64_t loss_noalloc_t(const DevProblem& P, int which, int kind, double lambda, double* sp, double* out,
+                              hipStream_t s) {
+    const void* pts = which == 0 ? P.pts : P.test_pts;
+    const int64_t nnz = which == 0 ? P.nnz : P.test_nnz;
+    const bool reg = which == 0 && kind != UPD_NZSL;
+    const int nb = red_blocks(nnz), wb = red_blocks(P.m), hb = red_blocks(P.n);
+    const int64_t need = std::max(nb, reg ? std::max(wb, hb) : 0);
+    if (need > RED_BLOCKS) return need;   // the array holds RED_BLOCKS partial sums
+    double* const totals = sp + RED_BLOCKS;
+    k_nzsl<T><<<nb, RED_THREADS, 0, s>>>((const typename PtOf<T>::type*)pts, nnz, (const T*)P.W, (const T*)P.H,
+                                         (int)P.r, P.ld, sp);
+    HIPK(hipGetLastError());
+    if (!reg) {
+        k_sum_partials<<<1, 64, 0, s>>>(sp, nb, out);
+        HIPK(hipGetLastError());
+        return 0;
+    }
+    k_sum_partials<<<1, 64, 0, s>>>(sp, nb, totals + 0);
+    const double* w1 = kind == UPD_NZSL_NZL2 ? P.cnt1 : nullptr;
+    const double* w2 = kind == UPD_NZSL_NZL2 ? P.cnt2 : nullptr;
+    k_sumsq<T><<<wb, RED_THREADS, 0, s>>>((const T*)P.W, 0, P.m, (int)P.r, P.ld, w1, sp);
+    k_sum_partials<<<1, 64, 0, s>>>(sp, wb, totals + 1);
+    k_sumsq<T><<<hb, RED_THREADS, 0, s>>>((const T*)P.H, 0, P.n, (int)P.r, P.ld, w2, sp);
+    k_sum_partials<<<1, 64, 0, s>>>(sp, hb, totals + 2);
+    k_loss_combine<<<1, 64, 0, s>>>(totals, lambda, out);
+    HIPK(hipGetLastError());
+    return 0;
+}
+
+int64_t device_loss_noalloc(const DevProblem& P, int which, int kind, double lambda, double* sp, double* out,
+                            hipStream_t s) {
+    switch (P.precision) {
+    case PREC_FP64_STRICT:
+    case PREC_FP64: return loss_noalloc_t<double>(P, which, kind, lambda, sp, out, s);
+    case PREC_FP32: return loss_noalloc_t<float>(P, which, kind, lambda, sp, out, s);
+    default: return loss_noalloc_t<bf16_t>(P, which, kind, lambda, sp, out, s);
     }
 }
 

@@ -132,6 +132,24 @@ int dsgdpp_epoch(dsgdpp_ctx* ctx, double eps, dsgdpp_epoch_stats* stats);
  * (mfdsgdpp.cc:60-66). */
 int dsgdpp_loss(dsgdpp_ctx* ctx, int32_t which, double* out);
 
+/* dsgdpp_loss without allocating or waiting.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc.
+ * Enqueues the loss of `which` on `stream` (a hipStream_t of the context's
+ * device, passed as void*; 0 = the null stream) with the kernels and grids of
+ * dsgdpp_loss, so the value is bit for bit dsgdpp_loss's, and writes it to
+ * the device double *d_out.  Returns before the value is ready; the
+ * context's earlier calls have completed (each is synchronous at return).
+ * The reduction's partial sums go to a fixed array in the library's device
+ * code, DSGDPP_NOALLOC_SCRATCH_BYTES per device, held until the process ends
+ * (its device address is resolved by dsgdpp_create): calls that overlap on
+ * one device race on it.  DSGDPP_E_UNSUPPORTED when the reduction would need
+ * more than DSGDPP_NOALLOC_PARTIALS partial sums (the reduction grid is
+ * capped at that many workgroups). */
+#define DSGDPP_NOALLOC_PARTIALS 2048
+#define DSGDPP_NOALLOC_SCRATCH_BYTES ((DSGDPP_NOALLOC_PARTIALS + 4) * 8)
+int dsgdpp_loss_noalloc(dsgdpp_ctx* ctx, int32_t which, double* d_out, void* stream);
+
 /* balance (balance_impl.cc:102-208, simple method) */
 int dsgdpp_balance(dsgdpp_ctx* ctx, int32_t type);
 
