@@ -19,6 +19,7 @@ SGD_ORDERS = {"SEQ": 0, "WR": 1, "WOR": 2}
 STRATUM_ORDERS = {"SEQ": 0, "RSEQ": 1, "WR": 2, "WOR": 3, "COWOR": 4}
 PRECISIONS = {"FP64_STRICT": 0, "FP64": 1, "FP32": 2, "BF16": 3}
 BALANCE = {"None": 0, "L2": 1, "NZL2": 2}
+ALGORITHMS = {"DSGDPP": 0, "DSGD": 1}
 TRAIN, TEST = 0, 1
 
 # every symbol include/dsgdpp.h declares
@@ -52,7 +53,8 @@ class Cfg(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("kernel", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("algorithm", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

@@ -10,6 +10,7 @@ Reference interface (uma-pi1/DSGDpp):
   * DsgdPpJob(...)  (mf/sgd/dsgdpp.h:39)
   * DsgdPpRunner(seed).run(job, loss, epochs, decay, trace, balance, test)
                     and .epoch(job, eps)           (mf/sgd/dsgdpp.h:82-134)
+  * DsgdJob / DsgdRunner, the same for DSGD        (mf/sgd/dsgd.h: d x d blocks)
   * Trace           (mf/trace.h:179-262)
 
 Every device operation goes through the C ABI (include/dsgdpp.h) of the
@@ -141,11 +142,12 @@ class DecayConstant:
 # device context
 # ---------------------------------------------------------------------------
 class DsgdPp:
-    """One DSGD++ problem on one GPU (a dsgdpp_ctx)."""
+    """One DSGD++ problem on one GPU (a dsgdpp_ctx); algorithm="DSGD" runs
+    DSGD instead (DsgdRunner, mf/sgd/dsgd_impl.h: d x d blocks, d sub-epochs)."""
 
     def __init__(self, update="Nzsl_L2(1)", regularize="None", loss="Nzsl_L2(1)", rank=10,
                  tasks_per_rank=1, world_size=1, sgd_order="WOR", stratum_order="COWOR",
-                 seed=0, precision="FP64_STRICT", truncate=None, device=0, kernel=0):
+                 seed=0, precision="FP64_STRICT", truncate=None, device=0, kernel=0, algorithm="DSGDPP"):
         self._lib = L.load()
         cfg = L.Cfg()
         cfg.abi_version = L.ABI_VERSION
@@ -161,6 +163,7 @@ class DsgdPp:
         cfg.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         cfg.device = int(device)
         cfg.kernel = int(kernel)
+        cfg.algorithm = L.ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
         h = ctypes.c_void_p()
         rc = self._lib.dsgdpp_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -169,7 +172,13 @@ class DsgdPp:
         self.rank = int(rank)
         self.d = int(world_size) * int(tasks_per_rank)
         self.precision = precision
+        self.algorithm = algorithm if isinstance(algorithm, str) else {0: "DSGDPP", 1: "DSGD"}[int(algorithm)]
         self.m = self.n = None
+
+    @property
+    def subepochs(self):
+        """Sub-epochs per epoch = column blocks: 2d (DSGD++) or d (DSGD)."""
+        return self.d if self.algorithm == "DSGD" else 2 * self.d
 
     def close(self):
         if getattr(self, "_h", None):
@@ -279,7 +288,7 @@ class DsgdPp:
 
     # parity hooks --------------------------------------------------------------
     def schedule(self):
-        out = np.empty((2 * self.d, self.d), dtype=np.uint32)
+        out = np.empty((self.subepochs, self.d), dtype=np.uint32)
         self._check(self._lib.dsgdpp_get_schedule(self._h, L.ptr(out, ctypes.c_uint32)))
         return out
 
@@ -297,7 +306,7 @@ class DsgdPp:
         return out[: n.value]
 
     def block_nnz(self, which=L.TRAIN):
-        out = np.empty((self.d, 2 * self.d), dtype=np.int64)
+        out = np.empty((self.d, self.subepochs), dtype=np.int64)
         self._check(self._lib.dsgdpp_block_nnz(self._h, which, L.ptr(out, ctypes.c_int64)))
         return out
 
@@ -313,6 +322,8 @@ class DsgdPp:
 class DsgdPpJob:
     """DsgdPpJob (mf/sgd/dsgdpp.h:39): data, factors, functors, orders, d = tasks per rank."""
 
+    algorithm = "DSGDPP"
+
     def __init__(self, m, n, train, W, H, update, regularize=None, sgd_order="WOR",
                  stratum_order="COWOR", tasks_per_rank=1, loss=None, test=None,
                  precision="FP64_STRICT", seed=0, kernel=0):
@@ -323,12 +334,19 @@ class DsgdPpJob:
         self.ctx = DsgdPp(update=update.spec(), regularize=self.regularize.spec(), loss=self.loss_spec,
                           rank=W.shape[1], tasks_per_rank=tasks_per_rank, sgd_order=sgd_order,
                           stratum_order=stratum_order, seed=seed, precision=precision,
-                          truncate=trunc, kernel=kernel)
+                          truncate=trunc, kernel=kernel, algorithm=self.algorithm)
         self.ctx.load_coo(L.TRAIN, m, n, *train)
         if test is not None:
             self.ctx.load_coo(L.TEST, m, n, *test)
         self.ctx.set_factors(W, H)
         self.has_test = test is not None
+
+
+class DsgdJob(DsgdPpJob):
+    """DsgdJob (mf/sgd/dsgd.h): the same job for DSGD -- d x d blocks, one d x d
+    schedule per epoch, d sub-epochs (dsgd_impl.h:81-234, 329-350)."""
+
+    algorithm = "DSGD"
 
 
 class TraceEntry:
@@ -425,3 +443,13 @@ class DsgdPpRunner:
             if on_epoch is not None:
                 on_epoch(ep + 1, cur, test, eps, stats)
         return trace
+
+
+class DsgdRunner(DsgdPpRunner):
+    """DsgdRunner (mf/sgd/dsgd.h): run and epoch as DsgdPpRunner's, on a DsgdJob
+    (the same defaultRunner loop, dsgd_impl.h:62-64)."""
+
+    def epoch(self, job, eps):
+        if job.algorithm != "DSGD":
+            raise TypeError("DsgdRunner runs a DsgdJob")
+        return job.ctx.epoch(eps)

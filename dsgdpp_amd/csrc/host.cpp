@@ -196,6 +196,12 @@ struct dsgdpp_ctx {
     int sgdOrder = DSGDPP_SGD_WOR, stratumOrder = DSGDPP_STRATUM_COWOR;
     uint64_t seed = 0;
     int precision = DSGDPP_FP64_STRICT, device = 0, kernel = 0;
+    // DSGD runs on the d x 2d machinery: its column block k is block 2k, every
+    // odd column block is empty (zero columns), and its sub-epoch s is
+    // sub-epoch 2s; the odd sub-epochs visit the empty blocks, which every
+    // kernel skips and which draw nothing.
+    int algorithm = DSGDPP_ALGO_DSGDPP;
+    bool dsgd() const { return algorithm == DSGDPP_ALGO_DSGD; }
     Stream master;
 
     int64_t m = -1, n = -1;
@@ -349,6 +355,9 @@ void configure(dsgdpp_ctx* c, const dsgdpp_cfg* cfg) {
     c->device = cfg->device;
     c->kernel = cfg->kernel;
     if (c->kernel < 0 || c->kernel > 4) throw Error(DSGDPP_E_ARG, "kernel must be 0..4");
+    if (cfg->algorithm != DSGDPP_ALGO_DSGDPP && cfg->algorithm != DSGDPP_ALGO_DSGD)
+        throw Error(DSGDPP_E_ARG, "algorithm must be DSGDPP_ALGO_DSGDPP or DSGDPP_ALGO_DSGD");
+    c->algorithm = cfg->algorithm;
     master_key(c->seed, &c->master.k0, &c->master.k1);
     c->master.next = 0;
 }
@@ -584,7 +593,9 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
     if (m >= INT32_MAX || n >= INT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "dimensions must fit in int32");
     if (nnz > 0 && (!I || !J || !X)) throw Error(DSGDPP_E_ARG, "null index or value array");
     if (c->m >= 0 && (c->m != m || c->n != n)) throw Error(DSGDPP_E_ARG, "matrix dimensions differ from the loaded data");
-    if (m < (int64_t)c->d || n < (int64_t)(2 * c->d)) throw Error(DSGDPP_E_ARG, "matrix is smaller than the d x 2d block grid");
+    if (m < (int64_t)c->d || n < (int64_t)(c->dsgd() ? c->d : 2 * c->d))
+        throw Error(DSGDPP_E_ARG, c->dsgd() ? "matrix is smaller than the d x d block grid"
+                                            : "matrix is smaller than the d x 2d block grid");
     for (int64_t p = 0; p < nnz; ++p)
         if (I[p] < 0 || I[p] >= m || J[p] < 0 || J[p] >= n)
             throw Error(DSGDPP_E_ARG, "index out of range at entry " + std::to_string(p));
@@ -595,7 +606,17 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
     c->m = m;
     c->n = n;
     c->off1 = defaultOffsets(m, c->d);
-    c->off2 = defaultOffsets(n, 2 * c->d);
+    if (c->dsgd()) {   // DSGD's d column blocks at the even positions, empty odd blocks between
+        const std::vector<int64_t> o = defaultOffsets(n, c->d);
+        c->off2.assign(2 * (size_t)c->d + 1, 0);
+        for (uint32_t k = 0; k < c->d; ++k) {
+            c->off2[2 * k] = o[k];
+            c->off2[2 * k + 1] = o[k + 1];
+        }
+        c->off2[2 * c->d] = n;
+    } else {
+        c->off2 = defaultOffsets(n, 2 * c->d);
+    }
     if (first) {
         c->P.m = m;
         c->P.n = n;
@@ -679,6 +700,23 @@ void requireReady(dsgdpp_ctx* c) {
 void drawPlan(dsgdpp_ctx* c, EpochPlan& P) {
     const uint32_t d = c->d, d2 = 2 * d;
     Stream& rng = c->master;
+    if (c->dsgd()) {
+        // DsgdRunner::epoch (dsgd_impl.h:329-350): one d x d schedule, then the
+        // task seeds; DSGD's sub-epoch s and column block k are sub-epoch 2s
+        // and block 2k here, and sub-epoch 2s + 1 visits the empty block 2k + 1
+        const std::vector<uint32_t> s0 = dsgdSchedule(c->w, c->t, c->stratumOrder, rng);
+        P.schedule.assign((size_t)d2 * d, 0);
+        for (uint32_t i = 0; i < d; ++i)
+            for (uint32_t id = 0; id < d; ++id) {
+                P.schedule[(size_t)(2 * i) * d + id] = 2 * s0[(size_t)i * d + id];
+                P.schedule[(size_t)(2 * i + 1) * d + id] = 2 * s0[(size_t)i * d + id] + 1;
+            }
+        P.seeds.resize(d);
+        for (uint32_t id = 0; id < d; ++id) P.seeds[id] = rng.u64();
+        P.drawn = true;
+        P.built = false;
+        return;
+    }
     // column-block halves for even / odd sub-epochs (:289-323)
     std::vector<uint32_t> even(d), odd(d);
     if (c->stratumOrder == DSGDPP_STRATUM_SEQ) {
@@ -745,6 +783,7 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
                 uint64_t draw = 0;
                 int64_t lastN = 0, lastBase = -1;
                 for (uint32_t sub = 0; sub < d2; ++sub) {
+                    if (c->dsgd() && (sub & 1)) continue;   // an empty column block no DSGD task visits
                     const size_t b = visitBlock((uint32_t)id, sub);
                     const int64_t nb = bs[b + 1] - bs[b];
                     const int64_t base = P.voff[(size_t)sub * d + id];
@@ -1574,7 +1613,12 @@ int dsgdpp_get_schedule(dsgdpp_ctx* c, uint32_t* out) {
         if (c->lastPlan < 0) throw Error(DSGDPP_E_STATE, "no epoch has run");
         if (!out) throw Error(DSGDPP_E_ARG, "null output");
         const auto& sch = c->plan[c->lastPlan].schedule;
-        std::copy(sch.begin(), sch.end(), out);
+        if (c->dsgd()) {   // d x d in DSGD's own numbering
+            for (size_t i = 0; i < c->d; ++i)
+                for (size_t id = 0; id < c->d; ++id) out[i * c->d + id] = sch[(2 * i) * c->d + id] / 2;
+        } else {
+            std::copy(sch.begin(), sch.end(), out);
+        }
     });
 }
 
@@ -1591,8 +1635,9 @@ int dsgdpp_get_permutation(dsgdpp_ctx* c, int32_t task, int32_t subepoch, uint32
                            int64_t* n) {
     return guarded(c, [&] {
         if (c->lastPlan < 0) throw Error(DSGDPP_E_STATE, "no epoch has run");
-        if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)(2 * c->d))
+        if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)((c->dsgd() ? 1 : 2) * c->d))
             throw Error(DSGDPP_E_ARG, "task or sub-epoch out of range");
+        if (c->dsgd()) subepoch *= 2;
         const EpochPlan& Pl = c->plan[c->lastPlan];
         const size_t b = (size_t)task * 2 * c->d + Pl.schedule[(size_t)subepoch * c->d + task];
         const int64_t nb = c->blocks[0].start[b + 1] - c->blocks[0].start[b];
@@ -1614,7 +1659,15 @@ int dsgdpp_block_nnz(dsgdpp_ctx* c, int32_t which, int64_t* out) {
         if (which != 0 && which != 1) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
         if (!c->blocks[which].loaded) throw Error(DSGDPP_E_STATE, "matrix not loaded");
         const auto& s = c->blocks[which].start;
-        for (size_t b = 0; b + 1 < s.size(); ++b) out[b] = s[b + 1] - s[b];
+        if (c->dsgd()) {   // d x d: DSGD's block k is column block 2k
+            for (size_t b1 = 0; b1 < c->d; ++b1)
+                for (size_t k = 0; k < c->d; ++k) {
+                    const size_t b = b1 * 2 * c->d + 2 * k;
+                    out[b1 * c->d + k] = s[b + 1] - s[b];
+                }
+        } else {
+            for (size_t b = 0; b + 1 < s.size(); ++b) out[b] = s[b + 1] - s[b];
+        }
     });
 }
 

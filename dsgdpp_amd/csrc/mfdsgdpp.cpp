@@ -1,7 +1,10 @@
-// mfdsgdpp -- command line of the GPU DSGD++ path, option-compatible with the
-// reference tool (tools/mfdsgdpp.cc:379-441; positional arguments :406-412;
-// exit code 0/1 :565).  Extra options: --precision (FP64_STRICT|FP64|FP32|BF16),
-// --device, --kernel.  Everything runs through the C ABI in include/dsgdpp.h.
+// mfdsgdpp / mfdsgd -- command lines of the GPU DSGD++ and DSGD paths,
+// option-compatible with the reference tools (tools/mfdsgdpp.cc:379-441,
+// positional arguments :406-412, exit code 0/1 :565; tools/mfdsgd.cc:50-233,
+// which adds --map-reduce).  The program name selects the algorithm: invoked
+// as "mfdsgd" it runs DSGD (d x d blocks).  Extra options: --precision
+// (FP64_STRICT|FP64|FP32|BF16), --device, --kernel.  Everything runs through
+// the C ABI in include/dsgdpp.h.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,11 +26,11 @@ int indexOf(const std::string& s, std::initializer_list<const char*> names) {
     return -1;
 }
 
-void usage() {
+void usage(bool dsgd) {
     fprintf(stderr,
             "Error: Options input-file, input-row-file, input-col-file, update, regularize, loss, rank, "
             "and decay have to be present\n"
-            "mfdsgdpp [options]\n\n"
+            "%s [options]\n\n"
             "Options:\n"
             "  --help                produce help message\n"
             "  --input-file          filename of data matrix\n"
@@ -47,19 +50,31 @@ void usage() {
             "  --update              SGD update function (Nzsl, Nzsl_L2(l), Nzsl_Nzl2(l))\n"
             "  --regularize          SGD regularization function (None)\n"
             "  --loss                loss function (Nzsl, Nzsl_L2(l), Nzsl_Nzl2(l))\n"
-            "  --abs                 absolute values after every step (rejected, as in the reference)\n"
+            "  --abs                 absolute values after every step (%s)\n"
             "  --truncate            truncation, e.g. --truncate \"(-1000,1000)\"\n"
             "  --decay               decay function (Const(e), BoldDriver(e), BoldDriver(e,dec,inc))\n"
             "  --balance             type of balancing (None, L2, NZL2)\n"
             "  --precision           FP64_STRICT (default), FP64, FP32 or BF16\n"
             "  --device              HIP device ordinal [0]\n"
             "  --kernel              epoch kernel: 0 auto, 1 one launch per stratum, 2 dataflow,\n"
-            "                        3 grouped, 4 grouped with H streamed from HBM\n");
+            "                        3 grouped, 4 grouped with H streamed from HBM\n"
+            "%s",
+            dsgd ? "mfdsgd" : "mfdsgdpp",
+            dsgd ? "not supported by the GPU path" : "rejected, as in the reference",
+            dsgd ? "  --map-reduce          whether to use the (slower) MapReduce implementation [false]\n"
+                   "                        (on one GPU both give the same updates)\n"
+                 : "");
+}
+
+bool endsWith(const std::string& s, const std::string& t) {
+    return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
+    const std::string self = argc > 0 ? argv[0] : "";
+    const bool dsgd = endsWith(self, "/mfdsgd") || self == "mfdsgd";
     std::map<std::string, std::string> opt;
     std::vector<std::string> pos;
     bool help = false, absFlag = false;
@@ -87,7 +102,7 @@ int main(int argc, char** argv) {
             fprintf(stderr, "Error: the option '--%s' requires an argument\n", k.c_str());
             return 1;
         }
-        bool ok = false;
+        bool ok = dsgd && k == "map-reduce";
         for (const char* n : known) ok = ok || k == n;
         if (!ok) {
             fprintf(stderr, "Error: unrecognised option '--%s'\n", k.c_str());
@@ -102,7 +117,12 @@ int main(int argc, char** argv) {
                           "rank", "decay"})
         if (!opt.count(r)) help = true;
     if (help) {
-        usage();
+        usage(dsgd);
+        return 1;
+    }
+    if (opt.count("map-reduce") && opt["map-reduce"] != "true" && opt["map-reduce"] != "false" &&
+        opt["map-reduce"] != "1" && opt["map-reduce"] != "0") {
+        fprintf(stderr, "Error: the argument ('%s') for option '--map-reduce' is invalid\n", opt["map-reduce"].c_str());
         return 1;
     }
     auto get = [&](const char* k, const char* def) { return opt.count(k) ? opt[k] : std::string(def); };
@@ -114,7 +134,9 @@ int main(int argc, char** argv) {
     if (str < 0) { fprintf(stderr, "Invalid arguments for stratumOrder; expected \"SEQ\", \"RSEQ\", \"WR\", \"WOR\" or \"COWOR\"\n"); return 1; }
     if (bal < 0) { fprintf(stderr, "Invalid arguments for balance; expected \"None\", \"L2\" or \"Nzl2\"\n"); return 1; }
     if (prec < 0) { fprintf(stderr, "Invalid arguments for precision\n"); return 1; }
-    if (absFlag) {   // runArgs accepts no --abs combination (mfdsgdpp.cc:165, :353)
+    // mfdsgdpp's runArgs accepts no --abs combination (mfdsgdpp.cc:165, :353);
+    // mfdsgd's generated combinations do, but not on this GPU path
+    if (absFlag) {
         fprintf(stderr, "Invalid combination of update, regularize, and loss arguments\n");
         return 1;
     }
@@ -136,7 +158,9 @@ int main(int argc, char** argv) {
     cfg.precision = prec;
     cfg.device = atoi(get("device", "0").c_str());
     cfg.kernel = atoi(get("kernel", "0").c_str());
-    printf("DSGD++ options\n    Seed: %llu\n    Epochs: %s\n    Factorization rank: %s\n"
+    cfg.algorithm = dsgd ? DSGDPP_ALGO_DSGD : DSGDPP_ALGO_DSGDPP;
+    printf(dsgd ? "DSGD options\n" : "DSGD++ options\n");
+    printf("    Seed: %llu\n    Epochs: %s\n    Factorization rank: %s\n"
            "    Update function: %s\n    Regularize function: %s\n    Loss function: %s\n    Decay: %s\n"
            "    Tasks per rank: %d\n    Precision: %s\n",
            (unsigned long long)seed, get("epochs", "10").c_str(), opt["rank"].c_str(), upd.c_str(), reg.c_str(),

@@ -48,6 +48,10 @@ extern "C" {
 #define DSGDPP_FP64 1        /* FP64, lane-parallel dot */
 #define DSGDPP_FP32 2        /* FP32 factors and update math */
 #define DSGDPP_BF16 3        /* BF16 factor storage, FP32 update math */
+/* algorithm: DSGD++ (d x 2d blocks, mf/sgd/dsgdpp.h) or DSGD (d x d blocks,
+ * mf/sgd/dsgd.h; the mfdsgd tool) */
+#define DSGDPP_ALGO_DSGDPP 0
+#define DSGDPP_ALGO_DSGD 1
 /* matrices */
 #define DSGDPP_TRAIN 0
 #define DSGDPP_TEST 1
@@ -82,7 +86,10 @@ typedef struct dsgdpp_cfg {
                                     chosen by 0 and 3 when a column block is too wide for the LDS
                                     tile).  Blocks whose prep state exceeds LDS keep it in global
                                     scratch (any grouped kernel). */
-    int32_t reserved[5];
+    int32_t algorithm;       /* DSGDPP_ALGO_DSGDPP (0, default) or DSGDPP_ALGO_DSGD: DsgdRunner
+                                (mf/sgd/dsgd_impl.h:81-234, 329-350) -- d x d blocks, d sub-epochs,
+                                one computeDsgdSchedule per epoch, on the same kernels */
+    int32_t reserved[4];
 } dsgdpp_cfg;
 
 typedef struct dsgdpp_epoch_stats {
@@ -174,9 +181,9 @@ typedef struct dsgdpp_run_cfg {
 int dsgdpp_run(dsgdpp_ctx* ctx, const dsgdpp_run_cfg* run);
 
 /* Parity hooks: the last epoch's 2d x d schedule (row = sub-epoch, column =
- * task, entry = column block; dsgdpp_impl.h:330-336), the seeds handed to the
- * d tasks (:344), and one block's visit order (block-local point indices in
- * stored order; sgd_impl.h:255-351). */
+ * task, entry = column block; dsgdpp_impl.h:330-336; DSGD: d x d,
+ * dsgd_impl.h:337), the seeds handed to the d tasks (:344), and one block's
+ * visit order (block-local point indices in stored order; sgd_impl.h:255-351). */
 int dsgdpp_get_schedule(dsgdpp_ctx* ctx, uint32_t* out);
 int dsgdpp_get_task_seeds(dsgdpp_ctx* ctx, uint64_t* out);
 int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint32_t* out,
@@ -188,7 +195,7 @@ int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint
  * ("two-slot groups in K of N blocks", or empty).  No counterpart in the
  * reference (a kernel-choice query). */
 int dsgdpp_grouped_ok(dsgdpp_ctx* ctx, int32_t* ok, char* why, int64_t cap);
-/* nnz per block (d x 2d, row-major) and dims {m, n, r, d, nnz_train, nnz_test} */
+/* nnz per block (d x 2d, row-major; DSGD: d x d) and dims {m, n, r, d, nnz_train, nnz_test} */
 int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);
 int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
 

@@ -300,6 +300,9 @@ struct Ctx {
     std::string updateStr, regularizeStr, lossStr;
     int64_t r = 0;
     uint32_t w = 1, t = 1, d = 1;
+    // algorithm: 0 DSGD++ (d x 2d blocks, 2d sub-epochs), 1 DSGD (d x d, d sub-epochs)
+    int algo = 0;
+    uint32_t nb2 = 2;   // column blocks
     int sgdOrder = 2, stratumOrder = 4;
     uint64_t seed = 0;
     int threads = 1;
@@ -325,7 +328,7 @@ struct Ctx {
 
 static void setErr(Ctx* c, const std::string& e) { c->err = e; }
 
-static int blockIndex(const Ctx* c, int64_t b1, int64_t b2) { return (int)(b1 * 2 * c->d + b2); }
+static int blockIndex(const Ctx* c, int64_t b1, int64_t b2) { return (int)(b1 * c->nb2 + b2); }
 
 // ReadMatrixBlocksMm (read_impl.h:365-486) + block sort (read_impl.h:163):
 // entries go to the block of (i,j) with block-local indices; each block is then
@@ -333,7 +336,7 @@ static int blockIndex(const Ctx* c, int64_t b1, int64_t b2) { return (int)(b1 * 
 // sort is stable and duplicates are summed in input order.
 static void blockCoo(Ctx* c, int64_t m, int64_t n, int64_t nnz, const int32_t* I,
                      const int32_t* J, const double* X, std::vector<Block>& out) {
-    uint32_t d = c->d, d2 = 2 * d;
+    uint32_t d = c->d, d2 = c->nb2;
     std::vector<uint32_t> rb(m), cb(n);
     for (uint32_t b = 0; b < d; ++b)
         for (int64_t i = c->off1[b]; i < c->off1[b + 1]; ++i) rb[i] = b;
@@ -382,7 +385,7 @@ static void computeNnz12(Ctx* c) {
     c->nnz2.assign(c->n, 0);
     c->nnzTrain = 0;
     for (uint32_t b1 = 0; b1 < c->d; ++b1)
-        for (uint32_t b2 = 0; b2 < 2 * c->d; ++b2) {
+        for (uint32_t b2 = 0; b2 < c->nb2; ++b2) {
             const Block& B = c->train[blockIndex(c, b1, b2)];
             for (size_t p = 0; p < B.i.size(); ++p) {
                 c->nnz1[c->off1[b1] + B.i[p]]++;
@@ -611,18 +614,30 @@ static void runBlock(Ctx* c, TaskRunner& tr, uint32_t b1, uint32_t b2, double ep
 
 // DsgdPpRunner::epoch (dsgdpp_impl.h:282-349) + DsgdPpTask::run (:78-276) on
 // one node: d tasks, 2d sub-epochs, barrier after each sub-epoch (:259-261).
+// DSGD: DsgdRunner::epoch (dsgd_impl.h:329-350) + DsgdTask::run (:81-234):
+// the d x d schedule of computeDsgdSchedule, then one seed per task in task
+// order (mpi2::seed, :344), d sub-epochs, task id on block (id, schedule(s, id))
+// (:111-112), one SgdRunner per task per epoch (:106), barrier after each
+// sub-epoch (:221-225).
 static double epoch(Ctx* c, double eps) {
     auto t0 = std::chrono::steady_clock::now();
-    c->schedule = dsgdppSchedule(c->w, c->t, c->stratumOrder, c->master, c->seeds);
     const uint32_t d = c->d;
+    if (c->algo == 1) {
+        c->schedule = dsgdSchedule(c->w, c->t, c->stratumOrder, c->master);
+        c->seeds.resize(d);
+        for (uint32_t id = 0; id < d; ++id) c->seeds[id] = c->master.next64();
+    } else {
+        c->schedule = dsgdppSchedule(c->w, c->t, c->stratumOrder, c->master, c->seeds);
+    }
+    const uint32_t subs = c->schedule.rows;
     std::vector<TaskRunner> runners(d);
     for (uint32_t id = 0; id < d; ++id) runners[id].rng = Rng::task(c->seeds[id]);
-    if (c->record) c->recPerm.assign((size_t)d * 2 * d, std::vector<uint32_t>());
+    if (c->record) c->recPerm.assign((size_t)d * subs, std::vector<uint32_t>());
     int th = std::max(1, std::min<int>(c->threads, (int)d));
     auto subepochs = [&](int k, Barrier* bar) {
-        for (uint32_t s = 0; s < 2 * d; ++s) {
+        for (uint32_t s = 0; s < subs; ++s) {
             for (uint32_t id = k; id < d; id += th) {
-                std::vector<uint32_t>* rec = c->record ? &c->recPerm[(size_t)id * 2 * d + s] : nullptr;
+                std::vector<uint32_t>* rec = c->record ? &c->recPerm[(size_t)id * subs + s] : nullptr;
                 runBlock(c, runners[id], id, c->schedule(s, id), eps, rec);
             }
             if (bar) bar->wait();
@@ -645,7 +660,7 @@ static double epoch(Ctx* c, double eps) {
 // ----------------------------------------------------------------------------
 // nzsl (nzsl.h:30-53) per block, block partials summed row-major (sum.h:28-30)
 static double nzsl(const Ctx* c, const std::vector<Block>& blocks) {
-    const uint32_t d = c->d, d2 = 2 * d;
+    const uint32_t d = c->d, d2 = c->nb2;
     const int64_t r = c->r;
     std::vector<double> part((size_t)d * d2, 0.0);
     auto work = [&](size_t lo, size_t hi) {
@@ -926,6 +941,7 @@ orc_ctx* orc_create(const char* update, const char* regularize, const char* loss
     if (sgd_order < 0 || sgd_order > 2) return fail("Invalid sgd order");
     if (stratum_order < 0 || stratum_order > 4) return fail("Invalid stratum order");
     c->r = rank; c->w = world; c->t = tasks_per_rank; c->d = world * tasks_per_rank;
+    c->nb2 = 2 * c->d;
     c->sgdOrder = sgd_order; c->stratumOrder = stratum_order; c->seed = seed;
     c->threads = threads > 0 ? threads : 1;
     c->master = Rng::master(seed);
@@ -939,12 +955,12 @@ int orc_load_coo(orc_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const
                  const int32_t* j, const double* x) {
     if (which != 0 && which != 1) { setErr(c, "which must be 0 or 1"); return -1; }
     if (c->m >= 0 && (c->m != m || c->n != n)) { setErr(c, "dimension mismatch"); return -1; }
-    if (m < (int64_t)c->d || n < (int64_t)(2 * c->d)) { setErr(c, "matrix smaller than the block grid"); return -1; }
+    if (m < (int64_t)c->d || n < (int64_t)c->nb2) { setErr(c, "matrix smaller than the block grid"); return -1; }
     for (int64_t p = 0; p < nnz; ++p)
         if (i[p] < 0 || i[p] >= m || j[p] < 0 || j[p] >= n) { setErr(c, "index out of range"); return -1; }
     c->m = m; c->n = n;
     c->off1 = blockOffsets(m, c->d);
-    c->off2 = blockOffsets(n, 2 * c->d);
+    c->off2 = blockOffsets(n, c->nb2);
     if (which == 0) {
         blockCoo(c, m, n, nnz, i, j, x, c->train);
         c->haveTrain = true;
@@ -1037,6 +1053,14 @@ int orc_get_schedule(orc_ctx* c, uint32_t* out) {
 
 int orc_set_record(orc_ctx* c, int on) { c->record = on != 0; return 0; }
 
+int orc_set_algorithm(orc_ctx* c, int algo) {
+    if (algo != 0 && algo != 1) { setErr(c, "algorithm must be 0 (DSGD++) or 1 (DSGD)"); return -1; }
+    if (c->m >= 0) { setErr(c, "set the algorithm before loading data"); return -1; }
+    c->algo = algo;
+    c->nb2 = algo == 1 ? c->d : 2 * c->d;
+    return 0;
+}
+
 int orc_set_arith(orc_ctx* c, int mode) {
     const int mutant = mode >> 4;
     mode &= 15;
@@ -1060,10 +1084,10 @@ int orc_set_arith(orc_ctx* c, int mode) {
 
 int orc_get_permutation(orc_ctx* c, int task, int subepoch, uint32_t* out, int64_t cap, int64_t* n) {
     if (!c->record || c->recPerm.empty()) { setErr(c, "recording off"); return -1; }
-    if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)(2 * c->d)) {
+    if (task < 0 || task >= (int)c->d || subepoch < 0 || subepoch >= (int)c->schedule.rows) {
         setErr(c, "task/subepoch out of range"); return -1;
     }
-    const auto& p = c->recPerm[(size_t)task * 2 * c->d + subepoch];
+    const auto& p = c->recPerm[(size_t)task * c->schedule.rows + subepoch];
     *n = (int64_t)p.size();
     if (out) {
         if (cap < (int64_t)p.size()) { setErr(c, "buffer too small"); return -1; }

@@ -28,7 +28,7 @@ int main(int argc, char** argv) {
                                   "output-row-file", "output-col-file", "trace", "trace-var", "epochs",
                                   "tasks-per-rank", "sgd-order", "stratum-order", "seed", "rank",
                                   "update", "regularize", "loss", "truncate", "decay", "balance",
-                                  "threads"};
+                                  "threads", "algorithm", "map-reduce"};
     bool help = false, absFlag = false;
     for (int a = 1; a < argc; ++a) {
         std::string s = argv[a];
@@ -83,6 +83,13 @@ int main(int argc, char** argv) {
                             atoll(opt["rank"].c_str()), 1, tasks, sgd, str, seed, threads, err,
                             sizeof err);
     if (!c) { fprintf(stderr, "%s\n", err); return 1; }
+    // --algorithm DSGD: the mfdsgd tool (tools/mfdsgd.cc: d x d blocks)
+    if (opt.count("algorithm") && opt["algorithm"] != "DSGDPP" &&
+        (opt["algorithm"] != "DSGD" || orc_set_algorithm(c, 1) != 0)) {
+        fprintf(stderr, "Invalid arguments for algorithm; expected \"DSGDPP\" or \"DSGD\"\n");
+        orc_destroy(c);
+        return 1;
+    }
     int rc = orc_load_mmc(c, 0, opt["input-file"].c_str());
     if (rc == 0 && opt.count("input-test-file") && !opt["input-test-file"].empty())
         rc = orc_load_mmc(c, 1, opt["input-test-file"].c_str());

@@ -54,6 +54,10 @@ int orc_balance(orc_ctx* c, int type);
 /* parity hooks (last epoch) */
 int orc_get_schedule(orc_ctx* c, uint32_t* out /* 2d x d row-major */);
 int orc_set_record(orc_ctx* c, int on);
+/* 0 DSGD++ (d x 2d blocks, 2d sub-epochs; the default), 1 DSGD (d x d blocks,
+ * d sub-epochs: DsgdRunner, mf/sgd/dsgd_impl.h:81-234, 329-350); before
+ * loading data */
+int orc_set_algorithm(orc_ctx* c, int algo);
 /* 0: FP64 reference arithmetic; 1/2/3: emulate the GPU grouped FP32/BF16/FP64
  * update (call after loading data and factors).  mode | (k << 4) plants
  * mistake k of orc_update_point_arith in the emulation (pin tests only). */

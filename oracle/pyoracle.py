@@ -15,6 +15,7 @@ CLI = os.path.join(HERE, "mfdsgdpp_cpu")
 SGD = {"SEQ": 0, "WR": 1, "WOR": 2}
 STRATUM = {"SEQ": 0, "RSEQ": 1, "WR": 2, "WOR": 3, "COWOR": 4}
 ARITH = {"FP64": 0, "FP32": 1, "BF16": 2, "FP64_GROUPED": 3}
+ALGORITHMS = {"DSGDPP": 0, "DSGD": 1}
 
 _lib = None
 
@@ -48,6 +49,7 @@ def lib():
             "orc_get_schedule": (i32, [P, pu32]),
             "orc_set_record": (i32, [P, i32]),
             "orc_set_arith": (i32, [P, i32]),
+            "orc_set_algorithm": (i32, [P, i32]),
             "orc_get_permutation": (i32, [P, i32, i32, pu32, i64, pi64]),
             "orc_get_task_seeds": (i32, [P, pu64]),
             "orc_block_nnz": (i32, [P, i32, pi64]),
@@ -79,7 +81,7 @@ def _p(a, t):
 class Oracle:
     def __init__(self, update="Nzsl_L2(1)", regularize="None", loss="Nzsl_L2(1)", truncate=None,
                  rank=10, world_size=1, tasks_per_rank=1, sgd_order="WOR", stratum_order="COWOR",
-                 seed=0, threads=1):
+                 seed=0, threads=1, algorithm="DSGDPP"):
         L = lib()
         err = ctypes.create_string_buffer(512)
         self._h = L.orc_create(update.encode(), regularize.encode(), loss.encode(),
@@ -89,6 +91,9 @@ class Oracle:
                                int(seed) & ((1 << 64) - 1), int(threads), err, 512)
         if not self._h:
             raise ValueError(err.value.decode())
+        self.algorithm = algorithm
+        if algorithm != "DSGDPP":
+            self._check(L.orc_set_algorithm(self._h, ALGORITHMS[algorithm]))
         self.rank = int(rank)
         self.d = int(world_size) * int(tasks_per_rank)
         self.m = self.n = None
@@ -150,8 +155,13 @@ class Oracle:
     def record(self, on=True):
         self._check(lib().orc_set_record(self._h, 1 if on else 0))
 
+    @property
+    def subepochs(self):
+        """Sub-epochs per epoch = column blocks: 2d (DSGD++) or d (DSGD)."""
+        return self.d if self.algorithm == "DSGD" else 2 * self.d
+
     def schedule(self):
-        out = np.empty((2 * self.d, self.d), dtype=np.uint32)
+        out = np.empty((self.subepochs, self.d), dtype=np.uint32)
         self._check(lib().orc_get_schedule(self._h, _p(out, ctypes.c_uint32)))
         return out
 
@@ -169,7 +179,7 @@ class Oracle:
         return out[: n.value]
 
     def block_nnz(self, which=0):
-        out = np.empty((self.d, 2 * self.d), dtype=np.int64)
+        out = np.empty((self.d, self.subepochs), dtype=np.int64)
         self._check(lib().orc_block_nnz(self._h, which, _p(out, ctypes.c_int64)))
         return out
 

@@ -128,3 +128,36 @@ def test_python_mirror_specs():
     b = D.BoldDriver(1.0)
     assert b(None, 5.0) == 1.0 and b(5.0, 4.0) == 1.05 and b(4.0, 4.0) == 0.525
     assert D.UpdateTruncate(D.UpdateNzsl(), -1, 2).truncate() == "(-1,2)"
+
+
+def test_mfdsgd_cli_options():
+    """bin/mfdsgd is the DSGD tool (tools/mfdsgd.cc:50-233): its usage names
+    it and --map-reduce; mfdsgdpp rejects --map-reduce; the oracle CLI's
+    --algorithm DSGD runs the d x d restatement."""
+    dsgd = os.path.join(ROOT, "dsgdpp_amd", "bin", "mfdsgd")
+    r = subprocess.run([dsgd], capture_output=True, text=True)
+    assert r.returncode == 1 and "mfdsgd [options]" in r.stderr and "--map-reduce" in r.stderr
+    base = ["--input-file", "a", "--input-row-file", "b", "--input-col-file", "c", "--update", "Nzsl",
+            "--regularize", "None", "--loss", "Nzsl", "--rank", "2", "--decay", "Const(1)"]
+    r = subprocess.run([dsgd] + base + ["--map-reduce", "maybe"], capture_output=True, text=True)
+    assert r.returncode == 1 and "map-reduce" in r.stderr
+    r = subprocess.run([CLI] + base + ["--map-reduce", "true"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unrecognised option '--map-reduce'" in r.stderr
+
+
+def test_oracle_cli_dsgd(tmp_path):
+    _write_mm(tmp_path, 10, 12, 3)
+    args = ["--input-file", str(tmp_path / "v.mmc"), "--input-row-file", str(tmp_path / "w.mma"),
+            "--input-col-file", str(tmp_path / "h.mma"), "--update", "Nzsl_L2(0.1)", "--regularize", "None",
+            "--loss", "Nzsl_L2(0.1)", "--rank", "3", "--decay", "BoldDriver(0.05)", "--epochs", "2",
+            "--tasks-per-rank", "3", "--seed", "3"]
+    r = subprocess.run([ORACLE_CLI] + args + ["--algorithm", "DSGD", "--output-row-file", str(tmp_path / "a.mma")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r2 = subprocess.run([ORACLE_CLI] + args + ["--output-row-file", str(tmp_path / "b.mma")],
+                        capture_output=True, text=True)
+    assert r2.returncode == 0, r2.stderr
+    # d x d and d x 2d blocking visit the points in other orders
+    assert (tmp_path / "a.mma").read_text() != (tmp_path / "b.mma").read_text()
+    r = subprocess.run([ORACLE_CLI] + args + ["--algorithm", "PSGD"], capture_output=True, text=True)
+    assert r.returncode == 1 and "algorithm" in r.stderr

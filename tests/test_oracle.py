@@ -484,3 +484,84 @@ def test_balance_in_run_loop_after_every_epoch():
     assert (W ** 2).sum() == pytest.approx((H ** 2).sum(), rel=1e-12)
     nz, rw, rh = _np_losses(p, W, H)
     assert r["loss"][-1] == pytest.approx(nz + 0.05 * (rw + rh), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# DSGD (DsgdRunner::epoch, dsgd_impl.h:329-350; DsgdTask::run, :81-234):
+# d x d blocks, the d x d schedule, one seed per task, d sub-epochs.  Pinned by
+# a brute-force epoch assembled from the separately pinned pieces (master
+# stream draws, block offsets, permute with carry-over, the reference update).
+# ---------------------------------------------------------------------------
+def _dsgd_brute_force(p, d, seed, eps, lam, order="WOR"):
+    m, n = p["m"], p["n"]
+    i, j, x = p["train"]
+    W, H = p["W0"].copy(), p["H0"].copy()
+    n1 = np.bincount(i, minlength=m).astype(float)
+    n2 = np.bincount(j, minlength=n).astype(float)
+    off1, off2 = po.block_offsets(m, d), po.block_offsets(n, d)
+    seeds = po.rng_draws(seed, 0, 0, d)          # SEQ strata draw nothing; then one seed per task
+    draw = [0] * d
+    last = [None] * d
+    for s in range(d):
+        for t in range(d):
+            b2 = (s + t) % d                     # SEQ schedule (dsgd_impl.h:253)
+            sel = [(a, b, v) for a, b, v in zip(i, j, x)
+                   if off1[t] <= a < off1[t + 1] and off2[b2] <= b < off2[b2 + 1]]
+            sel.sort(key=lambda q: (q[0], q[1]))  # blocks sorted row-major (read_impl.h:163)
+            k = len(sel)
+            if k == 0:
+                continue
+            if order == "WOR":
+                start = last[t] if last[t] is not None and len(last[t]) == k else None
+                perm = po.permute(int(seeds[t]), draw[t], k, start)
+                draw[t] += k
+                last[t] = perm
+            else:
+                perm = range(k)
+            for q in perm:
+                a, b, v = sel[q]
+                w, h = po.update_point("Nzsl_L2", lam, W[a], H[:, b], v, eps, n1[a], n2[b])
+                W[a], H[:, b] = w, h
+    return W, H
+
+
+@pytest.mark.parametrize("d,order", [(3, "WOR"), (4, "WOR"), (2, "SEQ")])
+def test_dsgd_epoch_brute_force(d, order):
+    p = random_problem(13, 11, 80, 3, seed=40 + d)
+    o = _oracle(p, "Nzsl_L2(0.4)", tasks=d, sgd_order=order, stratum_order="SEQ", seed=77, algorithm="DSGD")
+    o.record(True)
+    o.epoch(0.05)
+    W, H = _dsgd_brute_force(p, d, 77, 0.05, 0.4, order)
+    Wo, Ho = o.get_factors()
+    np.testing.assert_array_equal(Wo, W)
+    np.testing.assert_array_equal(Ho, H)
+    assert o.schedule().shape == (d, d)
+    assert (o.schedule() == (np.arange(d)[:, None] + np.arange(d)[None, :]) % d).all()
+    assert o.block_nnz().shape == (d, d) and o.block_nnz().sum() == 80
+
+
+@pytest.mark.parametrize("order", ["SEQ", "RSEQ", "WR", "WOR", "COWOR"])
+def test_dsgd_schedule_and_seeds_follow_the_master_stream(order):
+    """Epoch e's schedule is computeDsgdSchedule on the master stream, then
+    the d task seeds; the same stream as orc_dsgd_schedule plus d draws."""
+    p = random_problem(30, 40, 300, 3, seed=5)
+    w, t = (2, 2) if order == "COWOR" else (1, 4)
+    o = _oracle(p, "Nzsl", tasks=t, world_size=w, stratum_order=order, seed=123, algorithm="DSGD")
+    o.epoch(0.01)
+    np.testing.assert_array_equal(o.schedule(), po.dsgd_schedule(w, t, order, 123))
+    d = w * t
+    for row in o.schedule():                      # conflict-free sub-epochs
+        assert sorted(row) == list(range(d))
+    seeds = o.task_seeds()
+    assert len(set(int(s) for s in seeds)) == d
+
+
+def test_dsgd_run_loop_and_losses():
+    """The defaultRunner loop drives DSGD as it drives DSGD++
+    (dsgd_impl.h:62-64): losses decrease, the loss sums d x d blocks."""
+    p = random_problem(60, 80, 1500, 4, seed=9)
+    o = _oracle(p, "Nzsl_L2(0.05)", tasks=4, seed=3, algorithm="DSGD")
+    ref = _oracle(p, "Nzsl_L2(0.05)", tasks=4, seed=3)
+    assert o.loss(0) == pytest.approx(ref.loss(0), rel=1e-13)   # same data, other blocking
+    r = o.run(5, "BoldDriver(0.01)")
+    assert r["loss"][-1] < r["loss"][0]
