@@ -12,9 +12,12 @@ before timing starts.
 
 Prints ONE JSON line: metric/value (nnz updates/s), roofline of the epoch
 kernel (algorithmic bytes per update = (8 + s_x) + 4 r s_f, HIP-event time on
-its stream), and the CPU baseline (the oracle, an FP64 C++ restatement of the
-reference's mfdsgdpp run, with d = --tasks-per-rank = the host cores, whole
-epochs of the same data).
+its stream), the metric's second half (time_to_target: wall time to the plain
+FP64 GPU run's final test RMSE + 0.5 %, FP32 and BF16 with the learned step
+controller against it, every epoch evaluating training and test loss as the
+reference runner does), and the CPU baseline (the oracle, an FP64 C++
+restatement of the reference's mfdsgdpp run, with d = --tasks-per-rank = the
+host cores, whole epochs of the same data).
 """
 import argparse
 import json
@@ -44,6 +47,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0)
     ap.add_argument("--tasks", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-target", action="store_true",
+                    help="skip the time-to-target runs of the default line")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
@@ -138,6 +143,7 @@ def main():
             f"(perm {st['perm_seconds'] * 1e3:.1f}, strata {st['sgd_seconds'] * 1e3:.1f}), loss {cur:.6g}")
     test = g.loss(D.TEST)
     rmse = math.sqrt(test / nnz_test)
+    g.close()
 
     ms_per_step = wall / args.steps * 1e3
     value = nnz * args.steps / wall
@@ -149,6 +155,10 @@ def main():
     per_launch_bytes = nnz * b_upd * args.steps / launches
     avg_launch_s = sgd_s / launches
     achieved = per_launch_bytes / avg_launch_s / 1e9
+
+    ttt = None
+    if name == "zipf100m" and not args.no_time_to_target:
+        ttt = time_to_target(p, update, decay0, tasks, epochs=args.epochs or 30)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -192,6 +202,7 @@ def main():
         },
         "train_loss": cur,
         "test_rmse": rmse,
+        "time_to_target": ttt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -362,75 +373,98 @@ def report_precision(args):
     print(json.dumps(out))
 
 
-def report_time_to_rmse(args):
-    """Wall time to reach the plain FP64 GPU run's final test RMSE (+0.5%):
-    plain DSGD++ (FP64 factors and math, BoldDriver) for the full run,
-    then FP32 and BF16 factors, each with BoldDriver and with the learned step
-    controller, each stopping at the target.  Every epoch evaluates the
-    training and test loss, as the reference runner does."""
+def _run_to_target(D, p, update, tasks, prec, rule, target, max_epochs, seed=12345):
+    """One arm: a fresh context (load not timed), then epochs with the training
+    and test loss after each (sgd_impl.h:115-143) until the test RMSE reaches
+    `target` (None: all `max_epochs`).  Returns [(seconds since start, rmse)]."""
     import torch
+    g = _ctx(D, p, update, tasks, prec, seed=seed)
+    nt = len(p["test"][2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prev, cur = None, g.loss(D.TRAIN)
+    hist = []
+    for _ in range(max_epochs):
+        g.epoch(rule(prev, cur))
+        prev, cur = cur, g.loss(D.TRAIN)
+        hist.append((time.perf_counter() - t0, math.sqrt(g.loss(D.TEST) / nt)))
+        if target is not None and hist[-1][1] <= target:
+            break
+    g.close()
+    if getattr(rule, "mlp", None) is not None:
+        rule.mlp.close()
+    return hist
+
+
+def time_to_target(p, update, decay0, tasks, epochs=30, arms=(("FP32", "learned"), ("BF16", "learned"),
+                                                              ("FP32", "bolddriver"))):
+    """The metric's second half: plain DSGD++ in FP64 (the FP64 grouped
+    kernel, per-epoch loss within 1e-9 of the reference arithmetic) with
+    BoldDriver for `epochs` epochs sets the target, its final test RMSE +
+    0.5 %; each arm (precision, step rule) runs until it reaches the target.
+    Reported: seconds and epochs to the target, the RMSE reached, and the
+    plain run's time to the target over the arm's."""
     import dsgdpp_amd as D
-    from dsgdpp_amd import controller, data
+    from dsgdpp_amd import controller
+    rules = {"bolddriver": D.BoldDriver, "learned": controller.LearnedDecay}
+    t_all = time.time()
+    h64 = _run_to_target(D, p, update, tasks, "FP64", D.BoldDriver(decay0), None, epochs)
+    target = h64[-1][1] * 1.005
+    t64, e64 = next((t, k + 1) for k, (t, r) in enumerate(h64) if r <= target)
+    log(f"[time-to-target] FP64 plain: {len(h64)} epochs in {h64[-1][0]:.2f}s, final test rmse {h64[-1][1]:.5f}; "
+        f"target {target:.5f} reached at epoch {e64}, {t64:.2f}s")
+    out = {"target_test_rmse": target,
+           "rule": "target = plain FP64 run's final test RMSE x 1.005; training and test loss every epoch; "
+                   "context load not timed",
+           "fp64_plain": {"precision": "FP64", "step_rule": f"BoldDriver({decay0})", "epochs": len(h64),
+                          "seconds_total": h64[-1][0], "final_test_rmse": h64[-1][1],
+                          "epochs_to_target": e64, "seconds_to_target": t64}}
+    for prec, rname in arms:
+        h = _run_to_target(D, p, update, tasks, prec, rules[rname](decay0), target, epochs)
+        hit = [(t, k + 1) for k, (t, r) in enumerate(h) if r <= target]
+        key = f"{prec.lower()}_{rname}"
+        out[key] = {"precision": prec, "step_rule": rname, "reached": bool(hit),
+                    "epochs_to_target": hit[0][1] if hit else None, "seconds_to_target": hit[0][0] if hit else None,
+                    "test_rmse_reached": h[-1][1], "speedup_vs_fp64_plain": (t64 / hit[0][0]) if hit else None}
+        log(f"[time-to-target] {key}: {out[key]}")
+    chosen = out.get("fp32_learned") or {}
+    out["chosen"] = "fp32_learned"
+    out["speedup"] = chosen.get("speedup_vs_fp64_plain")
+    out["bench_seconds"] = time.time() - t_all
+    return out
+
+
+def report_time_to_rmse(args):
+    """The time_to_target object of the default line on its own (d = 128 by
+    default, as earlier records), plus the CPU oracle's time to the same
+    target."""
+    import torch
+    from dsgdpp_amd import data
     epochs = args.epochs or 30
     p = data.zipf_100m(device="cuda", shrink=args.shrink)
     torch.cuda.synchronize()
     update, decay0, tasks = "Nzsl_L2(0.05)", 0.01, args.tasks or 128
-    nt = len(p["test"][2])
-
-    def run(prec, rule, target=None, max_epochs=epochs):
-        g = _ctx(D, p, update, tasks, prec, seed=12345)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        prev, cur = None, g.loss(D.TRAIN)
-        hist = []
-        for e in range(max_epochs):
-            g.epoch(rule(prev, cur))
-            prev, cur = cur, g.loss(D.TRAIN)
-            rmse = math.sqrt(g.loss(D.TEST) / nt)
-            hist.append((time.perf_counter() - t0, rmse))
-            if target is not None and rmse <= target:
-                break
-        g.close()
-        return hist
-
-    h64 = run("FP64", D.BoldDriver(decay0))
-    target = h64[-1][1] * 1.005
-    log(f"[time-to-rmse] FP64 plain: {len(h64)} epochs, {h64[-1][0]:.2f}s, test rmse {h64[-1][1]:.5f}; "
-        f"target {target:.5f}")
-    t64 = next(t for t, r in h64 if r <= target)
-    arms = [(f"{prec.lower()}_{kind}", run(prec, rule(decay0), target))
-            for prec in ("FP32", "BF16")
-            for kind, rule in (("bolddriver", D.BoldDriver), ("learned", controller.LearnedDecay))]
-    res = {}
-    for name, h in arms:
-        hit = [t for t, r in h if r <= target]
-        res[name] = {"epochs": len(h), "seconds": hit[0] if hit else None, "final_test_rmse": h[-1][1],
-                     "reached": bool(hit), "speedup_vs_fp64": (t64 / hit[0]) if hit else None}
-        log(f"[time-to-rmse] {name}: {res[name]}")
+    res = time_to_target(p, update, decay0, tasks, epochs,
+                         arms=(("FP32", "bolddriver"), ("FP32", "learned"), ("BF16", "bolddriver"),
+                               ("BF16", "learned")))
     if not args.no_cpu_baseline:
-        res["cpu_fp64_oracle"] = cpu_time_to_rmse(p, update, decay0, target, args.cpu_threads, epochs)
+        res["cpu_fp64_oracle"] = cpu_time_to_rmse(p, update, decay0, res["target_test_rmse"], args.cpu_threads,
+                                                  epochs)
         log(f"[time-to-rmse] cpu_fp64_oracle: {res['cpu_fp64_oracle']}")
     print(json.dumps({"report": "time-to-rmse", "workload": f"zipf100m: {p['m']}x{p['n']}, "
-                      f"{len(p['train'][2])} train / {nt} test nonzeros, rank 50, {update}, start "
-                      f"BoldDriver({decay0}), {tasks}x{2 * tasks} blocks",
-                      "fp64_plain": {"precision": "FP64 (grouped kernel; per-epoch loss within 1e-9 of the "
-                                                  "reference arithmetic)",
-                                     "epochs": len(h64), "seconds_total": h64[-1][0], "final_test_rmse": h64[-1][1],
-                                     "seconds_to_target": t64},
-                      "target_test_rmse": target, **res}))
+                      f"{len(p['train'][2])} train / {len(p['test'][2])} test nonzeros, rank 50, {update}, start "
+                      f"eps {decay0}, {tasks}x{2 * tasks} blocks", **res}))
 
 
 def report_controller(args):
     """The step-size controller at large scale (dsgdpp_amd.data.controller_250m:
-    1M x 625k, 250M training nonzeros, Zipf(1.1), rank 100, FP32,
-    Nzsl_L2(0.05)): plain BoldDriver(0.001) for `epochs` epochs sets the target
-    (its best test RMSE + 0.5%: with rank 100 and noise 5 the test RMSE turns up
-    after a few epochs); the learned controller, started from the same step,
-    runs as many epochs.  Both evaluate training and test loss after every
-    epoch; reported are the times to the target and each run's best RMSE.
-    d = 256 by default: the column blocks are too wide for an LDS tile (H rows
-    stream from HBM, kernel 4) and the largest blocks keep their prep state in
-    global scratch; one task per CU."""
+    1M x 625k, 250M training nonzeros, Zipf(1.1), rank 100, Nzsl_L2(0.05)):
+    the plain FP64 run with BoldDriver(0.001) for `epochs` epochs sets the
+    target (its final test RMSE + 0.5 %); FP32 with the learned controller and
+    FP32 with BoldDriver, started from the same step, run until they reach it.
+    Every epoch evaluates training and test loss.  d = 256 by default: the
+    column blocks are too wide for an LDS tile (H rows stream from HBM,
+    kernel 4) and the largest blocks keep their prep state in global scratch."""
     import torch
     import dsgdpp_amd as D
     from dsgdpp_amd import controller, data
@@ -440,46 +474,29 @@ def report_controller(args):
     torch.cuda.synchronize()
     log(f"[controller] generated {len(p['train'][2])} nonzeros in {time.time() - t0:.1f}s")
     update, decay0, tasks = "Nzsl_L2(0.05)", 0.001, args.tasks or 256
-    nt = len(p["test"][2])
-
-    def run(name, rule, target=None):
-        t1 = time.time()
-        g = _ctx(D, p, update, tasks, "FP32", seed=12345)
-        log(f"[controller] {name}: context loaded in {time.time() - t1:.1f}s")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        prev, cur = None, g.loss(D.TRAIN)
-        hist, kernel = [], None
-        for e in range(epochs):
-            st = g.epoch(rule(prev, cur))
-            kernel = st["kernel"]
-            prev, cur = cur, g.loss(D.TRAIN)
-            rmse = math.sqrt(g.loss(D.TEST) / nt)
-            hist.append((time.perf_counter() - t0, rmse))
-            log(f"[controller] {name} epoch {e + 1}: test rmse {rmse:.5f} at {hist[-1][0]:.1f}s")
-            if target is not None and rmse <= target:
-                break
-        g.close()
-        return hist, kernel
-
-    hb, kern = run("bolddriver", D.BoldDriver(decay0))
-    target = min(r for _, r in hb) * 1.005
-    tb = next(t for t, r in hb if r <= target)
-    hl, _ = run("learned", controller.LearnedDecay(decay0))
-    hit = [t for t, r in hl if r <= target]
-
-    def best(h):
-        k = min(range(len(h)), key=lambda q: h[q][1])
-        return {"best_test_rmse": h[k][1], "best_epoch": k + 1, "seconds_at_best": h[k][0]}
-
+    h64 = _run_to_target(D, p, update, tasks, "FP64", D.BoldDriver(decay0), None, epochs)
+    target = h64[-1][1] * 1.005
+    t64, e64 = next((t, k + 1) for k, (t, r) in enumerate(h64) if r <= target)
+    best64 = min(range(len(h64)), key=lambda q: h64[q][1])
+    log(f"[controller] FP64 plain: final rmse {h64[-1][1]:.5f} after {h64[-1][0]:.1f}s; target {target:.5f} at "
+        f"epoch {e64} ({t64:.2f}s); best {h64[best64][1]:.5f} at epoch {best64 + 1}")
+    res = {}
+    for name, prec, rule in (("fp32_learned", "FP32", controller.LearnedDecay(decay0)),
+                             ("fp32_bolddriver", "FP32", D.BoldDriver(decay0))):
+        h = _run_to_target(D, p, update, tasks, prec, rule, target, epochs)
+        hit = [(t, k + 1) for k, (t, r) in enumerate(h) if r <= target]
+        res[name] = {"reached": bool(hit), "epochs_to_target": hit[0][1] if hit else None,
+                     "seconds_to_target": hit[0][0] if hit else None, "test_rmse_reached": h[-1][1],
+                     "speedup_vs_fp64_plain": (t64 / hit[0][0]) if hit else None}
+        log(f"[controller] {name}: {res[name]}")
     print(json.dumps({"report": "controller", "workload": f"controller_250m: {p['m']}x{p['n']}, "
-                      f"{len(p['train'][2])} train / {nt} test nonzeros, rank 100, {update}, FP32, start "
-                      f"eps {decay0}, {tasks}x{2 * tasks} blocks, kernel {kern}, {epochs} epochs each",
+                      f"{len(p['train'][2])} train / {len(p['test'][2])} test nonzeros, rank 100, {update}, "
+                      f"start eps {decay0}, {tasks}x{2 * tasks} blocks, {epochs} epochs for the plain run",
                       "target_test_rmse": target,
-                      "bolddriver": {"seconds_to_target": tb, "seconds_total": hb[-1][0], **best(hb)},
-                      "learned": {"seconds_to_target": hit[0] if hit else None, "reached": bool(hit),
-                                  "seconds_total": hl[-1][0], **best(hl),
-                                  "speedup_to_target": (tb / hit[0]) if hit else None}}))
+                      "fp64_plain": {"epochs": len(h64), "seconds_total": h64[-1][0], "final_test_rmse": h64[-1][1],
+                                     "epochs_to_target": e64, "seconds_to_target": t64,
+                                     "best_test_rmse": h64[best64][1], "best_epoch": best64 + 1},
+                      **res}))
 
 
 def bench_batch(args):
