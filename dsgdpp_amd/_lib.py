@@ -28,7 +28,8 @@ EXPORTS = (
     "dsgdpp_load_coo", "dsgdpp_load_mmc", "dsgdpp_set_factors", "dsgdpp_get_factors",
     "dsgdpp_load_factors_mma", "dsgdpp_write_factors_mma", "dsgdpp_epoch", "dsgdpp_loss",
     "dsgdpp_loss_noalloc", "dsgdpp_balance", "dsgdpp_run", "dsgdpp_get_schedule", "dsgdpp_get_task_seeds",
-    "dsgdpp_get_permutation", "dsgdpp_block_nnz", "dsgdpp_dims", "dsgdpp_grouped_ok",
+    "dsgdpp_get_permutation", "dsgdpp_block_nnz", "dsgdpp_dims", "dsgdpp_grouped_ok", "dsgdpp_get_block",
+    "dsgdpp_save_blocks", "dsgdpp_load_blocks", "dsgdpp_load_mmc_cached",
     "dsgdpp_mlp_num_params", "dsgdpp_mlp_create", "dsgdpp_mlp_create_error", "dsgdpp_mlp_last_error",
     "dsgdpp_mlp_destroy", "dsgdpp_mlp_set_params", "dsgdpp_mlp_get_params", "dsgdpp_mlp_forward",
     "dsgdpp_mlp_gradient", "dsgdpp_mlp_train",
@@ -99,6 +100,15 @@ def load():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
             "(the GPU path has no fallback)")
+    # torch wheels carry their own HIP runtime (torch/lib/libamdhip64.so and
+    # libhsa-runtime64.so, asked for by unversioned names).  Loaded after this
+    # library's runtime it would be a second runtime in the process, and it
+    # finds no device; loaded first, this library binds to it (the sonames
+    # match).  So torch, when installed, goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -128,6 +138,10 @@ def load():
         "dsgdpp_get_task_seeds": (i32, [P, pu64]),
         "dsgdpp_get_permutation": (i32, [P, i32, i32, pu32, i64, pi64]),
         "dsgdpp_block_nnz": (i32, [P, i32, pi64]),
+        "dsgdpp_get_block": (i32, [P, i32, i32, i32, pi32, pi32, pd, i64, pi64]),
+        "dsgdpp_save_blocks": (i32, [P, i32, ctypes.c_char_p]),
+        "dsgdpp_load_blocks": (i32, [P, i32, ctypes.c_char_p]),
+        "dsgdpp_load_mmc_cached": (i32, [P, i32, ctypes.c_char_p, ctypes.c_char_p, pi32]),
         "dsgdpp_dims": (i32, [P, pi64]),
         "dsgdpp_grouped_ok": (i32, [P, pi32, ctypes.c_char_p, i64]),
         "dsgdpp_mlp_num_params": (i32, []),

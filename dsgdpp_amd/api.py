@@ -226,6 +226,25 @@ class DsgdPp:
         dims = self.dims()
         self.m, self.n = int(dims[0]), int(dims[1])
 
+    def save_blocks(self, which, path):
+        """dsgdpp_save_blocks: the loaded matrix in the device layout, to a binary cache file."""
+        self._check(self._lib.dsgdpp_save_blocks(self._h, which, str(path).encode()))
+
+    def load_blocks(self, which, path):
+        """dsgdpp_load_blocks: a matrix from a binary cache file (in place of load_coo)."""
+        self._check(self._lib.dsgdpp_load_blocks(self._h, which, str(path).encode()))
+        d = self.dims()
+        self.m, self.n = int(d[0]), int(d[1])
+
+    def load_mmc_cached(self, which, path, cache_path):
+        """dsgdpp_load_mmc_cached; True when the cache was used."""
+        hit = ctypes.c_int32(0)
+        self._check(self._lib.dsgdpp_load_mmc_cached(self._h, which, str(path).encode(), str(cache_path).encode(),
+                                                     ctypes.byref(hit)))
+        d = self.dims()
+        self.m, self.n = int(d[0]), int(d[1])
+        return bool(hit.value)
+
     def set_factors(self, W, H):
         """W: m x r row-major; H: r x n (any layout; stored column-major)."""
         W = L.as_f64(W)
@@ -309,6 +328,17 @@ class DsgdPp:
         out = np.empty((self.d, self.subepochs), dtype=np.int64)
         self._check(self._lib.dsgdpp_block_nnz(self._h, which, L.ptr(out, ctypes.c_int64)))
         return out
+
+    def block(self, which, b1, b2):
+        """One block's points as stored on the device: (i, j, x) with global indices."""
+        n = ctypes.c_int64()
+        self._check(self._lib.dsgdpp_get_block(self._h, which, b1, b2, None, None, None, 0, ctypes.byref(n)))
+        k = max(n.value, 1)
+        i, j, x = np.empty(k, dtype=np.int32), np.empty(k, dtype=np.int32), np.empty(k)
+        self._check(self._lib.dsgdpp_get_block(self._h, which, b1, b2, L.ptr(i, ctypes.c_int32),
+                                               L.ptr(j, ctypes.c_int32), L.ptr(x, ctypes.c_double), k,
+                                               ctypes.byref(n)))
+        return i[: n.value], j[: n.value], x[: n.value]
 
     def dims(self):
         out = np.empty(6, dtype=np.int64)

@@ -1,8 +1,10 @@
 // Host side of dsgdpp_amd: the C ABI (include/dsgdpp.h), functor-string
-// parsing, d x 2d blocking into the device layout, the per-epoch stratum
-// schedule and task streams, the decay/loss/trace run loop, and Matrix Market
-// I/O.  Device work is in kernels.hip.
+// parsing, the d x 2d block layout, the per-epoch stratum schedule and task
+// streams, the decay/loss/trace run loop, Matrix Market I/O and the binary
+// block cache.  Device work is in kernels.hip, kernels_grouped.hip, ingest.hip
+// and mlp.hip.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <chrono>
@@ -207,7 +209,6 @@ struct dsgdpp_ctx {
     int64_t m = -1, n = -1;
     std::vector<int64_t> off1, off2;
     HostBlocks blocks[2];
-    std::vector<int64_t> nnz1, nnz2;
     bool haveFactors = false;
 
     hipStream_t stream = nullptr;
@@ -365,87 +366,11 @@ void configure(dsgdpp_ctx* c, const dsgdpp_cfg* cfg) {
 // ---------------------------------------------------------------------------
 // blocking: ReadMatrixBlocksMm (read_impl.h:365-486) puts each entry into the
 // block of its (row, column) with block-local indices and sorts every block
-// row-major (read_impl.h:163; ublas sort sums duplicates).  Here blocks are
-// sorted stably, duplicates summed in input order, and the result is written
-// as one array of 16-byte point records, block after block (d x 2d,
-// row-major), with global indices.
+// row-major (read_impl.h:163; ublas sort sums duplicates).  Here the device
+// does it (ingest.hip): blocks sorted stably, duplicates summed in input
+// order, the result one array of 16-byte point records, block after block
+// (d x 2d, row-major), with global indices.
 // ---------------------------------------------------------------------------
-template <typename Pt>
-void buildBlocks(dsgdpp_ctx* c, int64_t nnz, const int32_t* I, const int32_t* J, const double* X,
-                 std::vector<Pt>& out, std::vector<int64_t>& start) {
-    const uint32_t d = c->d, d2 = 2 * d;
-    const size_t nb = (size_t)d * d2;
-    std::vector<uint32_t> rb(c->m), cb(c->n);
-    for (uint32_t b = 0; b < d; ++b)
-        for (int64_t i = c->off1[b]; i < c->off1[b + 1]; ++i) rb[i] = b;
-    for (uint32_t b = 0; b < d2; ++b)
-        for (int64_t j = c->off2[b]; j < c->off2[b + 1]; ++j) cb[j] = b;
-    // counting sort by block, stable: thread t counts and scatters the t-th
-    // chunk of the input, its points placed after those of earlier chunks
-    const int th = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)hostThreads(), nnz / 65536 + 1,
-                                                                 (int64_t)(32 << 20) / (int64_t)nb}));
-    std::vector<int64_t> cnt(nb + 1, 0);
-    std::vector<std::vector<int64_t>> part(th);
-    auto blk = [&](int64_t p) { return (size_t)rb[I[p]] * d2 + cb[J[p]]; };
-    auto each = [&](auto fn) {
-        std::vector<std::thread> ts;
-        for (int t = 0; t < th; ++t) ts.emplace_back(fn, t, nnz * t / th, nnz * (t + 1) / th);
-        for (auto& x : ts) x.join();
-    };
-    each([&](int t, int64_t lo, int64_t hi) {
-        part[t].assign(nb, 0);
-        for (int64_t p = lo; p < hi; ++p) part[t][blk(p)]++;
-    });
-    for (size_t b = 0; b < nb; ++b) {
-        int64_t at = cnt[b];
-        for (int t = 0; t < th; ++t) {
-            const int64_t k = part[t][b];
-            part[t][b] = at;   // thread t's first position in block b
-            at += k;
-        }
-        cnt[b + 1] = at;
-    }
-    std::vector<int64_t> order(nnz);
-    each([&](int t, int64_t lo, int64_t hi) {
-        std::vector<int64_t>& pos = part[t];
-        for (int64_t p = lo; p < hi; ++p) order[pos[blk(p)]++] = p;
-    });
-    part.clear();
-    std::vector<int64_t> kept(nb, 0);
-    out.resize(nnz);
-    parallelFor((int64_t)nb, [&](int64_t lo, int64_t hi) {
-        for (int64_t b = lo; b < hi; ++b) {
-            int64_t* first = order.data() + cnt[b];
-            int64_t* last = order.data() + cnt[b + 1];
-            std::stable_sort(first, last, [&](int64_t a, int64_t q) {
-                return I[a] != I[q] ? I[a] < I[q] : J[a] < J[q];
-            });
-            Pt* dst = out.data() + cnt[b];
-            int64_t k = 0;
-            for (int64_t* it = first; it != last; ++it) {
-                const int64_t p = *it;
-                if (k > 0 && dst[k - 1].i == I[p] && dst[k - 1].j == J[p]) {
-                    dst[k - 1].x = (decltype(dst[0].x))((double)dst[k - 1].x + X[p]);
-                } else {
-                    Pt e{};
-                    e.i = I[p];
-                    e.j = J[p];
-                    e.x = (decltype(e.x))X[p];
-                    dst[k++] = e;
-                }
-            }
-            kept[b] = k;
-        }
-    });
-    // compact (duplicates removed)
-    start.assign(nb + 1, 0);
-    for (size_t b = 0; b < nb; ++b) start[b + 1] = start[b] + kept[b];
-    if (start[nb] != nnz) {
-        for (size_t b = 0; b < nb; ++b)
-            if (start[b] != cnt[b]) std::copy(out.begin() + cnt[b], out.begin() + cnt[b] + kept[b], out.begin() + start[b]);
-        out.resize(start[nb]);
-    }
-}
 
 // Blocks whose visits run two-slot groups (FP32 on the tile kernel, rank <= 64).
 // A visit's groups number at least its busiest column's point count c, and at
@@ -454,32 +379,25 @@ void buildBlocks(dsgdpp_ctx* c, int64_t nnz, const int32_t* I, const int32_t* J,
 // 45 max(c, n/2) < 54 max(c, n/4), i.e. c > 5n/12.  The prep and the epoch
 // kernel read the same flags, and a four-slot loop would also run a two-slot
 // visit correctly (its slots 2 and 3 hold dummy records).
-template <typename Pt>
-void choosePairs(dsgdpp_ctx* c, const std::vector<Pt>& pts, const std::vector<int64_t>& start) {
+void choosePairs(dsgdpp_ctx* c) {
     c->pairBlk.clear();
     c->pairBlocks = 0;
     c->G.pairs = nullptr;
-    if (!std::is_same<Pt, Pt32>::value || c->precision != DSGDPP_FP32 || !c->groupedOk || c->G.wide ||
-        c->P.ld > 64 || c->pairsMode == 0)
+    if (c->precision != DSGDPP_FP32 || !c->groupedOk || c->G.wide || c->P.ld > 64 || c->pairsMode == 0 ||
+        c->G.max_cols > 16384)
         return;
-    const uint32_t d = c->d, d2 = 2 * d;
-    std::vector<uint8_t> flag((size_t)d * d2, 0);
-    parallelFor((int64_t)d, [&](int64_t lo, int64_t hi) {
-        std::vector<int32_t> cnt;
-        for (int64_t b1 = lo; b1 < hi; ++b1)
-            for (uint32_t b2 = 0; b2 < d2; ++b2) {
-                const size_t b = (size_t)b1 * d2 + b2;
-                const int64_t n = start[b + 1] - start[b];
-                if (n == 0) continue;
-                if (c->pairsMode == 1) { flag[b] = 1; continue; }
-                const int64_t col0 = c->off2[b2];
-                cnt.assign((size_t)(c->off2[b2 + 1] - col0), 0);
-                int32_t top = 0;
-                for (int64_t p = start[b]; p < start[b + 1]; ++p) top = std::max(top, ++cnt[(size_t)(pts[p].j - col0)]);
-                flag[b] = 12 * (int64_t)top > 5 * n ? 1 : 0;
-            }
-    });
-    for (uint8_t f : flag) c->pairBlocks += f;
+    const std::vector<int64_t>& start = c->blocks[0].start;
+    const size_t nb = start.size() - 1;
+    std::vector<int32_t> top;
+    if (c->pairsMode != 1)
+        device_block_top(false, c->dPts[0], c->dBlk[0], c->off2.data(), (int64_t)c->off2.size() - 1, (int64_t)nb,
+                         c->G.max_cols, top, c->stream);
+    std::vector<uint8_t> flag(nb, 0);
+    for (size_t b = 0; b < nb; ++b) {
+        const int64_t n = start[b + 1] - start[b];
+        if (n > 0) flag[b] = c->pairsMode == 1 || 12 * (int64_t)top[b] > 5 * n ? 1 : 0;
+        c->pairBlocks += flag[b];
+    }
     if (c->pairBlocks == 0) return;
     c->pairBlk.swap(flag);
     dalloc(c->dPairs, c->pairBlk.size());
@@ -487,119 +405,92 @@ void choosePairs(dsgdpp_ctx* c, const std::vector<Pt>& pts, const std::vector<in
     c->G.pairs = c->dPairs;
 }
 
-template <typename Pt>
-void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const int32_t* J,
-                  const double* X) {
-    std::vector<Pt> pts;
+// The derived state of a loaded matrix whose point records and block starts
+// are on the device: for the training matrix the counts, 1/count arrays and
+// the FP32 records' row factors (nnz12, mf/matrix/op/nnz.h:34-58), block
+// extents, kernel eligibility and the two-slot flags.
+void finishBlocks(dsgdpp_ctx* c, int which) {
     HostBlocks& hb = c->blocks[which];
-    buildBlocks<Pt>(c, nnz, I, J, X, pts, hb.start);
-    hb.nnz = (int64_t)pts.size();
-    if constexpr (std::is_same<Pt, Pt32>::value) {
-        // training points carry 1/nnz of their row (FP32 bits) for the grouped prep kernel
-        if (which == DSGDPP_TRAIN) {
-            // a row's points lie in its row block's 2d blocks, which are contiguous
-            std::vector<int64_t> rc(c->m, 0);
-            const uint32_t d2 = 2 * c->d;
-            parallelFor((int64_t)c->d, [&](int64_t lo, int64_t hi) {
-                for (int64_t b1 = lo; b1 < hi; ++b1) {
-                    Pt* const p0 = pts.data() + hb.start[(size_t)b1 * d2];
-                    Pt* const p1 = pts.data() + hb.start[(size_t)(b1 + 1) * d2];
-                    for (Pt* e = p0; e != p1; ++e) rc[e->i]++;
-                    for (Pt* e = p0; e != p1; ++e) {
-                        const float f = (float)(1. / (double)rc[e->i]);
-                        memcpy(&e->aux, &f, 4);
-                    }
-                }
-            });
-        }
-    }
-    dalloc(c->dPts[which], pts.size() * sizeof(Pt));
-    HIPC(hipMemcpy(c->dPts[which], pts.data(), pts.size() * sizeof(Pt), hipMemcpyHostToDevice));
-    dalloc(c->dBlk[which], hb.start.size() * sizeof(int64_t));
-    HIPC(hipMemcpy(c->dBlk[which], hb.start.data(), hb.start.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     hb.loaded = true;
-    if (which == DSGDPP_TRAIN) {
-        // nnz12 (mf/matrix/op/nnz.h:34-58)
-        c->nnz1.assign(c->m, 0);
-        c->nnz2.assign(c->n, 0);
-        for (const Pt& e : pts) { c->nnz1[e.i]++; c->nnz2[e.j]++; }
-        std::vector<double> cnt1(c->m), cnt2(c->n);
-        for (int64_t i = 0; i < c->m; ++i) cnt1[i] = (double)c->nnz1[i];
-        for (int64_t j = 0; j < c->n; ++j) cnt2[j] = (double)c->nnz2[j];
-        auto upInv = [&](const std::vector<double>& cntv, void*& dst) {
-            if (c->fp64()) {   // f3 = 1./nnz1[i] (update-nzsl-l2.h:47-48), same rounding as on the host
-                std::vector<double> inv(cntv.size());
-                for (size_t k = 0; k < cntv.size(); ++k) inv[k] = 1. / cntv[k];
-                dalloc(dst, inv.size() * sizeof(double));
-                HIPC(hipMemcpy(dst, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
-            } else {
-                std::vector<float> inv(cntv.size());
-                for (size_t k = 0; k < cntv.size(); ++k) inv[k] = (float)(1. / cntv[k]);
-                dalloc(dst, inv.size() * sizeof(float));
-                HIPC(hipMemcpy(dst, inv.data(), inv.size() * sizeof(float), hipMemcpyHostToDevice));
-            }
-        };
-        void* inv1 = const_cast<void*>(c->P.inv1);
-        void* inv2 = const_cast<void*>(c->P.inv2);
-        upInv(cnt1, inv1);
-        upInv(cnt2, inv2);
-        c->P.inv1 = inv1;
-        c->P.inv2 = inv2;
-        double* dc1 = const_cast<double*>(c->P.cnt1);
-        double* dc2 = const_cast<double*>(c->P.cnt2);
-        dalloc(dc1, cnt1.size() * sizeof(double));
-        dalloc(dc2, cnt2.size() * sizeof(double));
-        HIPC(hipMemcpy(dc1, cnt1.data(), cnt1.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPC(hipMemcpy(dc2, cnt2.data(), cnt2.size() * sizeof(double), hipMemcpyHostToDevice));
-        c->P.cnt1 = dc1;
-        c->P.cnt2 = dc2;
-        c->P.pts = c->dPts[0];
-        c->P.blk = c->dBlk[0];
-        c->P.nnz = hb.nnz;
-        // grouped epoch: block extents and eligibility
-        int64_t maxBlock = 0;
-        for (size_t b = 0; b + 1 < hb.start.size(); ++b) maxBlock = std::max(maxBlock, hb.start[b + 1] - hb.start[b]);
-        int maxRows = 0, maxCols = 0;
-        for (size_t b = 0; b + 1 < c->off1.size(); ++b) maxRows = std::max<int>(maxRows, (int)(c->off1[b + 1] - c->off1[b]));
-        for (size_t b = 0; b + 1 < c->off2.size(); ++b) maxCols = std::max<int>(maxCols, (int)(c->off2[b + 1] - c->off2[b]));
-        c->G.max_rows = maxRows;
-        c->G.max_cols = maxCols;
-        c->maxBlock = maxBlock;
-        bool wide = false;
-        c->groupedOk = grouped_supported(c->P, c->upd, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
-                                         c->kernel == 4);
-        c->G.wide = wide;
-        if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
-        choosePairs<Pt>(c, pts, hb.start);
-        if (c->groupedOk) {
-            std::vector<int32_t> o2(c->off2.begin(), c->off2.end());
-            int32_t* d2p = const_cast<int32_t*>(c->G.off2);
-            dalloc(d2p, o2.size() * sizeof(int32_t));
-            HIPC(hipMemcpy(d2p, o2.data(), o2.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-            c->G.off2 = d2p;
-        }
-        if (c->kernel == 3 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "grouped kernel: " + c->groupedWhy);
-    } else {
+    if (which != DSGDPP_TRAIN) {
         c->P.test_pts = c->dPts[1];
         c->P.test_blk = c->dBlk[1];
         c->P.test_nnz = hb.nnz;
+        return;
     }
+    DerivedOut dv;
+    try {
+        device_derive(c->P, true, c->dPts[0], hb.nnz, dv, c->stream);
+    } catch (...) {
+        for (void* q : {dv.inv1, dv.inv2, (void*)dv.cnt1, (void*)dv.cnt2}) if (q) (void)hipFree(q);
+        throw;
+    }
+    for (const void* q : {c->P.inv1, c->P.inv2, (const void*)c->P.cnt1, (const void*)c->P.cnt2})
+        if (q) (void)hipFree(const_cast<void*>(q));
+    c->P.inv1 = dv.inv1;
+    c->P.inv2 = dv.inv2;
+    c->P.cnt1 = dv.cnt1;
+    c->P.cnt2 = dv.cnt2;
+    c->P.pts = c->dPts[0];
+    c->P.blk = c->dBlk[0];
+    c->P.nnz = hb.nnz;
+    // grouped epoch: block extents and eligibility
+    int64_t maxBlock = 0;
+    for (size_t b = 0; b + 1 < hb.start.size(); ++b) maxBlock = std::max(maxBlock, hb.start[b + 1] - hb.start[b]);
+    int maxRows = 0, maxCols = 0;
+    for (size_t b = 0; b + 1 < c->off1.size(); ++b) maxRows = std::max<int>(maxRows, (int)(c->off1[b + 1] - c->off1[b]));
+    for (size_t b = 0; b + 1 < c->off2.size(); ++b) maxCols = std::max<int>(maxCols, (int)(c->off2[b + 1] - c->off2[b]));
+    c->G.max_rows = maxRows;
+    c->G.max_cols = maxCols;
+    c->maxBlock = maxBlock;
+    bool wide = false;
+    c->groupedOk = grouped_supported(c->P, c->upd, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
+                                     c->kernel == 4);
+    c->G.wide = wide;
+    if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
+    choosePairs(c);
+    if (c->groupedOk) {
+        std::vector<int32_t> o2(c->off2.begin(), c->off2.end());
+        int32_t* d2p = const_cast<int32_t*>(c->G.off2);
+        dalloc(d2p, o2.size() * sizeof(int32_t));
+        HIPC(hipMemcpy(d2p, o2.data(), o2.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        c->G.off2 = d2p;
+    }
+    if (c->kernel == 3 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "grouped kernel: " + c->groupedWhy);
 }
 
-void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const int32_t* I,
-             const int32_t* J, const double* X) {
+void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const int32_t* J, const double* X) {
+    IngestOut io;
+    try {
+        device_ingest(IngestGeom{c->m, c->n, c->d, c->dsgd()}, c->fp64(), which == DSGDPP_TRAIN, nnz, I, J, X, io,
+                      c->stream);
+    } catch (...) {
+        if (io.pts) (void)hipFree(io.pts);
+        if (io.start_dev) (void)hipFree(io.start_dev);
+        throw;
+    }
+    dfree(c->dPts[which]);
+    c->dPts[which] = io.pts;
+    void* q = c->dBlk[which];
+    dfree(q);
+    c->dBlk[which] = io.start_dev;
+    HostBlocks& hb = c->blocks[which];
+    hb.start.swap(io.start);
+    hb.nnz = io.nnz;
+    finishBlocks(c, which);
+}
+
+// The checks and layout set-up every load of a matrix shares: dimensions
+// agree with the loaded data, the block grid fits, the block offsets and, on
+// the first load, the factor layout.
+void beginLoad(dsgdpp_ctx* c, int which, int64_t m, int64_t n) {
     if (which != DSGDPP_TRAIN && which != DSGDPP_TEST) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
-    if (m <= 0 || n <= 0 || nnz < 0) throw Error(DSGDPP_E_ARG, "invalid matrix dimensions");
+    if (m <= 0 || n <= 0) throw Error(DSGDPP_E_ARG, "invalid matrix dimensions");
     if (m >= INT32_MAX || n >= INT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "dimensions must fit in int32");
-    if (nnz > 0 && (!I || !J || !X)) throw Error(DSGDPP_E_ARG, "null index or value array");
     if (c->m >= 0 && (c->m != m || c->n != n)) throw Error(DSGDPP_E_ARG, "matrix dimensions differ from the loaded data");
     if (m < (int64_t)c->d || n < (int64_t)(c->dsgd() ? c->d : 2 * c->d))
         throw Error(DSGDPP_E_ARG, c->dsgd() ? "matrix is smaller than the d x d block grid"
                                             : "matrix is smaller than the d x 2d block grid");
-    for (int64_t p = 0; p < nnz; ++p)
-        if (I[p] < 0 || I[p] >= m || J[p] < 0 || J[p] >= n)
-            throw Error(DSGDPP_E_ARG, "index out of range at entry " + std::to_string(p));
-    if (which == DSGDPP_TRAIN && nnz >= (int64_t)UINT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "too many nonzeros");
     const bool first = c->m < 0;
     HIPC(hipStreamSynchronize(c->prepStream));   // a look-ahead plan may still read the old blocks
     ++c->dataVersion;
@@ -631,8 +522,18 @@ void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const 
         c->P.ld = (c->r + vec - 1) / vec * vec;
         c->dataflowOk = dataflow_fits(c->P, c->device);
     }
-    if (c->fp64()) uploadBlocks<Pt64>(c, which, nnz, I, J, X);
-    else uploadBlocks<Pt32>(c, which, nnz, I, J, X);
+}
+
+void loadCoo(dsgdpp_ctx* c, int which, int64_t m, int64_t n, int64_t nnz, const int32_t* I,
+             const int32_t* J, const double* X) {
+    if (nnz < 0) throw Error(DSGDPP_E_ARG, "invalid matrix dimensions");
+    if (nnz > 0 && (!I || !J || !X)) throw Error(DSGDPP_E_ARG, "null index or value array");
+    for (int64_t p = 0; p < nnz; ++p)
+        if (I[p] < 0 || I[p] >= m || J[p] < 0 || J[p] >= n)
+            throw Error(DSGDPP_E_ARG, "index out of range at entry " + std::to_string(p));
+    if (nnz >= (int64_t)UINT32_MAX) throw Error(DSGDPP_E_UNSUPPORTED, "too many nonzeros");
+    beginLoad(c, which, m, n);
+    uploadBlocks(c, which, nnz, I, J, X);
 }
 
 // factors: host doubles <-> device storage (W row-major, H column-major r x n)
@@ -1268,6 +1169,140 @@ void writeMma(const std::string& fn, int64_t rows, int64_t cols, const std::vect
 }
 
 // ---------------------------------------------------------------------------
+// Binary block cache: a loaded matrix in the device layout -- blocked, sorted,
+// duplicates summed -- with the blocking it was made for, so a later load of
+// the same data skips parsing, sorting and blocking.  Little-endian:
+//   BlockCacheHeader, int64 start[d * 2d + 1], 16-byte point records[nnz].
+// ---------------------------------------------------------------------------
+struct BlockCacheHeader {
+    char magic[8];            // "DSGDBLK1"
+    uint32_t version;         // 1
+    uint32_t record_bytes;    // 16
+    uint32_t fp64;            // records carry x as double (FP64 modes) or float
+    uint32_t algorithm;       // DSGDPP_ALGO_*
+    int64_t m, n;
+    uint32_t d, reserved;
+    int64_t nnz;              // point records
+    int64_t src_size;         // the Matrix Market file it was read from (0: none)
+    int64_t src_mtime_ns;
+};
+static_assert(sizeof(BlockCacheHeader) == 72, "cache header layout");
+
+bool fileStamp(const std::string& fn, int64_t& size, int64_t& mtime_ns) {
+    struct stat st;
+    if (stat(fn.c_str(), &st) != 0) return false;
+    size = (int64_t)st.st_size;
+    mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000LL + (int64_t)st.st_mtim.tv_nsec;
+    return true;
+}
+
+void saveBlocks(dsgdpp_ctx* c, int which, const std::string& fn, int64_t src_size, int64_t src_mtime_ns) {
+    if (which != DSGDPP_TRAIN && which != DSGDPP_TEST) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+    const HostBlocks& hb = c->blocks[which];
+    if (!hb.loaded) throw Error(DSGDPP_E_STATE, "matrix not loaded");
+    BlockCacheHeader h{};
+    memcpy(h.magic, "DSGDBLK1", 8);
+    h.version = 1;
+    h.record_bytes = 16;
+    h.fp64 = c->fp64() ? 1 : 0;
+    h.algorithm = (uint32_t)c->algorithm;
+    h.m = c->m;
+    h.n = c->n;
+    h.d = c->d;
+    h.nnz = hb.nnz;
+    h.src_size = src_size;
+    h.src_mtime_ns = src_mtime_ns;
+    std::vector<char> recs((size_t)hb.nnz * 16);
+    HIPC(hipStreamSynchronize(c->stream));
+    if (hb.nnz > 0) HIPC(hipMemcpy(recs.data(), c->dPts[which], recs.size(), hipMemcpyDeviceToHost));
+    const std::string tmp = fn + ".tmp";
+    {
+        std::ofstream out(tmp, std::ios::binary);
+        if (!out.is_open()) throw Error(DSGDPP_E_IO, "Cannot open file " + tmp);
+        out.write((const char*)&h, sizeof h);
+        out.write((const char*)hb.start.data(), (std::streamsize)(hb.start.size() * sizeof(int64_t)));
+        out.write(recs.data(), (std::streamsize)recs.size());
+        if (!out) throw Error(DSGDPP_E_IO, "Cannot write file " + tmp);
+    }
+    if (rename(tmp.c_str(), fn.c_str()) != 0) throw Error(DSGDPP_E_IO, "Cannot write file " + fn);
+}
+
+bool readCacheHeader(const std::string& fn, BlockCacheHeader& h) {
+    std::ifstream in(fn, std::ios::binary);
+    if (!in.is_open()) return false;
+    in.read((char*)&h, sizeof h);
+    return (bool)in && memcmp(h.magic, "DSGDBLK1", 8) == 0 && h.version == 1 && h.record_bytes == 16;
+}
+
+template <typename Pt>
+void checkCachedRecords(const dsgdpp_ctx* c, const std::vector<int64_t>& start, const std::vector<char>& raw,
+                        const std::string& fn) {
+    // every record inside its block and the block sorted row-major without
+    // repeats, as the blocking leaves it; anything else is not a cache file
+    const Pt* pts = (const Pt*)raw.data();
+    const uint32_t d2 = 2 * c->d;
+    for (size_t b = 0; b + 1 < start.size(); ++b) {
+        const int64_t r0 = c->off1[b / d2], r1 = c->off1[b / d2 + 1], q0 = c->off2[b % d2], q1 = c->off2[b % d2 + 1];
+        for (int64_t p = start[b]; p < start[b + 1]; ++p) {
+            const Pt& e = pts[p];
+            const bool in = e.i >= r0 && e.i < r1 && e.j >= q0 && e.j < q1;
+            const bool order = p == start[b] || e.i > pts[p - 1].i || (e.i == pts[p - 1].i && e.j > pts[p - 1].j);
+            if (!in || !order) throw Error(DSGDPP_E_IO, "Corrupt block cache " + fn);
+        }
+    }
+}
+
+void loadBlocks(dsgdpp_ctx* c, int which, const std::string& fn) {
+    BlockCacheHeader h;
+    if (!readCacheHeader(fn, h)) throw Error(DSGDPP_E_IO, "Not a block cache file: " + fn);
+    if (h.fp64 != (c->fp64() ? 1u : 0u) || h.algorithm != (uint32_t)c->algorithm || h.d != c->d)
+        throw Error(DSGDPP_E_ARG, "block cache " + fn + " was written for another blocking, algorithm or precision");
+    if (h.nnz < 0 || h.nnz >= (int64_t)UINT32_MAX) throw Error(DSGDPP_E_IO, "Corrupt block cache " + fn);
+    std::ifstream in(fn, std::ios::binary);
+    in.seekg(sizeof h);
+    const size_t nb = (size_t)c->d * 2 * c->d;
+    std::vector<int64_t> start(nb + 1);
+    in.read((char*)start.data(), (std::streamsize)(start.size() * sizeof(int64_t)));
+    std::vector<char> raw((size_t)h.nnz * 16);
+    in.read(raw.data(), (std::streamsize)raw.size());
+    if (!in) throw Error(DSGDPP_E_IO, "Unexpected EOF in block cache " + fn);
+    if (start[0] != 0 || start[nb] != h.nnz) throw Error(DSGDPP_E_IO, "Corrupt block cache " + fn);
+    for (size_t b = 0; b < nb; ++b)
+        if (start[b + 1] < start[b]) throw Error(DSGDPP_E_IO, "Corrupt block cache " + fn);
+    beginLoad(c, which, h.m, h.n);
+    if (c->fp64()) checkCachedRecords<Pt64>(c, start, raw, fn);
+    else checkCachedRecords<Pt32>(c, start, raw, fn);
+    void* pts = nullptr;
+    dalloc(pts, raw.size());
+    dfree(c->dPts[which]);
+    c->dPts[which] = pts;
+    if (!raw.empty()) HIPC(hipMemcpy(pts, raw.data(), raw.size(), hipMemcpyHostToDevice));
+    dalloc(c->dBlk[which], start.size() * sizeof(int64_t));
+    HIPC(hipMemcpy(c->dBlk[which], start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HostBlocks& hb = c->blocks[which];
+    hb.start.swap(start);
+    hb.nnz = h.nnz;
+    finishBlocks(c, which);
+}
+
+// load_mmc through the cache: the cache when it was written from this very
+// file (size and modification time) with this blocking, else the file, then
+// the cache (re)written
+bool loadMmcCached(dsgdpp_ctx* c, int which, const std::string& fn, const std::string& cache) {
+    int64_t size = 0, mtime = 0;
+    if (!fileStamp(fn, size, mtime)) throw Error(DSGDPP_E_IO, "Cannot open file " + fn);
+    BlockCacheHeader h;
+    if (readCacheHeader(cache, h) && h.src_size == size && h.src_mtime_ns == mtime &&
+        h.fp64 == (c->fp64() ? 1u : 0u) && h.algorithm == (uint32_t)c->algorithm && h.d == c->d) {
+        loadBlocks(c, which, cache);
+        return true;
+    }
+    loadMmc(c, which, fn);
+    saveBlocks(c, which, cache, size, mtime);
+    return false;
+}
+
+// ---------------------------------------------------------------------------
 // R trace (mf/trace.h:46-262)
 // ---------------------------------------------------------------------------
 struct TraceEntry {
@@ -1496,6 +1531,29 @@ int dsgdpp_load_mmc(dsgdpp_ctx* c, int32_t which, const char* path) {
     });
 }
 
+int dsgdpp_save_blocks(dsgdpp_ctx* c, int32_t which, const char* path) {
+    return guarded(c, [&] {
+        if (!path || !*path) throw Error(DSGDPP_E_ARG, "empty path");
+        saveBlocks(c, which, path, 0, 0);
+    });
+}
+
+int dsgdpp_load_blocks(dsgdpp_ctx* c, int32_t which, const char* path) {
+    return guarded(c, [&] {
+        if (!path || !*path) throw Error(DSGDPP_E_ARG, "empty path");
+        loadBlocks(c, which, path);
+    });
+}
+
+int dsgdpp_load_mmc_cached(dsgdpp_ctx* c, int32_t which, const char* path, const char* cache_path,
+                           int32_t* from_cache) {
+    return guarded(c, [&] {
+        if (!path || !cache_path || !*cache_path) throw Error(DSGDPP_E_ARG, "empty path");
+        const bool hit = loadMmcCached(c, which, path, cache_path);
+        if (from_cache) *from_cache = hit ? 1 : 0;
+    });
+}
+
 int dsgdpp_set_factors(dsgdpp_ctx* c, const double* W, const double* H) {
     return guarded(c, [&] {
         if (c->m < 0) throw Error(DSGDPP_E_STATE, "load the training matrix first");
@@ -1667,6 +1725,37 @@ int dsgdpp_block_nnz(dsgdpp_ctx* c, int32_t which, int64_t* out) {
                 }
         } else {
             for (size_t b = 0; b + 1 < s.size(); ++b) out[b] = s[b + 1] - s[b];
+        }
+    });
+}
+
+int dsgdpp_get_block(dsgdpp_ctx* c, int32_t which, int32_t b1, int32_t b2, int32_t* i, int32_t* j, double* x,
+                     int64_t cap, int64_t* n) {
+    return guarded(c, [&] {
+        if (which != 0 && which != 1) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+        if (!c->blocks[which].loaded) throw Error(DSGDPP_E_STATE, "matrix not loaded");
+        const int nb2 = c->dsgd() ? (int)c->d : 2 * (int)c->d;
+        if (b1 < 0 || b1 >= (int)c->d || b2 < 0 || b2 >= nb2) throw Error(DSGDPP_E_ARG, "block out of range");
+        const size_t b = (size_t)b1 * 2 * c->d + (size_t)(c->dsgd() ? 2 * b2 : b2);
+        const auto& st = c->blocks[which].start;
+        const int64_t k = st[b + 1] - st[b];
+        if (n) *n = k;
+        if (!i && !j && !x) return;
+        if (cap < k) throw Error(DSGDPP_E_ARG, "output buffer too small");
+        std::vector<unsigned char> raw((size_t)k * 16);
+        HIPC(hipStreamSynchronize(c->stream));
+        if (k > 0) HIPC(hipMemcpy(raw.data(), (const char*)c->dPts[which] + st[b] * 16, raw.size(), hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < k; ++p) {
+            const unsigned char* e = raw.data() + p * 16;
+            int32_t ii, jj;
+            memcpy(&ii, e, 4);
+            memcpy(&jj, e + 4, 4);
+            double v;
+            if (c->fp64()) memcpy(&v, e + 8, 8);
+            else { float f; memcpy(&f, e + 8, 4); v = f; }
+            if (i) i[p] = ii;
+            if (j) j[p] = jj;
+            if (x) x[p] = v;
         }
     });
 }

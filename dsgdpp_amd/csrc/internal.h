@@ -129,6 +129,36 @@ struct PrepDesc {
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 // grouped epoch eligibility; *wide (optional) is set when only the variant
 // that streams H rows from HBM fits (FP32 blocks too wide for the LDS tile)
+// ingest.hip: blocking on the device.  device_ingest uploads the COO triples,
+// sorts them by (block, row, column) and writes the deduplicated point records
+// (Pt32 or Pt64) and block starts into new device buffers (out.pts,
+// out.start_dev; the caller owns them); out.start is the host copy.
+// device_derive forms, for a training matrix, the 1/count arrays and counts
+// (new device buffers in d) and the FP32 records' row factor.
+// device_block_top gives each block's largest point count in one column.
+struct IngestGeom {
+    int64_t m, n;
+    uint32_t d;
+    bool dsgd;   // DSGD: d column blocks at the even positions of the 2d layout
+};
+struct IngestOut {
+    void* pts = nullptr;
+    int64_t* start_dev = nullptr;
+    std::vector<int64_t> start;
+    int64_t nnz = 0;
+};
+struct DerivedOut {
+    void* inv1 = nullptr;
+    void* inv2 = nullptr;
+    double* cnt1 = nullptr;
+    double* cnt2 = nullptr;
+};
+void device_ingest(const IngestGeom& G, bool pt64, bool train, int64_t nnz, const int32_t* I, const int32_t* J,
+                   const double* X, IngestOut& out, hipStream_t s);
+void device_derive(const DevProblem& P, bool train, void* pts, int64_t u, DerivedOut& d, hipStream_t s);
+void device_block_top(bool pt64, const void* pts, const int64_t* d_start, const int64_t* h_off2, int64_t nb2,
+                      int64_t nb, int max_cols, std::vector<int32_t>& top, hipStream_t s);
+
 // (residency is checked on the variant U selects)
 bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_block, int max_rows, int max_cols,
                        int device, std::string* why, bool* wide = nullptr, bool force_wide = false);

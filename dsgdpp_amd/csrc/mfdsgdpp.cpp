@@ -58,6 +58,8 @@ void usage(bool dsgd) {
             "  --device              HIP device ordinal [0]\n"
             "  --kernel              epoch kernel: 0 auto, 1 one launch per stratum, 2 dataflow,\n"
             "                        3 grouped, 4 grouped with H streamed from HBM\n"
+            "  --block-cache         directory of binary block caches: a matrix file's blocked form is\n"
+            "                        kept there and used while the file and the blocking are unchanged\n"
             "%s",
             dsgd ? "mfdsgd" : "mfdsgdpp",
             dsgd ? "not supported by the GPU path" : "rejected, as in the reference",
@@ -82,7 +84,7 @@ int main(int argc, char** argv) {
                                   "output-row-file", "output-col-file", "trace", "trace-var", "epochs",
                                   "tasks-per-rank", "sgd-order", "stratum-order", "seed", "rank",
                                   "update", "regularize", "loss", "truncate", "decay", "balance",
-                                  "precision", "device", "kernel"};
+                                  "precision", "device", "kernel", "block-cache"};
     for (int a = 1; a < argc; ++a) {
         std::string s = argv[a];
         if (s.rfind("--", 0) != 0) {
@@ -170,9 +172,21 @@ int main(int argc, char** argv) {
         fprintf(stderr, "%s\n", dsgdpp_create_error());
         return 1;
     }
-    int rc = dsgdpp_load_mmc(ctx, DSGDPP_TRAIN, opt["input-file"].c_str());
+    // with --block-cache DIR, DIR/<file name>.<d>.blk holds the blocked form of each matrix file
+    const std::string cacheDir = get("block-cache", "");
+    auto load = [&](int which, const std::string& file) {
+        if (cacheDir.empty()) return dsgdpp_load_mmc(ctx, which, file.c_str());
+        const size_t slash = file.find_last_of('/');
+        const std::string cache = cacheDir + "/" + (slash == std::string::npos ? file : file.substr(slash + 1)) +
+                                  "." + std::to_string(cfg.tasks_per_rank) + (dsgd ? ".dsgd" : "") + ".blk";
+        int32_t hit = 0;
+        const int r = dsgdpp_load_mmc_cached(ctx, which, file.c_str(), cache.c_str(), &hit);
+        if (r == 0) printf("%s %s\n", hit ? "Read block cache" : "Wrote block cache", cache.c_str());
+        return r;
+    };
+    int rc = load(DSGDPP_TRAIN, opt["input-file"]);
     const std::string testFile = get("input-test-file", "");
-    if (rc == 0 && !testFile.empty()) rc = dsgdpp_load_mmc(ctx, DSGDPP_TEST, testFile.c_str());
+    if (rc == 0 && !testFile.empty()) rc = load(DSGDPP_TEST, testFile);
     if (rc == 0) rc = dsgdpp_load_factors_mma(ctx, opt["input-row-file"].c_str(), opt["input-col-file"].c_str());
     const std::string trace = get("trace", ""), traceVar = get("trace-var", "trace"), decay = opt["decay"];
     if (rc == 0) {

@@ -120,6 +120,22 @@ int dsgdpp_load_coo(dsgdpp_ctx* ctx, int32_t which, int64_t m, int64_t n, int64_
  * (read_impl.h:46-116). */
 int dsgdpp_load_mmc(dsgdpp_ctx* ctx, int32_t which, const char* path);
 
+/* Binary block cache (no counterpart in the reference, which reads and
+ * blocks its text files on every run: read_impl.h:365-486).  The file holds a
+ * loaded matrix in the device layout -- blocked, sorted row-major, duplicates
+ * summed -- and the blocking it was made for.  dsgdpp_save_blocks writes
+ * matrix `which`; dsgdpp_load_blocks loads such a file in place of
+ * dsgdpp_load_coo / dsgdpp_load_mmc (E_ARG when it was written for another
+ * d, algorithm or record precision, E_IO when it is not a valid cache file).
+ * dsgdpp_load_mmc_cached loads `path` through `cache_path`: from the cache
+ * when it was written from this very file (size and modification time) with
+ * this blocking, else from the file, then (re)writes the cache; *from_cache
+ * (optional) tells which. */
+int dsgdpp_save_blocks(dsgdpp_ctx* ctx, int32_t which, const char* path);
+int dsgdpp_load_blocks(dsgdpp_ctx* ctx, int32_t which, const char* path);
+int dsgdpp_load_mmc_cached(dsgdpp_ctx* ctx, int32_t which, const char* path, const char* cache_path,
+                           int32_t* from_cache);
+
 /* Factors: W row-major m x r, H column-major r x n (column j's r values
  * contiguous), as DenseMatrix / DenseMatrixCM (mf/types.h:37-39). */
 int dsgdpp_set_factors(dsgdpp_ctx* ctx, const double* W, const double* H);
@@ -195,6 +211,13 @@ int dsgdpp_get_permutation(dsgdpp_ctx* ctx, int32_t task, int32_t subepoch, uint
  * ("two-slot groups in K of N blocks", or empty).  No counterpart in the
  * reference (a kernel-choice query). */
 int dsgdpp_grouped_ok(dsgdpp_ctx* ctx, int32_t* ok, char* why, int64_t cap);
+/* One block's points as stored on the device (global 0-based indices, row-major,
+ * duplicates summed; x as stored: FP32 modes round it to float); *n gets the
+ * count (call with null arrays to size them).  A parity hook of the blocking
+ * (ReadMatrixBlocksMm + sort, read_impl.h:365-486, :163); b2 is DSGD's
+ * column block for DSGD. */
+int dsgdpp_get_block(dsgdpp_ctx* ctx, int32_t which, int32_t b1, int32_t b2, int32_t* i, int32_t* j, double* x,
+                     int64_t cap, int64_t* n);
 /* nnz per block (d x 2d, row-major; DSGD: d x d) and dims {m, n, r, d, nnz_train, nnz_test} */
 int dsgdpp_block_nnz(dsgdpp_ctx* ctx, int32_t which, int64_t* out);
 int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
