@@ -160,6 +160,17 @@ __device__ __forceinline__ void dataflow_signal_wave(int32_t* ready, int b2) {
     if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(ready + b2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Release of a handoff whose data went out with agent-coherent (SC1) stores
+// (the staged H of the grouped epoch kernels): once those stores are complete
+// (vmcnt 0) the data is at the agent's coherence point, so the counter may
+// move without the L2 write-back of an agent-scope release fence.  The
+// workgroup fence keeps the compiler from moving the stores past the add.
+__device__ __forceinline__ void dataflow_signal_coherent_wave(int32_t* ready, int b2) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), expcnt and lgkmcnt unconstrained
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(ready + b2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void dataflow_signal(int32_t* ready, int b2) {
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_fetch_add(ready + b2, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

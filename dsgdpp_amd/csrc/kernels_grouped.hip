@@ -372,6 +372,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint3
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, RSRC_WORD3);
 }
 
+// Agent-coherent accesses (cache policy SC1).  A column block's H rows pass
+// from one task's workgroup to the next one's, which may run on another XCD
+// with its own L2.  Written and read with SC1 they go through the agent's
+// coherence point, so a handoff needs neither an L2 write-back at its release
+// nor an L2 invalidation at its acquire; both act on the whole L2 of the XCD
+// and would also write back and drop the W rows and group records that the
+// XCD's other tasks are working on.
+constexpr int CPOL_SC1 = 16;
+
 template <typename T>
 struct Vec4;
 template <>
@@ -387,6 +396,16 @@ struct Vec4<float> {
         u32x4 u;
         u.x = __float_as_uint(v.x); u.y = __float_as_uint(v.y); u.z = __float_as_uint(v.z); u.w = __float_as_uint(v.w);
         __builtin_amdgcn_raw_buffer_store_b128(u, r, (int)off, 0, 0);
+    }
+    // agent-coherent (SC1) forms, for H handed between workgroups
+    static __device__ __forceinline__ float4 cload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, CPOL_SC1);
+        return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
+    }
+    static __device__ __forceinline__ void cstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float4 v) {
+        u32x4 u;
+        u.x = __float_as_uint(v.x); u.y = __float_as_uint(v.y); u.z = __float_as_uint(v.z); u.w = __float_as_uint(v.w);
+        __builtin_amdgcn_raw_buffer_store_b128(u, r, (int)off, 0, CPOL_SC1);
     }
     static __device__ __forceinline__ float round(float v) { return v; }
     // a prefetched row as loaded, and its values at the time of use
@@ -435,6 +454,16 @@ struct Vec4<bf16_t> {
         u.x = pack(v.x, v.y);
         u.y = pack(v.z, v.w);
         __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)off, 0, 0);
+    }
+    static __device__ __forceinline__ float4 cload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, CPOL_SC1);
+        return unpack(u.x, u.y);
+    }
+    static __device__ __forceinline__ void cstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float4 v) {
+        u32x2 u;
+        u.x = pack(v.x, v.y);
+        u.y = pack(v.z, v.w);
+        __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)off, 0, CPOL_SC1);
     }
     // value the factor takes once stored
     static __device__ __forceinline__ float round(float v) { return (float)(__bf16)v; }
@@ -544,6 +573,7 @@ template <typename T, int KIND, int VEC>
 __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, float* lds) {
     constexpr int ROW = 64 * VEC;
     constexpr int CH = 8;                                  // float4s in flight per lane
+    constexpr uint32_t ES = Vec4<T>::BYTES;
     const int lane = threadIdx.x & 63;
     const int hb = hbuf_floats(A.max_cols, VEC);
     float* const hbuf0 = lds + GROUP_RING * GROUP_SLOTS * ROW;
@@ -555,6 +585,7 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
     auto write_back = [&]() {   // visit k - 1, buffer (k - 1) & 1
         const float* hbp = hbuf0 + ((k - 1) & 1) * hb;
         const int total = pend_ncol * 16 * VEC;
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)pend_col0 * A.ld, (uint32_t)(pend_ncol * A.ld) * ES);
         for (int base = 0; base < total; base += 64 * CH) {
             float4 v[CH];
 #pragma unroll
@@ -565,10 +596,10 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const int e = base + u * 64 + lane, c = e / (16 * VEC), q = e % (16 * VEC);
-                if (e < total && 4 * q < A.ld) Vec4<T>::store(A.H + (int64_t)(pend_col0 + c) * A.ld + 4 * q, v[u]);
+                if (e < total && 4 * q < A.ld) Vec4<T>::cstore(Hr, ((uint32_t)c * A.ld + 4 * q) * ES, v[u]);
             }
         }
-        dataflow_signal_wave(A.ready, pend_b2);
+        dataflow_signal_coherent_wave(A.ready, pend_b2);
         pend_b2 = -1;
     };
     for (int s = 0; s < 2 * d; ++s) {
@@ -594,7 +625,8 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // H is read with coherent loads: ordering them after the counter is enough
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (A.prof) c_df += clock64() - c0;
         float* const hbp = hbuf0 + (k & 1) * hb;
         // the first 128 columns' 1/nnz are loaded before the H rows, so the two
@@ -605,12 +637,13 @@ __device__ __forceinline__ void grouped_mover(const GArgs<T>& A, const int t, fl
             if (lane + 64 < ncol) inv_b = A.inv2[col0 + lane + 64];
         }
         const int total = ncol * 16 * VEC;
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * ES);
         for (int base = 0; base < total; base += 64 * CH) {
             float4 v[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const int e = base + u * 64 + lane, c = e / (16 * VEC), q = e % (16 * VEC);
-                v[u] = (e < total && 4 * q < A.ld) ? Vec4<T>::load(A.H + (int64_t)(col0 + c) * A.ld + 4 * q)
+                v[u] = (e < total && 4 * q < A.ld) ? Vec4<T>::cload(Hr, ((uint32_t)c * A.ld + 4 * q) * ES)
                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
@@ -1076,6 +1109,7 @@ __device__ __forceinline__ void grouped_mover_f64(const GArgs64& A, const int t,
     auto write_back = [&]() {   // visit k - 1, buffer (k - 1) & 1
         const char* hbp = hbuf0 + ((k - 1) & 1) * hb;
         const int total = pend_ncol * 16;
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)pend_col0 * A.ld, (uint32_t)(pend_ncol * A.ld) * 8u);
         for (int base = 0; base < total; base += 64 * CH) {
             D4 v[CH];
 #pragma unroll
@@ -1087,13 +1121,14 @@ __device__ __forceinline__ void grouped_mover_f64(const GArgs64& A, const int t,
             for (int u = 0; u < CH; ++u) {
                 const int e = base + u * 64 + lane, c = e >> 4, q = e & 15;
                 if (e < total && 4 * q < A.ld) {
-                    uint4* dst = (uint4*)(A.H + (int64_t)(pend_col0 + c) * A.ld + 4 * q);
-                    dst[0] = lo2(v[u].x, v[u].y);
-                    dst[1] = lo2(v[u].z, v[u].w);
+                    const uint32_t off = ((uint32_t)c * A.ld + 4 * q) * 8u;
+                    const uint4 a = lo2(v[u].x, v[u].y), b = lo2(v[u].z, v[u].w);
+                    __builtin_amdgcn_raw_buffer_store_b128(u32x4{a.x, a.y, a.z, a.w}, Hr, (int)off, 0, CPOL_SC1);
+                    __builtin_amdgcn_raw_buffer_store_b128(u32x4{b.x, b.y, b.z, b.w}, Hr, (int)(off + 16), 0, CPOL_SC1);
                 }
             }
         }
-        dataflow_signal_wave(A.ready, pend_b2);
+        dataflow_signal_coherent_wave(A.ready, pend_b2);
         pend_b2 = -1;
     };
     for (int s = 0; s < 2 * d; ++s) {
@@ -1118,18 +1153,22 @@ __device__ __forceinline__ void grouped_mover_f64(const GArgs64& A, const int t,
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // H is read with coherent loads: ordering them after the counter is enough
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (A.prof) c_df += clock64() - c0;
         char* const hbp = hbuf0 + (k & 1) * hb;
         const int total = ncol * 16;
+        const __amdgpu_buffer_rsrc_t Hr = make_rsrc(A.H + (int64_t)col0 * A.ld, (uint32_t)(ncol * A.ld) * 8u);
         for (int base = 0; base < total; base += 64 * CH) {
             D4 v[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const int e = base + u * 64 + lane, c = e >> 4, q = e & 15;
                 if (e < total && 4 * q < A.ld) {
-                    const uint4* src = (const uint4*)(A.H + (int64_t)(col0 + c) * A.ld + 4 * q);
-                    v[u] = d4_of(src[0], src[1]);
+                    const uint32_t off = ((uint32_t)c * A.ld + 4 * q) * 8u;
+                    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(Hr, (int)off, 0, CPOL_SC1);
+                    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(Hr, (int)(off + 16), 0, CPOL_SC1);
+                    v[u] = d4_of(make_uint4(a.x, a.y, a.z, a.w), make_uint4(b.x, b.y, b.z, b.w));
                 } else {
                     v[u] = D4{0., 0., 0., 0.};
                 }
