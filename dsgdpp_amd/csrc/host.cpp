@@ -160,12 +160,13 @@ struct HostBlocks {
 
 // Everything one epoch needs besides eps and the factors.
 struct EpochPlan {
-    bool drawn = false, built = false, grouped = false, dataflow = false;
+    bool drawn = false, built = false, grouped = false, dataflow = false, level = false;
     uint64_t dataVersion = 0;
     std::vector<uint32_t> schedule;   // 2d x d
     std::vector<uint64_t> seeds;      // d
     std::vector<int64_t> voff;        // 2d x d
     std::vector<int64_t> goff;        // 2d x d: first group of each visit (grouped epoch)
+    std::vector<int64_t> loff;        // 2d x d: first level start of each visit (level epoch)
     std::vector<int32_t> need;        // 2d x d
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
@@ -181,8 +182,10 @@ struct EpochPlan {
     uint32_t* perm = nullptr;
     Pt32* grec = nullptr;
     int32_t* ngroups = nullptr;
+    LevelBuffers LB;                  // level epoch: records by level, level starts, prep scratch
+    int64_t* dLoff = nullptr;
     size_t jobsCap = 0, chainsCap = 0;
-    int64_t permCap = 0, groupCap = 0;
+    int64_t permCap = 0, groupCap = 0, levelCap = 0;
     hipEvent_t evStart = nullptr, evEnd = nullptr;
 };
 
@@ -227,6 +230,15 @@ struct dsgdpp_ctx {
     bool dataflowOk = false;
     bool groupedOk = false;
     std::string groupedWhy;
+    // level epoch (kernel 5): chosen at load for blocks with few points per
+    // row and column, or forced; never inside a batch (batches run the
+    // grouped kernel)
+    bool levelOk = false;
+    std::string levelWhy;
+    LevelIndex LX;
+    int levelK = 1;                // workgroups per task
+    int32_t* dLevelBar = nullptr;  // d level barrier counters
+    int batches = 0;
     GroupedBuffers G;
     hipStream_t prepStream = nullptr;
     EpochPlan plan[2];
@@ -355,7 +367,7 @@ void configure(dsgdpp_ctx* c, const dsgdpp_cfg* cfg) {
     c->precision = cfg->precision;
     c->device = cfg->device;
     c->kernel = cfg->kernel;
-    if (c->kernel < 0 || c->kernel > 4) throw Error(DSGDPP_E_ARG, "kernel must be 0..4");
+    if (c->kernel < 0 || c->kernel > 5) throw Error(DSGDPP_E_ARG, "kernel must be 0..5");
     if (cfg->algorithm != DSGDPP_ALGO_DSGDPP && cfg->algorithm != DSGDPP_ALGO_DSGD)
         throw Error(DSGDPP_E_ARG, "algorithm must be DSGDPP_ALGO_DSGDPP or DSGDPP_ALGO_DSGD");
     c->algorithm = cfg->algorithm;
@@ -403,6 +415,69 @@ void choosePairs(dsgdpp_ctx* c) {
     dalloc(c->dPairs, c->pairBlk.size());
     HIPC(hipMemcpy(c->dPairs, c->pairBlk.data(), c->pairBlk.size(), hipMemcpyHostToDevice));
     c->G.pairs = c->dPairs;
+}
+
+// Level epoch (kernel 5): a block runs a level at a time, one lane per point
+// (kernels_level.hip), which pays when its rows and columns hold few points
+// each.  Auto (kernel 0) takes it when the mean non-empty block holds at least
+// LEVEL_MIN_MEAN points and LEVEL_DEPTH_RATIO times the longest row or column
+// run of any block -- a lower bound of that block's level count.  A level
+// costs a few microseconds whatever its size (a barrier of the task's
+// workgroups and a round trip of the factor rows), a four-slot group of the
+// grouped kernel about 0.12 us, so a block of n points and D levels is faster
+// level by level when n / D is some hundreds or more.
+constexpr double LEVEL_MIN_MEAN = 2048;
+constexpr double LEVEL_DEPTH_RATIO = 256;
+
+void freeLevelIndex(dsgdpp_ctx* c) {
+    void* q;
+    q = c->LX.corder; dfree(q);
+    q = c->LX.cinfo; dfree(q);
+    q = c->LX.rinfo; dfree(q);
+    c->LX = LevelIndex{};
+}
+
+void chooseLevel(dsgdpp_ctx* c) {
+    freeLevelIndex(c);
+    c->levelOk = false;
+    c->levelWhy.clear();
+    if (c->kernel != 0 && c->kernel != 5) return;
+    auto refuse = [&](const std::string& why) {
+        c->levelWhy = why;
+        if (c->kernel == 5) throw Error(DSGDPP_E_UNSUPPORTED, "level kernel: " + why);
+    };
+    const std::vector<int64_t>& start = c->blocks[0].start;
+    const int64_t nb = (int64_t)start.size() - 1;
+    int64_t nonempty = 0;
+    for (int64_t b = 0; b < nb; ++b) nonempty += start[b + 1] > start[b] ? 1 : 0;
+    const double mean = nonempty ? (double)c->blocks[0].nnz / (double)nonempty : 0.;
+    std::string why;
+    if (!level_supported(c->P, c->sgdOrder, &why)) return refuse(why);
+    const int K = level_tasks_width(c->P, c->upd, c->device);
+    if (K < 1) return refuse("its d workgroups do not all fit on the device at once");
+    if (c->kernel == 0 && mean < LEVEL_MIN_MEAN) return refuse("blocks too small");
+    std::vector<int32_t> col0(c->off2.begin(), c->off2.end());
+    int32_t* d0 = nullptr;
+    dalloc(d0, col0.size() * sizeof(int32_t));
+    std::vector<int32_t> runmax;
+    try {
+        HIPC(hipMemcpy(d0, col0.data(), col0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        build_level_index(c->P, d0, nb, (int)(2 * c->d), c->LX, runmax, c->stream);
+    } catch (...) {
+        void* q = d0;
+        dfree(q);
+        freeLevelIndex(c);
+        throw;
+    }
+    { void* q = d0; dfree(q); }
+    int32_t deep = 0;
+    for (int32_t v : runmax) deep = std::max(deep, v);
+    if (c->kernel == 0 && (double)deep * LEVEL_DEPTH_RATIO > mean) {
+        freeLevelIndex(c);
+        return refuse("a row or column holds too many of a block's points");
+    }
+    c->levelK = K;
+    c->levelOk = true;
 }
 
 // The derived state of a loaded matrix whose point records and block starts
@@ -457,6 +532,7 @@ void finishBlocks(dsgdpp_ctx* c, int which) {
         c->G.off2 = d2p;
     }
     if (c->kernel == 3 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "grouped kernel: " + c->groupedWhy);
+    chooseLevel(c);
 }
 
 void uploadBlocks(dsgdpp_ctx* c, int which, int64_t nnz, const int32_t* I, const int32_t* J, const double* X) {
@@ -658,22 +734,25 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
     // a WR stratum order may send a task to the same block twice in one epoch
     P.voff.assign((size_t)d2 * d, 0);
     P.goff.assign((size_t)d2 * d, 0);
-    int64_t total = 0, gtotal = 0;
+    P.loff.assign((size_t)d2 * d, 0);
+    int64_t total = 0, gtotal = 0, ordinal = 0;
     for (uint32_t id = 0; id < d; ++id)
         for (uint32_t sub = 0; sub < d2; ++sub) {
             const size_t b = visitBlock(id, sub);
             const int64_t nb = bs[b + 1] - bs[b];
             P.voff[(size_t)sub * d + id] = total;
             P.goff[(size_t)sub * d + id] = gtotal;
+            P.loff[(size_t)sub * d + id] = total + ordinal++;   // a visit's level starts: at most nb + 1
             total += nb;
             gtotal += (2 * nb + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;   // groups < 2 points, padded
         }
-    P.grouped = c->groupedOk && (c->kernel == 0 || c->kernel == 3 || c->kernel == 4);
-    P.dataflow = P.grouped || (c->kernel != 1 && c->dataflowOk);
+    P.level = c->levelOk && c->batches == 0;
+    P.grouped = !P.level && c->groupedOk && (c->kernel == 0 || c->kernel == 3 || c->kernel == 4);
+    P.dataflow = P.level || P.grouped || (c->kernel != 1 && c->dataflowOk);
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
     int64_t maxBlock = 0;
-    if (c->sgdOrder != DSGDPP_SGD_SEQ || P.grouped) {
+    if (c->sgdOrder != DSGDPP_SGD_SEQ || P.grouped || P.level) {
         // each task's stream is consumed block after block in sub-epoch order
         // (one SgdRunner per task per epoch, :112)
         std::vector<std::vector<PermJob>> perTask(d);
@@ -750,6 +829,18 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
             dalloc(P.grec, (size_t)gtotal * GROUP_SLOTS * sizeof(Pt32));
         }
         if (P.grouped && !P.ngroups) dalloc(P.ngroups, (size_t)d2 * d * sizeof(int32_t));
+        if (P.level && total > P.levelCap) {
+            P.levelCap = total;
+            dalloc(P.LB.lrec, (size_t)total * sizeof(uint4));
+            dalloc(P.LB.lstart, (size_t)(total + (int64_t)d2 * d) * sizeof(int32_t));
+            dalloc(P.LB.pos, (size_t)total * sizeof(uint32_t));
+            dalloc(P.LB.prv, (size_t)total * 2 * sizeof(int32_t));
+            dalloc(P.LB.lev, (size_t)total * sizeof(int32_t));
+            for (uint32_t*& f : P.LB.fy) dalloc(f, (size_t)(total + (int64_t)d2 * d) * sizeof(uint32_t));
+        }
+        if (P.level && !P.LB.nlev) dalloc(P.LB.nlev, (size_t)d2 * d * sizeof(int32_t));
+        if (P.level && !P.dLoff) dalloc(P.dLoff, (size_t)d2 * d * sizeof(int64_t));
+        P.LB.loff = P.dLoff;
     }
     if (jobs.size() > P.jobsCap) {
         P.jobsCap = jobs.size() * 2;
@@ -783,6 +874,8 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
     HIPC(hipMemcpyAsync(P.dVoff, P.voff.data(), P.voff.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (P.grouped)
         HIPC(hipMemcpyAsync(P.dGoff, P.goff.data(), P.goff.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (P.level)
+        HIPC(hipMemcpyAsync(P.dLoff, P.loff.data(), P.loff.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
     if (P.dataflow)
         HIPC(hipMemcpyAsync(P.dNeed, P.need.data(), P.need.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (!P.jobs.empty())
@@ -812,7 +905,12 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
         P.dataVersion = c->dataVersion;
         return;
     }
-    if (P.grouped) {
+    if (P.level) {
+        // the visit orders, then each visit's levels
+        if (c->sgdOrder == DSGDPP_SGD_WOR) launch_permute_par(P.dJobs, P.dChains, nchains, P.dLoff, P.perm, P.LB, s);
+        launch_level_prep(Dp, c->LX, P.LB, P.dJobs, (int)P.jobs.size(),
+                          c->sgdOrder == DSGDPP_SGD_WOR ? P.perm : nullptr, s);
+    } else if (P.grouped) {
         // blocks whose prep state does not fit in LDS keep it in a scratch buffer
         const size_t spill = std::max(prep_spill_bytes(P.capSmall, G.max_rows, G.max_cols, P.nSmall),
                                       prep_spill_bytes(maxBlock, G.max_rows, G.max_cols, nchains - P.nSmall));
@@ -868,10 +966,24 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     }
     DevProblem Dp = c->P;
     Dp.perm = P.perm;
-    if (P.grouped) {
+    if (P.level) {
+        if (!c->dLevelBar) dalloc(c->dLevelBar, (size_t)d * sizeof(int32_t));
+        LevelBuffers LB = P.LB;
+        LB.bar = c->dLevelBar;
+        LB.K = c->levelK;
+        if (c->kprof) {
+            if (!c->G.prof) {
+                unsigned long long* q = nullptr;
+                dalloc(q, (size_t)d * 5 * sizeof(unsigned long long));
+                c->G.prof = q;
+            }
+            LB.prof = c->G.prof;
+        }
+        launch_epoch_levels(Dp, c->upd, LB, L, c->stream);
+    } else if (P.grouped) {
         GroupedBuffers G = c->G;
         G.grec = P.grec;
-            G.ngroups = P.ngroups;
+        G.ngroups = P.ngroups;
         if (c->kprof && !c->G.prof) {
             unsigned long long* q = nullptr;
             dalloc(q, (size_t)d * 5 * sizeof(unsigned long long));
@@ -908,6 +1020,16 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
         fprintf(stderr, "[kprof] prep clocks/visit: visit order %.3g scheduling %.3g fill %.3g (%llu visits)\n",
                 h[0] / v, h[1] / v, h[2] / v, h[3]);
     }
+    if (c->kprof && P.level && c->G.prof) {
+        // wall-clock sums of the level epoch kernel (first workgroup of each task)
+        std::vector<unsigned long long> h((size_t)d * 4);
+        HIPC(hipMemcpy(h.data(), c->G.prof, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        double w = 0, l = 0, b = 0, n = 0;
+        for (size_t t = 0; t < d; ++t) { w += h[4 * t]; l += h[4 * t + 1]; b += h[4 * t + 2]; n += h[4 * t + 3]; }
+        const double us = 1e3 / std::max(1, c->wallKhz);
+        fprintf(stderr, "[kprof] level epoch, mean per task: dataflow wait %.1f us, levels %.1f us, barriers %.1f us, "
+                        "%.0f levels (K = %d workgroups per task)\n", w / d * us, l / d * us, b / d * us, n / d, c->levelK);
+    }
     if (c->kprof && P.grouped && c->G.prof) {
         // clock counters of the grouped epoch kernel: the slowest task and the mean
         std::vector<unsigned long long> h((size_t)d * 5);
@@ -934,7 +1056,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
         st->device_seconds = b / 1e3;
         st->updates = c->blocks[0].nnz;
         st->launches = P.dataflow ? 1 : d2;
-        st->kernel = P.grouped ? (c->G.wide ? 4 : 3) : P.dataflow ? 2 : 1;
+        st->kernel = P.level ? 5 : P.grouped ? (c->G.wide ? 4 : 3) : P.dataflow ? 2 : 1;
     }
 }
 
@@ -1491,9 +1613,19 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
         q = P.perm; dfree(q);
         q = P.grec; dfree(q);
         q = P.ngroups; dfree(q);
+        q = P.LB.lrec; dfree(q);
+        q = P.LB.lstart; dfree(q);
+        q = P.LB.pos; dfree(q);
+        q = P.LB.prv; dfree(q);
+        q = P.LB.lev; dfree(q);
+        q = P.LB.nlev; dfree(q);
+        for (uint32_t* f : P.LB.fy) { q = f; dfree(q); }
+        q = P.dLoff; dfree(q);
         if (P.evStart) (void)hipEventDestroy(P.evStart);
         if (P.evEnd) (void)hipEventDestroy(P.evEnd);
     }
+    freeLevelIndex(c);
+    { void* q = c->dLevelBar; dfree(q); }
     dfree(c->P.W);
     dfree(c->P.H);
     void* v;
@@ -1808,6 +1940,7 @@ struct dsgdpp_batch {
     int32_t* dStatus = nullptr;
     int32_t* hStatus = nullptr;   // pinned
     int wallKhz = 0;              // wall-clock ticks per millisecond
+    bool counted = false;         // its contexts' batch counts include it
     std::string err;
 };
 
@@ -1817,6 +1950,8 @@ thread_local std::string g_batchCreateError;
 
 void batchDestroy(dsgdpp_batch* b) {
     if (!b) return;
+    if (b->counted)
+        for (dsgdpp_ctx* c : b->ctx) --c->batches;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -1852,7 +1987,7 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         requireReady(c);
         EpochPlan& P = c->plan[c->cur];
         if (!P.drawn) drawPlan(c, P);
-        if (!P.built || P.dataVersion != c->dataVersion) {
+        if (!P.built || P.dataVersion != c->dataVersion || P.level) {   // a level plan predates the batch
             HIPC(hipStreamSynchronize(c->prepStream));
             buildPlan(c, P, c->prepStream);
         }
@@ -2027,6 +2162,8 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             }
             b->ctx.push_back(c);
         }
+        for (dsgdpp_ctx* c : b->ctx) ++c->batches;   // their plans now take the grouped kernel
+        b->counted = true;
         HIPC(hipSetDevice(b->device));
         b->useGraph = use_graph != 0;
         b->argBytes = grouped_args_bytes(b->precision);   // the kernel indexes the table by sizeof

@@ -88,6 +88,51 @@ struct EpochLaunch {
 void launch_epoch(const DevProblem& P, const UpdParams& U, const EpochLaunch& L, hipStream_t s);
 bool dataflow_fits(const DevProblem& P, int device);
 
+// kernels_level.hip: the level-parallel epoch (kernel 5) for blocks whose rows
+// and columns hold few points.  At load, a column index of every block
+// (corder: the block's points by column, cinfo: each point's column run in it)
+// and each point's row run (rinfo); per visit, a prep kernel gives every point
+// its level and writes the visit's records sorted by level; the epoch kernel
+// runs the levels in order, one lane per point.
+struct LevelIndex {
+    uint32_t* corder = nullptr;   // nnz: global point ids, by (block, column), stable
+    uint2* cinfo = nullptr;       // nnz: (first corder position of the point's column run, length)
+    uint2* rinfo = nullptr;       // nnz: (first point of the point's row run, length)
+};
+struct LevelBuffers {   // one epoch plan's
+    uint4* lrec = nullptr;        // records by level, a run per visit (at voff)
+    int32_t* lstart = nullptr;    // level starts, levels + 1 per visit (at loff)
+    const int64_t* loff = nullptr;   // 2d x d
+    int32_t* nlev = nullptr;      // 2d x d
+    uint32_t* pos = nullptr;      // prep scratch: one per visit entry
+    int32_t* prv = nullptr;       // prep scratch: two per visit entry
+    int32_t* lev = nullptr;       // prep scratch: one per visit entry
+    uint32_t* fy[4] = {nullptr, nullptr, nullptr, nullptr};   // visit-order scratch: points + visits each (at loff)
+    int32_t* bar = nullptr;       // d: level barrier counters of the epoch kernel (the context's)
+    int K = 1;                    // workgroups per task of the epoch kernel
+    unsigned long long* prof = nullptr;   // optional: 4 wall-clock sums per task (DSGDPP_KPROF)
+};
+// rank and point order the level kernel takes (why: the reason when not)
+bool level_supported(const DevProblem& P, int sgd_order, std::string* why);
+// workgroups per task (at most 16) such that all d of them per task are
+// resident at once; 0 when not even one per task is
+int level_tasks_width(const DevProblem& P, const UpdParams& U, int device);
+// bytes of a factor buffer of `rows` rows (with the spare tail rows)
+size_t level_factor_bytes(const DevProblem& P, int64_t rows);
+// builds the index (new device buffers in out; the caller owns them) and each
+// block's longest row or column run (runmax, d * 2d)
+void build_level_index(const DevProblem& P, const int32_t* d_col0, int64_t nb, int nb2, LevelIndex& out,
+                       std::vector<int32_t>& runmax, hipStream_t s);
+// WOR visit orders (SgdRunner::permute with carry-over) with every swap resolved
+// in parallel: one workgroup of LV_THREADS per chain of visits
+void launch_permute_par(const PermJob* d_jobs, const int32_t* d_chain_start, int nchains, const int64_t* d_loff,
+                        uint32_t* perm, const LevelBuffers& B, hipStream_t s);
+// perm: the visit orders (WOR), or null (SEQ: storage order)
+void launch_level_prep(const DevProblem& P, const LevelIndex& X, const LevelBuffers& B, const PermJob* d_jobs,
+                       int njobs, const uint32_t* perm, hipStream_t s);
+void launch_epoch_levels(const DevProblem& P, const UpdParams& U, const LevelBuffers& B, const EpochLaunch& L,
+                         hipStream_t s);
+
 // kernels_grouped.hip: the grouped epoch.  A prep kernel turns every visit
 // into its visit order and groups of up to GROUP_SLOTS independent points
 // (packed 16-byte records in (group, slot) order); the epoch kernel runs one

@@ -57,7 +57,7 @@ void usage(bool dsgd) {
             "  --precision           FP64_STRICT (default), FP64, FP32 or BF16\n"
             "  --device              HIP device ordinal [0]\n"
             "  --kernel              epoch kernel: 0 auto, 1 one launch per stratum, 2 dataflow,\n"
-            "                        3 grouped, 4 grouped with H streamed from HBM\n"
+            "                        3 grouped, 4 grouped with H streamed from HBM, 5 level-parallel\n"
             "  --block-cache         directory of binary block caches: a matrix file's blocked form is\n"
             "                        kept there and used while the file and the blocking are unchanged\n"
             "%s",

@@ -85,7 +85,11 @@ typedef struct dsgdpp_cfg {
                                 4 = grouped dataflow with H rows streamed from HBM (all precisions;
                                     chosen by 0 and 3 when a column block is too wide for the LDS
                                     tile).  Blocks whose prep state exceeds LDS keep it in global
-                                    scratch (any grouped kernel). */
+                                    scratch (any grouped kernel);
+                                5 = level dataflow: a block runs level by level, one lane per point
+                                    (ranks up to 16 in FP64 modes, 32 in FP32/BF16; WOR and SEQ point
+                                    orders); chosen by 0 when blocks are large and their rows and
+                                    columns hold few points each (the reference's quick-start shape). */
     int32_t algorithm;       /* DSGDPP_ALGO_DSGDPP (0, default) or DSGDPP_ALGO_DSGD: DsgdRunner
                                 (mf/sgd/dsgd_impl.h:81-234, 329-350) -- d x d blocks, d sub-epochs,
                                 one computeDsgdSchedule per epoch, on the same kernels */
@@ -99,7 +103,8 @@ typedef struct dsgdpp_epoch_stats {
     double sgd_seconds;      /* HIP-event time of the stratum kernels */
     int64_t updates;         /* points visited */
     int64_t launches;        /* stratum kernel launches */
-    int32_t kernel;          /* epoch kernel used: 1 per-stratum, 2 dataflow, 3 grouped dataflow, 4 grouped with H from HBM */
+    int32_t kernel;          /* epoch kernel used: 1 per-stratum, 2 dataflow, 3 grouped dataflow, 4 grouped with H from HBM,
+                                5 level dataflow */
     int32_t reserved;
 } dsgdpp_epoch_stats;
 

@@ -312,13 +312,15 @@ def test_grouped_fp64_wide_bitexact_vs_emulation(rank, update, truncate):
         np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
     g.close()
     if rank == 16:   # 2 x 4 blocks of 3,000 columns: the FP64 tile does not fit
+        # (kernel 3 asked: on this shape, sparse rows and columns, the automatic
+        # choice is the level kernel, tested in test_gpu_level.py)
         rng = np.random.default_rng(9)
         m, n, nnz = 6_000, 12_000, 60_000
         cells = np.unique(rng.integers(0, m * n, size=nnz + nnz // 10))[:nnz]
         rng.shuffle(cells)
         q = dict(m=m, n=n, train=(cells // n, cells % n, rng.normal(size=len(cells))), rank=rank,
                  W0=rng.uniform(-0.5, 0.5, (m, rank)), H0=rng.uniform(-0.5, 0.5, (rank, n)))
-        g, o = make_pair(q, update=update, tasks=2, seed=21, precision="FP64")
+        g, o = make_pair(q, update=update, tasks=2, seed=21, precision="FP64", kernel=3)
         o.set_arith("FP64_GROUPED")
         for e in range(2):
             st = g.epoch(0.01)
@@ -545,7 +547,9 @@ def test_grouped_prep_state_in_global_memory(m, n, nnz, kernel):
     rng.shuffle(cells)
     p = dict(m=m, n=n, train=(cells // n, cells % n, rng.normal(size=len(cells))), rank=16,
              W0=rng.uniform(-0.5, 0.5, (m, 16)), H0=rng.uniform(-0.5, 0.5, (16, n)))
-    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=2, seed=17, precision="FP32")
+    # kernel 3 asked (the grouped family: tile or streaming); on the sparse
+    # square shape the automatic choice would be the level kernel
+    g, o = make_pair(p, update="Nzsl_L2(0.05)", tasks=2, seed=17, precision="FP32", kernel=3)
     ok, why = g.grouped_ok()
     assert ok, why
     o.set_arith("FP32")
