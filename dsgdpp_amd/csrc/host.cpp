@@ -967,7 +967,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     DevProblem Dp = c->P;
     Dp.perm = P.perm;
     if (P.level) {
-        if (!c->dLevelBar) dalloc(c->dLevelBar, (size_t)d * sizeof(int32_t));
+        if (!c->dLevelBar) dalloc(c->dLevelBar, (size_t)4 * d * sizeof(int32_t));
         LevelBuffers LB = P.LB;
         LB.bar = c->dLevelBar;
         LB.K = c->levelK;
@@ -1022,13 +1022,16 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     }
     if (c->kprof && P.level && c->G.prof) {
         // wall-clock sums of the level epoch kernel (first workgroup of each task)
-        std::vector<unsigned long long> h((size_t)d * 4);
+        std::vector<unsigned long long> h((size_t)d * 5);
         HIPC(hipMemcpy(h.data(), c->G.prof, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        double w = 0, l = 0, b = 0, n = 0;
-        for (size_t t = 0; t < d; ++t) { w += h[4 * t]; l += h[4 * t + 1]; b += h[4 * t + 2]; n += h[4 * t + 3]; }
+        double w = 0, l = 0, b = 0, n = 0, loc = 0;
+        for (size_t t = 0; t < d; ++t) {
+            w += h[5 * t]; l += h[5 * t + 1]; b += h[5 * t + 2]; n += h[5 * t + 3]; loc += h[5 * t + 4];
+        }
         const double us = 1e3 / std::max(1, c->wallKhz);
         fprintf(stderr, "[kprof] level epoch, mean per task: dataflow wait %.1f us, levels %.1f us, barriers %.1f us, "
-                        "%.0f levels (K = %d workgroups per task)\n", w / d * us, l / d * us, b / d * us, n / d, c->levelK);
+                        "%.0f levels (K = %d workgroups per task, %.0f of %u tasks on one XCD)\n", w / d * us, l / d * us,
+                b / d * us, n / d, c->levelK, loc, d);
     }
     if (c->kprof && P.grouped && c->G.prof) {
         // clock counters of the grouped epoch kernel: the slowest task and the mean

@@ -40,7 +40,7 @@ namespace dsgdpp {
 
 constexpr int LV_THREADS = 1024;   // threads of a prep or epoch workgroup
 constexpr int LV_HIST = 8192;      // levels counted per pass of the prep's LDS histogram
-constexpr int LV_MAX_K = 16;       // workgroups per task of the epoch kernel
+constexpr int LV_MAX_K = 16;       // workgroups per task of the epoch kernel (half the CUs of an XCD: the look-ahead prep runs beside it)
 static_assert(LV_HIST % LV_THREADS == 0, "every thread scans the same number of counters");
 
 namespace {
@@ -453,8 +453,8 @@ struct LvArgs {
     const int32_t* need;
     int32_t* ready;
     int32_t* status;
-    int32_t* bar;       // d: the level barrier counter of each task
-    unsigned long long* prof;   // optional: per task {dataflow wait, levels, barriers, level count} (clocks)
+    int32_t* bar;       // 4d: per task the start-up counter, the level counters of the two modes, its XCD mask
+    unsigned long long* prof;   // optional: per task {dataflow wait, levels, barriers, level count, LOCAL}
     T* W;
     T* H;
     const void* inv1;   // FP64 modes: 1/nnz of every row (double); FP32/BF16 records carry it
@@ -465,16 +465,23 @@ struct LvArgs {
     double eps, lambda, tmin, tmax;
 };
 
-constexpr int LV_SC1 = 16;   // cache policy bits of an agent-coherent buffer access
+// Cache policies of the factor accesses: LV_SC1 (agent scope) reaches the
+// device's coherence point, so workgroups on different XCDs see each other's
+// rows; LV_SC0 (workgroup scope) passes the CU's L1 and stops at the XCD's L2,
+// which is coherent for workgroups on one XCD.
+constexpr int LV_SC1 = 16;
+constexpr int LV_SC0 = 1;
 typedef uint32_t lv_u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t lv_rsrc(const void* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
+template <int CP>
 __device__ __forceinline__ lv_u32x4 lv_cload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, LV_SC1);
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, CP);
 }
+template <int CP>
 __device__ __forceinline__ void lv_cstore(__amdgpu_buffer_rsrc_t r, uint32_t off, lv_u32x4 v) {
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, LV_SC1);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, CP);
 }
 __device__ __forceinline__ double lv_d(uint32_t lo, uint32_t hi) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
@@ -482,15 +489,15 @@ __device__ __forceinline__ double lv_d(uint32_t lo, uint32_t hi) {
 
 // a factor row of up to R elements in registers, as the compute type, through
 // coherent 16-byte accesses at byte offset `off` (ld is a multiple of 16 bytes)
-template <typename T, int R>
+template <typename T, int R, int CP>
 struct LvRow;
-template <int R>
-struct LvRow<double, R> {
+template <int R, int CP>
+struct LvRow<double, R, CP> {
     static __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, uint32_t off, int ld, double* v) {
 #pragma unroll
         for (int c = 0; c < R / 2; ++c) {
             lv_u32x4 u = {0u, 0u, 0u, 0u};
-            if (2 * c < ld) u = lv_cload(rs, off + 16u * c);
+            if (2 * c < ld) u = lv_cload<CP>(rs, off + 16u * c);
             v[2 * c] = lv_d(u.x, u.y);
             v[2 * c + 1] = lv_d(u.z, u.w);
         }
@@ -501,18 +508,18 @@ struct LvRow<double, R> {
             if (2 * c < ld) {
                 const uint64_t a = (uint64_t)__double_as_longlong(v[2 * c]), b = (uint64_t)__double_as_longlong(v[2 * c + 1]);
                 const lv_u32x4 u = {(uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32)};
-                lv_cstore(rs, off + 16u * c, u);
+                lv_cstore<CP>(rs, off + 16u * c, u);
             }
     }
     static __device__ __forceinline__ double round(double x) { return x; }
 };
-template <int R>
-struct LvRow<float, R> {
+template <int R, int CP>
+struct LvRow<float, R, CP> {
     static __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, uint32_t off, int ld, float* v) {
 #pragma unroll
         for (int c = 0; c < R / 4; ++c) {
             lv_u32x4 u = {0u, 0u, 0u, 0u};
-            if (4 * c < ld) u = lv_cload(rs, off + 16u * c);
+            if (4 * c < ld) u = lv_cload<CP>(rs, off + 16u * c);
             v[4 * c] = __uint_as_float(u.x);
             v[4 * c + 1] = __uint_as_float(u.y);
             v[4 * c + 2] = __uint_as_float(u.z);
@@ -525,18 +532,18 @@ struct LvRow<float, R> {
             if (4 * c < ld) {
                 const lv_u32x4 u = {__float_as_uint(v[4 * c]), __float_as_uint(v[4 * c + 1]),
                                     __float_as_uint(v[4 * c + 2]), __float_as_uint(v[4 * c + 3])};
-                lv_cstore(rs, off + 16u * c, u);
+                lv_cstore<CP>(rs, off + 16u * c, u);
             }
     }
     static __device__ __forceinline__ float round(float x) { return x; }
 };
-template <int R>
-struct LvRow<bf16_t, R> {
+template <int R, int CP>
+struct LvRow<bf16_t, R, CP> {
     static __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, uint32_t off, int ld, float* v) {
 #pragma unroll
         for (int c = 0; c < R / 8; ++c) {
             lv_u32x4 u = {0u, 0u, 0u, 0u};
-            if (8 * c < ld) u = lv_cload(rs, off + 16u * c);
+            if (8 * c < ld) u = lv_cload<CP>(rs, off + 16u * c);
             const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -555,7 +562,7 @@ struct LvRow<bf16_t, R> {
                 for (int k = 0; k < 4; ++k)
                     w[k] = (__float_as_uint(v[8 * c + 2 * k]) >> 16) | (__float_as_uint(v[8 * c + 2 * k + 1]) & 0xffff0000u);
                 const lv_u32x4 u = {w[0], w[1], w[2], w[3]};
-                lv_cstore(rs, off + 16u * c, u);
+                lv_cstore<CP>(rs, off + 16u * c, u);
             }
     }
     // round to nearest even, as v_cvt_pk_bf16_f32 does in the grouped kernels
@@ -574,7 +581,7 @@ struct LvPoint<Pt32> {
 };
 
 // One SGD step of point e, in one lane.
-template <typename T, int KIND, bool TRUNC, bool STRICT, int R>
+template <typename T, int KIND, bool TRUNC, bool STRICT, int R, int CP>
 __device__ __forceinline__ void level_update(const LvArgs<T>& A, const typename PtOf<T>::type& e) {
     using C = typename St<T>::C;
     using PT = typename PtOf<T>::type;
@@ -583,8 +590,8 @@ __device__ __forceinline__ void level_update(const LvArgs<T>& A, const typename 
     const uint32_t rowb = (uint32_t)A.ld * (uint32_t)sizeof(T);
     const uint32_t wo = (uint32_t)e.i * rowb, ho = (uint32_t)e.j * rowb;
     C w[R], h[R];
-    LvRow<T, R>::load(Wr, wo, A.ld, w);
-    LvRow<T, R>::load(Hr, ho, A.ld, h);
+    LvRow<T, R, CP>::load(Wr, wo, A.ld, w);
+    LvRow<T, R, CP>::load(Hr, ho, A.ld, h);
 #pragma unroll
     for (int z = 0; z < R; ++z) {   // elements past the rank count as zeros (the grouped kernels' padding)
         w[z] = z < r ? w[z] : (C)0;
@@ -666,25 +673,27 @@ __device__ __forceinline__ void level_update(const LvArgs<T>& A, const typename 
             wn = (wn <= tmin) ? tmin + eps : ((wn >= tmax) ? tmax - eps : wn);
             hn = (hn <= tmin) ? tmin + eps : ((hn >= tmax) ? tmax - eps : hn);
         }
-        w[z] = LvRow<T, R>::round(wn);
-        h[z] = LvRow<T, R>::round(hn);
+        w[z] = LvRow<T, R, CP>::round(wn);
+        h[z] = LvRow<T, R, CP>::round(hn);
     }
     // the padding past the rank stores zeros, as the factor upload leaves it
-    LvRow<T, R>::store(Wr, wo, A.ld, w);
-    LvRow<T, R>::store(Hr, ho, A.ld, h);
+    LvRow<T, R, CP>::store(Wr, wo, A.ld, w);
+    LvRow<T, R, CP>::store(Hr, ho, A.ld, h);
 }
 
-// The task's K workgroups meet: each wave's coherent stores are complete, then
-// every workgroup adds one to the task's counter and waits for all K of this
-// round.  False when the wait gave up (WaitLimit).
+// The task's K workgroups meet: each wave's stores are complete, then every
+// workgroup adds one to the task's counter and waits for all K of this round.
+// (Global atomics execute at the device's coherence point whatever their
+// scope, so the counter is read there too, with agent-scope loads.)  False
+// when the wait gave up (WaitLimit).
 __device__ __forceinline__ bool lv_task_barrier(int32_t* bar, int32_t target, uint32_t wait_ticks, int32_t* status,
                                                 int32_t* gave_up) {
-    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's stores are at the coherence point
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's stores have landed
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        WaitLimit wl(wait_ticks, status);
         int32_t g = 0;
+        WaitLimit wl(wait_ticks, status);
+        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (wl.expired()) {
                 g = 1;
@@ -698,12 +707,18 @@ __device__ __forceinline__ bool lv_task_barrier(int32_t* bar, int32_t target, ui
     return *gave_up == 0;
 }
 
-template <typename T, int KIND, bool TRUNC, bool STRICT, int R>
-__global__ __launch_bounds__(LV_THREADS) void k_epoch_levels(LvArgs<T> A) {
+// The epoch of task t as seen by its k-th workgroup.  LOCAL: all K workgroups
+// of the task are on one XCD (factor rows through its L2, SC0); the column
+// blocks still pass between tasks on different XCDs, so a visit starts with an
+// agent-scope acquire (the L2 drops what may be stale) and ends with an
+// agent-scope release (the L2 writes back) before the signal.
+template <typename T, int KIND, bool TRUNC, bool STRICT, int R, bool LOCAL>
+__device__ __forceinline__ void level_task(const LvArgs<T>& A, const int t, const int k, int32_t* gave_up) {
     using PT = typename PtOf<T>::type;
-    __shared__ int32_t gave_up;
-    const int K = A.K, t = blockIdx.x / K, k = blockIdx.x % K, d = A.d, tid = threadIdx.x;
-    int32_t rounds = 0;   // task barriers passed
+    constexpr int CP = LOCAL ? LV_SC0 : LV_SC1;
+    const int K = A.K, d = A.d, tid = threadIdx.x;
+    int32_t* const bar = A.bar + (LOCAL ? 2 * d : d) + t;
+    int32_t rounds = 0;   // task barriers passed on this counter
     const bool prof = A.prof && k == 0 && tid == 0;
     unsigned long long c_wait = 0, c_lev = 0, c_bar = 0, n_lev = 0;
     for (int s = 0; s < 2 * d; ++s) {
@@ -713,8 +728,8 @@ __global__ __launch_bounds__(LV_THREADS) void k_epoch_levels(LvArgs<T> A) {
         if (A.blk[b + 1] == A.blk[b]) continue;
         const unsigned long long c0 = prof ? wall_clock64() : 0;
         if (tid == 0) {
-            // the previous users of the column block wrote H with coherent
-            // stores, complete before their signal: polling the counter is enough
+            // the previous users of the column block completed their stores
+            // (and, LOCAL, wrote their L2 back) before their signal
             WaitLimit wl(A.wait_ticks, A.status);
             int32_t g = 0;
             const int32_t need = A.need[visit];
@@ -725,10 +740,11 @@ __global__ __launch_bounds__(LV_THREADS) void k_epoch_levels(LvArgs<T> A) {
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            gave_up = g;
+            *gave_up = g;
         }
         __syncthreads();
-        if (gave_up) return;
+        if (*gave_up) return;
+        if constexpr (LOCAL) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (prof) c_wait += wall_clock64() - c0;
         const int nl = A.nlev[visit];
         const int32_t* const ls = A.lstart + A.loff[visit];
@@ -740,12 +756,11 @@ __global__ __launch_bounds__(LV_THREADS) void k_epoch_levels(LvArgs<T> A) {
                 const uint4 u = lr[q];
                 PT pt;
                 __builtin_memcpy(&pt, &u, sizeof(PT));
-                level_update<T, KIND, TRUNC, STRICT, R>(A, pt);
+                level_update<T, KIND, TRUNC, STRICT, R, CP>(A, pt);
             }
-            __builtin_amdgcn_s_waitcnt(0x0F70);
             const unsigned long long c2 = prof ? wall_clock64() : 0;
             // the level's stores before the next level's loads, in all K workgroups
-            if (!lv_task_barrier(A.bar + t, K * ++rounds, A.wait_ticks, A.status, &gave_up)) return;
+            if (!lv_task_barrier(bar, K * ++rounds, A.wait_ticks, A.status, gave_up)) return;
             if (prof) {
                 c_lev += c2 - c1;
                 c_bar += wall_clock64() - c2;
@@ -753,21 +768,57 @@ __global__ __launch_bounds__(LV_THREADS) void k_epoch_levels(LvArgs<T> A) {
             }
         }
         // every workgroup's stores of the visit are complete: one signal
-        if (k == 0 && tid == 0) __hip_atomic_fetch_add(A.ready + b2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == 0 && tid == 0) {
+            if constexpr (LOCAL) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(A.ready + b2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     if (prof) {
-        A.prof[4 * t + 0] = c_wait;
-        A.prof[4 * t + 1] = c_lev;
-        A.prof[4 * t + 2] = c_bar;
-        A.prof[4 * t + 3] = n_lev;
+        A.prof[5 * t + 0] = c_wait;
+        A.prof[5 * t + 1] = c_lev;
+        A.prof[5 * t + 2] = c_bar;
+        A.prof[5 * t + 3] = n_lev;
+        A.prof[5 * t + 4] = LOCAL ? 1 : 0;
     }
 }
+
+// Workgroup b runs on XCD b % 8 when the hardware deals workgroups round-robin
+// over the eight XCDs (checked at run time): the task slots of XCD x are
+// tasks x, x + 8, ...; a task's K workgroups are b = x + 8 (slot K + k).
+// Workgroups whose task does not exist leave at once.  Every workgroup notes
+// its XCD in its task's mask; after one agent-scope barrier the task runs
+// LOCAL when the mask has one bit.
+template <typename T, int KIND, bool TRUNC, bool STRICT, int R>
+__global__ __launch_bounds__(LV_THREADS) void k_epoch_levels(LvArgs<T> A) {
+    __shared__ int32_t gave_up;
+    const int x = blockIdx.x % 8, j = blockIdx.x / 8;
+    const int t = x + 8 * (j / A.K), k = j % A.K;
+    if (t >= A.d) return;
+    const int d = A.d;
+    if (threadIdx.x == 0) {
+        const uint32_t xcc = (uint32_t)__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 15u;   // HW_REG_XCC_ID
+        __hip_atomic_fetch_or((uint32_t*)A.bar + 3 * d + t, 1u << xcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the mark before the barrier's count
+    }
+    if (!lv_task_barrier(A.bar + t, A.K, A.wait_ticks, A.status, &gave_up)) return;
+    __shared__ uint32_t mask;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // every workgroup's mark after the barrier
+        mask = __hip_atomic_load((uint32_t*)A.bar + 3 * d + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if ((mask & (mask - 1u)) == 0u) level_task<T, KIND, TRUNC, STRICT, R, true>(A, t, k, &gave_up);
+    else level_task<T, KIND, TRUNC, STRICT, R, false>(A, t, k, &gave_up);
+}
+
+// workgroups of the epoch launch: eight XCD columns of ceil(d / 8) task slots of K
+int level_grid(int d, int K) { return 8 * ((d + 7) / 8) * K; }
 
 // the completion counters and the give-up word, cleared by a kernel (no memset
 // node in a captured graph)
 __global__ void k_level_reset(int32_t* ready, int n, int32_t* bar, int d, int32_t* status) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) ready[k] = 0;
-    for (int k = threadIdx.x; k < d; k += blockDim.x) bar[k] = 0;
+    for (int k = threadIdx.x; k < 4 * d; k += blockDim.x) bar[k] = 0;
     if (status && threadIdx.x == 0) *status = 0;
 }
 
@@ -833,7 +884,7 @@ void launch_levels_t(const DevProblem& P, const UpdParams& U, const LevelBuffers
     k_level_reset<<<1, 256, 0, s>>>(L.ready, 2 * P.d, B.bar, P.d, L.status);
     HIPK(hipGetLastError());
     void* args[] = {&A};
-    launch_dataflow(level_fn(P, U), dim3(P.d * B.K), dim3(LV_THREADS), args, 0, s, "k_epoch_levels");
+    launch_dataflow(level_fn(P, U), dim3(level_grid(P.d, B.K)), dim3(LV_THREADS), args, 0, s, "k_epoch_levels");
 }
 
 struct DevScratch {   // device scratch of one index build, freed on every exit
@@ -882,7 +933,8 @@ int level_tasks_width(const DevProblem& P, const UpdParams& U, int device) {
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, level_fn(P, U), LV_THREADS, 0) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    const int64_t fit = (int64_t)per_cu * cus / std::max(1, P.d);   // workgroups per task that are resident together
+    // the whole grid (level_grid) resident at once
+    const int64_t fit = (int64_t)per_cu * cus / level_grid(std::max(1, P.d), 1);
     return (int)std::min<int64_t>(fit, LV_MAX_K);
 }
 
