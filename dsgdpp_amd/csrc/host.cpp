@@ -2043,7 +2043,7 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     // set the LDS, and with it the occupancy, of every other problem's prep
     struct PrepClass {
         std::vector<int2> chains;
-        size_t lds = 0;
+        size_t lds = 0, order_lds = 0;
     };
     std::map<std::pair<int, int>, PrepClass> classes;
     for (size_t k = 0; k < K; ++k) {
@@ -2067,6 +2067,7 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         PrepClass& pc = classes[{prep_narrow(N.maxBlock) ? 1 : 0, (int)std::max<size_t>(1, PREP_LDS_PER_CU / l)}];
         for (int ch = 0; ch + 1 < (int)N.chains.size(); ++ch) pc.chains.push_back(make_int2(idx, ch));
         pc.lds = std::max(pc.lds, l);
+        pc.order_lds = std::max(pc.order_lds, prep_desc_order_lds(D));
         ahead.push_back(&N);
     }
     if (!ahead.empty()) {
@@ -2095,7 +2096,8 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
                                 b->prepStream));
         for (size_t q = 0; q < order.size(); ++q)
             launch_prep_grouped_multi(b->dDesc, b->dCmap + start[q], (int)order[q].second->chains.size(),
-                                      order[q].first.first == 1, order[q].second->lds, b->prepStream);
+                                      order[q].first.first == 1, order[q].second->lds, b->prepStream,
+                                      order[q].second->order_lds);
         for (EpochPlan* N : ahead) HIPC(hipEventRecord(N->evEnd, b->prepStream));
     }
     const auto t_ahead = std::chrono::steady_clock::now();

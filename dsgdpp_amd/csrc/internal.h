@@ -169,7 +169,8 @@ struct PrepDesc {
     int32_t* ngroups;
     int32_t order, cap, rows_cap, cols_cap;
     uint32_t dummy_row, dummy_col;
-    int32_t wide, unused;   // wide: H forwarding entries for the streaming kernel
+    int32_t wide;   // H forwarding entries for the streaming kernel
+    int32_t pre;    // WOR visit orders precomputed into perm (k_permute_lds_multi) before the prep
 };
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 // grouped epoch eligibility; *wide (optional) is set when only the variant
@@ -211,11 +212,15 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
                    int order, int64_t max_block);
 bool prep_narrow(int64_t max_block);
 size_t prep_desc_lds(const PrepDesc& D);
+// LDS of the visit-order kernel for the descriptor's chains (0: the prep draws them)
+size_t prep_desc_order_lds(const PrepDesc& D);
 // scratch the prep of `nchains` chains of blocks up to max_block points needs
 // in global memory (0 when its state fits in LDS)
 size_t prep_spill_bytes(int64_t max_block, int max_rows, int max_cols, int nchains);
+// order_lds > 0: first the visit orders of the chains whose descriptor has pre
+// set (narrow: 16-bit indices, as the prep's)
 void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,
-                               hipStream_t s);
+                               hipStream_t s, size_t order_lds);
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block,
                          hipStream_t s);

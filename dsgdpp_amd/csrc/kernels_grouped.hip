@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -89,10 +90,156 @@ __device__ __forceinline__ uint32_t lane_gather(uint32_t v, int src) {
 // the visit order and the row/column tags in 16 bits, which fits more
 // workgroups per CU.
 constexpr int64_t PREP_NARROW = 8000;    // tag ((group + 2) << 2 | slot) < 2^16 with group < 2 n
-static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow) {
+// pre: the visit orders come precomputed in perm (k_permute_lds), so the prep
+// keeps no visit-order array
+static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow, bool pre = false) {
     const size_t es = narrow ? 2 : 4;
     auto up4 = [](size_t b) { return (b + 3) & ~(size_t)3; };
-    return up4(cap * es) + up4((size_t)rows * es) + up4((size_t)cols * es) + ((2 * (size_t)cap + GROUP_PAD + 7) >> 3) * 4;
+    return (pre ? 0 : up4(cap * es)) + up4((size_t)rows * es) + up4((size_t)cols * es) +
+           ((2 * (size_t)cap + GROUP_PAD + 7) >> 3) * 4;
+}
+
+// ---------------------------------------------------------------------------
+// WOR visit orders for the grouped prep: one wave per chain of visits whose
+// orders carry over, every swap of a visit resolved at once in LDS.
+// SgdRunner::permute (sgd_impl.h:255-278): step s = 0 .. n-2 swaps positions s
+// and t_s = s + nextInt(n - s).  With all t_s drawn up front:
+//   - a_s, the value step s moves out of position s, is the value the last
+//     earlier step aimed at s moved there (a_{L(s)}), or the initial value at
+//     s; so a_s is the initial value at the end of the chain s, L(s), ...;
+//   - position s ends with a_s when t_s = s, else with a_{M(s)} (M(s): the
+//     last step before s aimed at t_s) or the initial value at t_s;
+//   - position n - 1 ends with what the last step aimed at it moved there.
+// The steps aimed at each position come from a counting sort by target, the
+// chains resolve by pointer doubling.
+// ---------------------------------------------------------------------------
+constexpr size_t FY_LDS_MAX = 96 * 1024;   // blocks needing more keep the prep's own visit order
+template <typename IX>
+__host__ __device__ inline size_t fy_lds(int64_t cap) { return (size_t)(cap + 1) * (2 * sizeof(IX) + 8); }
+
+template <typename IX>
+__device__ __forceinline__ void fy_chain(const PermJob* __restrict__ jobs, const int32_t* __restrict__ chain_start,
+                                         const int chain, uint32_t* __restrict__ perm, uint32_t* sm, const int cap) {
+    const int lane = threadIdx.x;
+    uint32_t* const start = sm;                 // per position: first of its steps in `steps`; then n + 1
+    uint32_t* const root = sm + (cap + 1);      // cursor of the counting sort, then L(s) and chain ends
+    IX* const tgt = (IX*)(sm + 2 * (cap + 1));  // per step: t_s
+    IX* const steps = tgt + (cap + 1);          // steps by target, ascending within a target
+    for (int jx = chain_start[chain]; jx < chain_start[chain + 1]; ++jx) {
+        const PermJob J = jobs[jx];
+        const int n = (int)J.n, ns = n - 1;
+        const uint32_t* const init = J.carry_base >= 0 ? perm + J.carry_base : nullptr;   // else the identity
+        uint32_t* const out = perm + J.base;
+        auto init_at = [&](int v) -> uint32_t { return init ? init[v] : (uint32_t)v; };
+        for (int v = lane; v <= n; v += 64) start[v] = 0;
+        __syncthreads();
+        for (int st = lane; st < ns; st += 64) {
+            const int t = st + (int)bounded(stream_draw(J.k0, J.k1, J.draw + st), (uint64_t)(n - st));
+            tgt[st] = (IX)t;
+            atomicAdd(start + t, 1u);
+        }
+        __syncthreads();
+        {   // exclusive prefix sum: a contiguous run per lane, then across lanes
+            const int per = (n + 63) / 64, lo = min(n, lane * per), hi = min(n, lo + per);
+            uint32_t sum = 0;
+            for (int v = lo; v < hi; ++v) sum += start[v];
+            uint32_t x = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+                if (lane >= o) x += y;
+            }
+            uint32_t run = x - sum;
+            for (int v = lo; v < hi; ++v) {
+                const uint32_t c = start[v];
+                start[v] = run;
+                run += c;
+            }
+            if (lane == 63) start[n] = x;
+        }
+        for (int v = lane; v < n; v += 64) root[v] = 0;
+        __syncthreads();
+        for (int st = lane; st < ns; st += 64) {
+            const int t = (int)tgt[st];
+            steps[start[t] + atomicAdd(root + t, 1u)] = (IX)st;
+        }
+        __syncthreads();
+        for (int v = lane; v < n; v += 64) {   // a position draws one step on average
+            const uint32_t a = start[v], b = start[v + 1];
+            for (uint32_t x = a + 1; x < b; ++x) {
+                const IX key = steps[x];
+                uint32_t y = x;
+                while (y > a && steps[y - 1] > key) {
+                    steps[y] = steps[y - 1];
+                    --y;
+                }
+                steps[y] = key;
+            }
+        }
+        __syncthreads();
+        for (int st = lane; st < ns; st += 64) {   // L(s), or s itself at the end of a chain
+            const uint32_t a = start[st], b = start[st + 1];
+            uint32_t L = (uint32_t)st;
+            if (b > a) {
+                const uint32_t last = steps[b - 1];
+                if (last != (uint32_t)st) L = last;
+                else if (b - 1 > a) L = steps[b - 2];
+            }
+            root[st] = L;
+        }
+        __syncthreads();
+        for (;;) {   // pointer doubling to the chain ends
+            bool changed = false;
+            for (int st = lane; st < ns; st += 64) {
+                const uint32_t r = root[st], rr = root[r];
+                if (rr != r) {
+                    root[st] = rr;
+                    changed = true;
+                }
+            }
+            __syncthreads();
+            if (!__any(changed)) break;
+        }
+        for (int v = lane; v < n; v += 64) {
+            uint32_t val;
+            if (v < ns) {
+                const uint32_t t = tgt[v];
+                if (t == (uint32_t)v) {
+                    val = init_at((int)root[v]);
+                } else {
+                    const uint32_t a = start[t];
+                    uint32_t y = a;
+                    while (steps[y] != (IX)v) ++y;
+                    val = y > a ? init_at((int)root[steps[y - 1]]) : init_at((int)t);
+                }
+            } else {
+                const uint32_t a = start[v], b = start[v + 1];
+                val = b > a ? init_at((int)root[steps[b - 1]]) : init_at(v);
+            }
+            out[v] = val;
+        }
+        __syncthreads();   // the chain's next visit may start from this order
+    }
+}
+
+template <typename IX>
+__global__ __launch_bounds__(64) void k_permute_lds(const PermJob* __restrict__ jobs,
+                                                    const int32_t* __restrict__ chain_start, uint32_t* __restrict__ perm,
+                                                    int cap) {
+    extern __shared__ uint32_t sm[];
+    fy_chain<IX>(jobs, chain_start, blockIdx.x, perm, sm, cap);
+}
+
+// the chains of many problems in one launch (the batched prep's descriptors);
+// problems whose prep draws its own visit orders are skipped
+template <typename IX>
+__global__ __launch_bounds__(64) void k_permute_lds_multi(const PrepDesc* __restrict__ descs,
+                                                          const int2* __restrict__ cmap) {
+    extern __shared__ uint32_t sm[];
+    const int p = uniform_i32(cmap[blockIdx.x].x), chain = uniform_i32(cmap[blockIdx.x].y);
+    const PrepDesc& D = descs[p];
+    if (!uniform_i32(D.pre)) return;
+    fy_chain<IX>(D.jobs, D.chain_start, chain, D.perm, sm, D.cap);
 }
 
 // A record's value words: x as float and the row's 1/nnz (FP32/BF16 points),
@@ -114,12 +261,15 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                                            int32_t* __restrict__ ngroups, const int cap, const int rows_cap,
                                            const int cols_cap, const uint32_t dummy_row, const uint32_t dummy_col,
                                            unsigned long long* __restrict__ pprof, uint32_t* sm,
-                                           const bool wide = false) {
+                                           const bool wide = false, const bool pre = false) {
     auto up4 = [](int k) { return (k * (int)sizeof(IX) + 3) >> 2; };   // words
+    // pre (WOR only): the visit orders are already in perm, no LDS copy
+    const bool own = !(pre && order == 2);
+    const int vw = own ? up4(cap) : 0;
     IX* vis = (IX*)sm;                               // visit order
-    IX* lastR = (IX*)(sm + up4(cap));                // per local row: tag of its last update
-    IX* lastC = (IX*)(sm + up4(cap) + up4(rows_cap));  // per local column: the same
-    uint32_t* gmw = sm + up4(cap) + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 bits each
+    IX* lastR = (IX*)(sm + vw);                      // per local row: tag of its last update
+    IX* lastC = (IX*)(sm + vw + up4(rows_cap));      // per local column: the same
+    uint32_t* gmw = sm + vw + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 bits each
     const int lane = threadIdx.x;
     unsigned long long t_order = 0, t_sched = 0, t_fill = 0;
     for (int jx = chain_start[chain]; jx < chain_start[chain + 1]; ++jx) {
@@ -127,7 +277,9 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         const int n = (int)J.n;   // < 2^22
         const unsigned long long t0 = pprof ? clock64() : 0;
         // 1. visit order (sgd_impl.h:198-351)
-        if (order == 0) {             // SEQ
+        if (!own) {
+            // the visit order is in perm already
+        } else if (order == 0) {      // SEQ
             for (int k = lane; k < n; k += 64) vis[k] = (IX)k;
             __syncthreads();
         } else if (order == 1) {      // WR
@@ -208,8 +360,13 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             e.i = J.row0;
             e.j = J.col0;
             if (v < n) {
-                const uint32_t q = vis[v];
-                perm[J.base + v] = q;
+                uint32_t q;
+                if (own) {
+                    q = vis[v];
+                    perm[J.base + v] = q;
+                } else {
+                    q = perm[J.base + v];
+                }
                 e = pts[J.s0 + q];
             }
             return e;
@@ -321,10 +478,10 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                                                      const PT* __restrict__ pts, uint32_t* __restrict__ perm,
                                                      uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
                                                      int rows_cap, int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
-                                                     unsigned long long* __restrict__ pprof, int wide) {
+                                                     unsigned long long* __restrict__ pprof, int wide, int pre) {
     extern __shared__ uint32_t sm[];
     prep_chain<IX, PT>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
-                       dummy_row, dummy_col, pprof, sm, wide != 0);
+                       dummy_row, dummy_col, pprof, sm, wide != 0, pre != 0);
 }
 
 // Blocks whose prep state does not fit in LDS: the same prep with its state
@@ -358,7 +515,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped_multi(const PrepDesc* __res
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(PrepDesc) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
     prep_chain<IX, Pt32>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap,
-                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm, D.wide != 0);
+                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm, D.wide != 0, D.pre != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1918,13 +2075,49 @@ bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_bloc
 
 template <typename IX, typename PT>
 static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
-                          const int32_t* d_chain_start, int nchains, int order, int cap, size_t bytes, hipStream_t s) {
+                          const int32_t* d_chain_start, int nchains, int order, int cap, size_t bytes, bool pre,
+                          hipStream_t s) {
     const void* fn = (const void*)k_prep_grouped<IX, PT>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
     k_prep_grouped<IX, PT><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const PT*)P.pts, P.perm,
                                                       (uint4*)B.grec, B.ngroups, cap, B.max_rows, B.max_cols, drow,
-                                                      dcol, B.pprof, B.wide ? 1 : 0);
+                                                      dcol, B.pprof, B.wide ? 1 : 0, pre ? 1 : 0);
+    HIPK(hipGetLastError());
+}
+
+// Who draws the WOR visit orders: k_permute_lds before the prep, or the prep
+// itself.  Measured on MI355X: the separate kernel shortens a batch's preps
+// (batch64: 224 against 233 ms per epoch), while beside a single problem's
+// epoch kernel the prep's extra resident waves cost that kernel more than the
+// prep gains (100M-nonzero workload: 29.7 against 29.4 ms).  So batches use it
+// and single contexts do not; DSGDPP_PREP_ORDER=1 or 0 sets both.
+static bool orders_first(bool batch) {
+    const char* e = getenv("DSGDPP_PREP_ORDER");
+    const int mode = e && (e[0] == '0' || e[0] == '1') ? e[0] - '0' : -1;
+    return mode < 0 ? batch : mode == 1;
+}
+
+// The LDS of the visit-order kernel for blocks up to cap points, with 16-bit
+// indices where the prep has them (cap < PREP_NARROW); 0 when that exceeds
+// FY_LDS_MAX (the prep then draws the orders itself).
+static size_t fy_bytes(int64_t cap) {
+    const size_t b = cap < PREP_NARROW ? fy_lds<uint16_t>(cap) : fy_lds<uint32_t>(cap);
+    return b <= FY_LDS_MAX ? b : 0;
+}
+
+static void launch_permute_lds(const PermJob* d_jobs, const int32_t* d_chain_start, int nchains, uint32_t* perm,
+                               int cap, hipStream_t s) {
+    const size_t bytes = fy_bytes(cap);
+    if (cap < PREP_NARROW) {
+        HIPK(hipFuncSetAttribute((const void*)k_permute_lds<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)bytes));
+        k_permute_lds<uint16_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, perm, cap);
+    } else {
+        HIPK(hipFuncSetAttribute((const void*)k_permute_lds<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)bytes));
+        k_permute_lds<uint32_t><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, perm, cap);
+    }
     HIPK(hipGetLastError());
 }
 
@@ -1954,19 +2147,24 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     if (nchains <= 0) return;
     const int cap = (int)std::max<int64_t>(max_block, 1);
     const bool narrow = cap < PREP_NARROW;
-    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow);
+    // WOR visit orders from the parallel kernel when its LDS allows; the prep
+    // then keeps no visit-order array (more of its waves fit on a CU)
+    const bool pre = order == 2 && orders_first(false) && fy_bytes(cap) > 0 &&
+                     prep_lds(cap, B.max_rows, B.max_cols, narrow, true) <= PREP_LDS_PER_CU;
+    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow, pre);
     const bool f64 = P.precision == PREC_FP64 || P.precision == PREC_FP64_STRICT;
     if (bytes > PREP_LDS_PER_CU) {   // the state goes to global scratch
         if (f64) prep_spill_t<Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, s);
         else prep_spill_t<Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, s);
         return;
     }
+    if (pre) launch_permute_lds(d_jobs, d_chain_start, nchains, P.perm, cap, s);
     if (f64) {
-        if (narrow) prep_launch_t<uint16_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
-        else prep_launch_t<uint32_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
+        if (narrow) prep_launch_t<uint16_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
+        else prep_launch_t<uint32_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
     } else {
-        if (narrow) prep_launch_t<uint16_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
-        else prep_launch_t<uint32_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, s);
+        if (narrow) prep_launch_t<uint16_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
+        else prep_launch_t<uint32_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
     }
 }
 
@@ -1986,16 +2184,30 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
     D.dummy_row = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2);
     D.dummy_col = (uint32_t)B.max_cols;
     D.wide = B.wide ? 1 : 0;
+    const bool narrow = D.cap < PREP_NARROW;
+    D.pre = order == 2 && orders_first(true) && fy_bytes(D.cap) > 0 &&
+            prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, true) <= PREP_LDS_PER_CU;
     return D;
 }
 
 bool prep_narrow(int64_t max_block) { return std::max<int64_t>(max_block, 1) < PREP_NARROW; }
 
-size_t prep_desc_lds(const PrepDesc& D) { return prep_lds(D.cap, D.rows_cap, D.cols_cap, D.cap < PREP_NARROW); }
+size_t prep_desc_lds(const PrepDesc& D) {
+    return prep_lds(D.cap, D.rows_cap, D.cols_cap, D.cap < PREP_NARROW, D.pre != 0);
+}
+
+size_t prep_desc_order_lds(const PrepDesc& D) { return D.pre ? fy_bytes(D.cap) : 0; }
 
 void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,
-                               hipStream_t s) {
+                               hipStream_t s, size_t order_lds) {
     if (nchains <= 0) return;
+    if (order_lds > 0) {   // the WOR visit orders of the problems whose descriptor says so, first
+        const void* f = narrow ? (const void*)k_permute_lds_multi<uint16_t> : (const void*)k_permute_lds_multi<uint32_t>;
+        HIPK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)order_lds));
+        if (narrow) k_permute_lds_multi<uint16_t><<<nchains, 64, order_lds, s>>>(d_descs, d_cmap);
+        else k_permute_lds_multi<uint32_t><<<nchains, 64, order_lds, s>>>(d_descs, d_cmap);
+        HIPK(hipGetLastError());
+    }
     const void* fn = narrow ? (const void*)k_prep_grouped_multi<uint16_t> : (const void*)k_prep_grouped_multi<uint32_t>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (narrow) k_prep_grouped_multi<uint16_t><<<nchains, 64, lds, s>>>(d_descs, d_cmap);

@@ -722,3 +722,47 @@ def test_dataflow_wait_limit_reports_instead_of_hanging(monkeypatch, precision, 
         np.testing.assert_array_equal(Wg, Wo, err_msg=f"epoch {e}")
         np.testing.assert_array_equal(Hg, Ho, err_msg=f"epoch {e}")
     g.close()
+
+
+@pytest.mark.parametrize("mode", ["1", "0"])
+def test_grouped_prep_visit_orders_either_way(monkeypatch, mode):
+    """The WOR visit orders of the grouped prep come from the parallel
+    visit-order kernel (DSGDPP_PREP_ORDER=1, the batches' default) or from the
+    prep itself (0, the single contexts' default): the orders equal the
+    oracle's and the factors its FP32 emulation, bit for bit, either way,
+    including orders that carry over between equal-size blocks."""
+    print(f"case: grouped prep, DSGDPP_PREP_ORDER={mode}", flush=True)
+    monkeypatch.setenv("DSGDPP_PREP_ORDER", mode)
+    p = skewed_problem(400, 300, 9000, 24, seed=77)
+    g, o = make_pair(p, update="Nzsl_L2(0.1)", tasks=3, seed=61, precision="FP32", kernel=3)
+    o.set_arith("FP32")
+    for e in range(3):
+        assert g.epoch(0.02)["kernel"] in (3, 4)
+        o.epoch(0.02)
+        for t in range(3):
+            for s in (0, 2, 5):
+                np.testing.assert_array_equal(g.permutation(t, s), o.permutation(t, s), err_msg=f"{e} {t} {s}")
+        np.testing.assert_array_equal(g.get_factors()[0], o.get_factors()[0], err_msg=f"epoch {e}")
+        np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"epoch {e}")
+    g.close()
+    # equal block sizes: the orders carry over from block to block
+    rng = np.random.default_rng(5)
+    d, rows, cols = 2, 30, 30
+    i, j = [], []
+    for b1 in range(d):
+        for b2 in range(2 * d):
+            cells = rng.choice(rows * cols, size=200, replace=False)
+            i += list(b1 * rows + cells // cols)
+            j += list(b2 * cols + cells % cols)
+    q = dict(m=d * rows, n=2 * d * cols, train=(np.array(i), np.array(j), rng.normal(size=len(i))), rank=8,
+             W0=rng.uniform(-0.5, 0.5, (d * rows, 8)), H0=rng.uniform(-0.5, 0.5, (8, 2 * d * cols)))
+    g, o = make_pair(q, update="Nzsl_L2(0.1)", tasks=d, seed=3, precision="FP32", kernel=3)
+    o.set_arith("FP32")
+    for e in range(2):
+        g.epoch(0.05)
+        o.epoch(0.05)
+        for t in range(d):
+            for s in range(2 * d):
+                np.testing.assert_array_equal(g.permutation(t, s), o.permutation(t, s))
+        np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"carry-over epoch {e}")
+    g.close()
