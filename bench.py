@@ -515,8 +515,7 @@ def bench_batch(args):
     meta = [dict(m=p["m"], n=p["n"], nnz=len(p["train"][2]), rank=p["rank"]) for p in probs]
     budget = torch.cuda.get_device_properties(0).multi_processor_count   # one resident task per CU
     d = B.plan_tasks(meta, budget, rates)
-    parts = B.partition(d, budget, [B.predict_time(m, dk, rates) for m, dk in zip(meta, d)])
-    log(f"[bench] tasks per problem: {d} (sum {sum(d)}), {len(parts)} launch(es) per epoch")
+    log(f"[bench] tasks per problem: {d} (sum {sum(d)}), one launch per epoch")
     t0 = time.time()
     ctxs = []
     for k, p in enumerate(probs):
@@ -532,9 +531,11 @@ def bench_batch(args):
         ctxs.append(g)
         if (k + 1) % 8 == 0:
             log(f"[bench] {k + 1} contexts loaded ({time.time() - t0:.1f}s)")
-    parts = B.partition(d, budget, [B.predict_time(m, dk, rates) for m, dk in zip(meta, d)])
-    batches = [B.DsgdPpBatch([ctxs[k] for k in part], use_graph=True) for part in parts]
-    log(f"[bench] contexts loaded + batches created in {time.time() - t0:.1f}s")
+    # one batch: tasks beyond one per CU queue on the CUs' workgroups (the
+    # library packs them, longest work first)
+    parts = [list(range(len(ctxs)))]
+    batches = [B.DsgdPpBatch(ctxs, use_graph=True)]
+    log(f"[bench] contexts loaded + batch created in {time.time() - t0:.1f}s")
 
     class _All:
         def epoch(self, eps):

@@ -1931,7 +1931,9 @@ struct dsgdpp_batch {
     size_t descCap = 0, cmapCap = 0;
     uint8_t* hArgs = nullptr;    // pinned host staging of the argument table
     void* dArgs = nullptr;
-    int2* dTmap = nullptr;
+    int2* dTmap = nullptr;        // (problem, task) items, workgroup after workgroup
+    int32_t* dWstart = nullptr;   // nblocks + 1: each workgroup's first item
+    int tasks = 0;                // items in all
     std::vector<int32_t*> ready;
     std::vector<int> dsz;
     hipGraph_t graph = nullptr;
@@ -1967,6 +1969,8 @@ void batchDestroy(dsgdpp_batch* b) {
     dfree(q);
     q = b->dTmap;
     dfree(q);
+    q = b->dWstart;
+    dfree(q);
     q = b->dDesc;
     dfree(q);
     q = b->dCmap;
@@ -1979,6 +1983,59 @@ void batchDestroy(dsgdpp_batch* b) {
     delete b;
 }
 
+
+int deviceCUs(int device) {
+    int cus = 0;
+    HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    return std::max(1, cus);
+}
+
+// The batch's (problem, task) items on at most `slots` workgroups (one per CU
+// at most: two tasks on a CU run at about half speed each).  When every task
+// has a workgroup of its own, that is all.  Otherwise, longest-processing-time
+// first: problems in decreasing order of work per task (points per task,
+// rank classes above 64 weighted 1.7), each problem's d tasks onto the d
+// least loaded workgroups, which then hold their items in that one order of
+// the problems and at most one task of each problem (the batch kernel's
+// progress argument).  Returns the number of workgroups.
+int packBatch(const std::vector<dsgdpp_ctx*>& ctx, int slots, std::vector<int2>& items, std::vector<int32_t>& wstart) {
+    int tasks = 0;
+    for (dsgdpp_ctx* c : ctx) tasks += (int)c->d;
+    items.clear();
+    wstart.clear();
+    if (tasks <= slots) {
+        for (size_t k = 0; k < ctx.size(); ++k)
+            for (int t = 0; t < (int)ctx[k]->d; ++t) items.push_back(make_int2((int)k, t));
+        for (int w = 0; w <= tasks; ++w) wstart.push_back(w);
+        return tasks;
+    }
+    std::vector<double> work(ctx.size());
+    for (size_t k = 0; k < ctx.size(); ++k) {
+        if ((int)ctx[k]->d > slots)
+            throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " has more tasks than the device has CUs");
+        work[k] = (double)ctx[k]->blocks[0].nnz / ctx[k]->d * (ctx[k]->P.ld > 64 ? 1.7 : 1.0);
+    }
+    std::vector<int> order(ctx.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return work[a] > work[c]; });
+    std::vector<double> load(slots, 0.0);
+    std::vector<std::vector<int2>> lists(slots);
+    std::vector<int> ws(slots);
+    std::iota(ws.begin(), ws.end(), 0);
+    for (int k : order) {
+        std::stable_sort(ws.begin(), ws.end(), [&](int a, int c) { return load[a] < load[c]; });
+        for (int t = 0; t < (int)ctx[k]->d; ++t) {
+            lists[ws[t]].push_back(make_int2(k, t));
+            load[ws[t]] += work[k];
+        }
+    }
+    for (int w = 0; w < slots; ++w) {
+        wstart.push_back((int32_t)items.size());
+        items.insert(items.end(), lists[w].begin(), lists[w].end());
+    }
+    wstart.push_back((int32_t)items.size());
+    return slots;
+}
 
 void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -2020,13 +2077,13 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     if (b->useGraph && !b->exec) {
         // one graph: the counters' reset and the launch; the table it reads is refreshed above
         HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
+        launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->dWstart, b->nblocks, b->lds,
                            b->ready.data(), b->dsz.data(), (int)K, b->dStatus, b->stream);
         HIPC(hipStreamEndCapture(b->stream, &b->graph));
         HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     if (b->useGraph) HIPC(hipGraphLaunch(b->exec, b->stream));
-    else launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->nblocks, b->lds,
+    else launch_epoch_batch(b->precision, b->kind, b->trunc, b->dArgs, b->dTmap, b->dWstart, b->nblocks, b->lds,
                             b->ready.data(), b->dsz.data(), (int)K, b->dStatus, b->stream);
     HIPC(hipMemcpyAsync(b->hStatus, b->dStatus, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
     HIPC(hipEventRecord(b->ev1, b->stream));
@@ -2172,21 +2229,21 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
         HIPC(hipSetDevice(b->device));
         b->useGraph = use_graph != 0;
         b->argBytes = grouped_args_bytes(b->precision);   // the kernel indexes the table by sizeof
-        std::vector<int2> tmap;
         for (int k = 0; k < count; ++k) {
             dsgdpp_ctx* c = b->ctx[k];
-            for (int t = 0; t < (int)c->d; ++t) tmap.push_back(make_int2(k, t));
+            b->tasks += (int)c->d;
             b->lds = std::max(b->lds, batch_epoch_lds(c->G.max_cols, c->P.ld, c->G.wide));
             if (!c->dReady) dalloc(c->dReady, (size_t)2 * c->d * sizeof(int32_t));
             b->ready.push_back(c->dReady);
             b->dsz.push_back((int)c->d);
             HIPC(hipStreamSynchronize(c->stream));
         }
-        b->nblocks = (int)tmap.size();
         const int cap = batch_capacity(b->precision, b->kind, b->trunc, b->lds, b->device);
-        if (b->nblocks > cap)
-            throw Error(DSGDPP_E_UNSUPPORTED, "the batch has " + std::to_string(b->nblocks) + " tasks; at most " +
-                                                  std::to_string(cap) + " can be resident at once");
+        std::vector<int2> items;
+        std::vector<int32_t> wstart;
+        int slots = std::min(cap, deviceCUs(b->device));
+        if (const char* e = getenv("DSGDPP_BATCH_SLOTS")) slots = std::max(1, std::min(slots, atoi(e)));
+        b->nblocks = packBatch(b->ctx, slots, items, wstart);
         HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         HIPC(hipStreamCreateWithFlags(&b->prepStream, hipStreamNonBlocking));
         HIPC(hipEventCreate(&b->ev0));
@@ -2198,8 +2255,10 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
         HIPC(hipDeviceGetAttribute(&b->wallKhz, hipDeviceAttributeWallClockRate, b->device));
         if (b->wallKhz <= 0) b->wallKhz = 100000;
         dalloc(b->dArgs, (size_t)count * b->argBytes);
-        dalloc(b->dTmap, tmap.size() * sizeof(int2));
-        HIPC(hipMemcpy(b->dTmap, tmap.data(), tmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+        dalloc(b->dTmap, items.size() * sizeof(int2));
+        HIPC(hipMemcpy(b->dTmap, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+        dalloc(b->dWstart, wstart.size() * sizeof(int32_t));
+        HIPC(hipMemcpy(b->dWstart, wstart.data(), wstart.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     } catch (const std::exception& e) {
         g_batchCreateError = e.what();
         const Error* ee = dynamic_cast<const Error*>(&e);

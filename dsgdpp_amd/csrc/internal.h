@@ -235,9 +235,11 @@ void fill_grouped_args(const DevProblem& P, const UpdParams& U, const GroupedBuf
                        int32_t* status, uint32_t wait_ticks, void* out);
 size_t batch_epoch_lds(int max_cols, int64_t max_ld, bool wide);
 int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device);
-void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
-                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, int32_t* status,
-                        hipStream_t s);
+// nblocks workgroups; workgroup b runs the (problem, task) items
+// d_items[d_wstart[b] .. d_wstart[b + 1]) in order
+void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_items,
+                        const int32_t* d_wstart, int nblocks, size_t lds, int32_t* const* ready, const int* dsz,
+                        int nprob, int32_t* status, hipStream_t s);
 double device_nzsl(const DevProblem& P, int which, double* d_partials, double* h_pinned,
                    hipStream_t s);
 double device_sumsq(const DevProblem& P, int which_factor, const double* d_weights,

@@ -1942,30 +1942,38 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
     }
 }
 
-// Many problems in one persistent launch: workgroup -> (problem, task) by
-// `tmap`; each problem's arguments are read from device memory, so a captured
-// launch replays with new step sizes and plans.
+// Many problems in one persistent launch.  Workgroup b runs the (problem,
+// task) items items[wstart[b] .. wstart[b+1]) one after the other; each
+// problem's arguments are read from device memory, so a captured launch
+// replays with new step sizes and plans.  The host lists every workgroup's
+// items in one global order of the problems and gives a workgroup at most one
+// task of a problem, so a task only ever waits for tasks that are running or
+// that follow only earlier problems on their workgroups: every wait ends.
 template <typename T, int KIND, bool TRUNC>
 __global__ __launch_bounds__(128) void k_epoch_grouped_batch(const GArgs<T>* __restrict__ probs,
-                                                            const int2* __restrict__ tmap) {
+                                                            const int2* __restrict__ items,
+                                                            const int32_t* __restrict__ wstart) {
     extern __shared__ float lds[];
-    const int p = uniform_i32(tmap[blockIdx.x].x), t = uniform_i32(tmap[blockIdx.x].y);
-    GArgs<T> A;
-    const uint32_t* src = (const uint32_t*)(probs + p);
-    uint32_t* dst = (uint32_t*)&A;
+    const int i0 = uniform_i32(wstart[blockIdx.x]), i1 = uniform_i32(wstart[blockIdx.x + 1]);
+    for (int it = i0; it < i1; ++it) {
+        const int p = uniform_i32(items[it].x), t = uniform_i32(items[it].y);
+        GArgs<T> A;
+        const uint32_t* src = (const uint32_t*)(probs + p);
+        uint32_t* dst = (uint32_t*)&A;
 #pragma unroll
-    for (int k = 0; k < (int)(sizeof(GArgs<T>) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
-    {
+        for (int k = 0; k < (int)(sizeof(GArgs<T>) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
         if (A.wide) {   // the whole workgroup takes this branch; one wave does the task
             if (threadIdx.x < 64) {
                 if (A.ld <= 64) wide_task<T, KIND, TRUNC, 1>(A, t, lds);
                 else wide_task<T, KIND, TRUNC, 2>(A, t, lds);
             }
-            return;
+        } else if (A.ld <= 64) {
+            grouped_task<T, KIND, TRUNC, 1>(A, t, lds);
+        } else {
+            grouped_task<T, KIND, TRUNC, 2>(A, t, lds);
         }
+        __syncthreads();   // both waves are done with the LDS before the next item
     }
-    if (A.ld <= 64) grouped_task<T, KIND, TRUNC, 1>(A, t, lds);
-    else grouped_task<T, KIND, TRUNC, 2>(A, t, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -2446,15 +2454,15 @@ int batch_capacity(int precision, int kind, bool trunc, size_t lds, int device) 
     return per_cu * cus;
 }
 
-void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_tmap, int nblocks,
-                        size_t lds, int32_t* const* ready, const int* dsz, int nprob, int32_t* status,
-                        hipStream_t s) {
+void launch_epoch_batch(int precision, int kind, bool trunc, const void* d_args, const int2* d_items,
+                        const int32_t* d_wstart, int nblocks, size_t lds, int32_t* const* ready, const int* dsz,
+                        int nprob, int32_t* status, hipStream_t s) {
     // (batch_capacity has set the function's dynamic LDS limit; nothing here but
     // stream work, so the sequence can be captured into a graph)
     for (int k = 0; k < nprob; ++k) k_zero_i32<<<1, 256, 0, s>>>(ready[k], 2 * dsz[k]);
     k_zero_i32<<<1, 64, 0, s>>>(status, 1);
     const void* fn = precision == PREC_FP32 ? batch_fn_p<float>(kind, trunc) : batch_fn_p<bf16_t>(kind, trunc);
-    void* args[] = {const_cast<void**>(&d_args), const_cast<int2**>(&d_tmap)};
+    void* args[] = {const_cast<void**>(&d_args), const_cast<int2**>(&d_items), const_cast<int32_t**>(&d_wstart)};
     HIPK(hipLaunchKernel(fn, dim3(nblocks), dim3(128), args, lds, s));
 }
 

@@ -198,3 +198,40 @@ def test_batch_wait_limit_reports_instead_of_hanging(monkeypatch):
     bt.close()
     for g in ctxs:
         g.close()
+
+
+@pytest.mark.gpu
+def test_batch_packs_more_tasks_than_cus():
+    """40 problems of 8 tasks: more tasks than the device has CUs, so the
+    library queues several tasks on each workgroup of the one launch (every
+    workgroup's tasks in one order of the problems, at most one task of a
+    problem per workgroup).  Every problem ends each epoch bit-identical to the
+    oracle's FP32 emulation; a problem of heavy blocks makes the others wait
+    on its column blocks across the queues."""
+    from test_gpu_parity import imbalanced_problem
+    from dsgdpp_amd import data
+    print("case: batch of 40 x 8 tasks (packed onto the CUs' workgroups)", flush=True)
+    probs = []
+    for k in range(40):
+        if k == 7:
+            probs.append((imbalanced_problem(16, d=8, rows=40, cols=40, heavy=800, light=10, seed=k), 8, 0.05))
+        else:
+            p = data.uniform_problem(240 + 8 * k, 160 + 4 * k, 3000 + 50 * k, 0, rank=16 if k % 3 else 64,
+                                     seed=100 + k)
+            probs.append((p, 8, 0.01 + 0.001 * k))
+    ctxs = [_ctx(p, d, seed=50 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    orcs = [_oracle(p, d, seed=50 + k, lam=lam) for k, (p, d, lam) in enumerate(probs)]
+    bt = B.DsgdPpBatch(ctxs, use_graph=True)
+    eps = np.linspace(0.005, 0.02, len(probs))
+    for e in range(2):
+        st = bt.epoch(eps)
+        assert st["launches"] == 1
+        for k, o in enumerate(orcs):
+            o.epoch(eps[k])
+            Wb, Hb = ctxs[k].get_factors()
+            Wo, Ho = o.get_factors()
+            np.testing.assert_array_equal(Wb, Wo, err_msg=f"problem {k} epoch {e}")
+            np.testing.assert_array_equal(Hb, Ho, err_msg=f"problem {k} epoch {e}")
+    bt.close()
+    for g in ctxs:
+        g.close()
