@@ -726,7 +726,8 @@ void drawPlan(dsgdpp_ctx* c, EpochPlan& P) {
 // Visit-order jobs, dataflow dependencies and the visit-order kernel of a
 // drawn plan, all enqueued on `s`.  With launchPrep false the caller launches
 // the grouped prep (one launch for many plans) and records P.evEnd.
-void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = true) {
+// ahead: a look-ahead plan, whose prep runs beside the current epoch
+void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = true, bool ahead = false) {
     const uint32_t d = c->d, d2 = 2 * d;
     const std::vector<int64_t>& bs = c->blocks[0].start;
     auto visitBlock = [&](uint32_t id, uint32_t sub) { return (size_t)id * d2 + P.schedule[(size_t)sub * d + id]; };
@@ -925,8 +926,9 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
         G.spill = c->G.spill;
         G.spill_bytes = c->G.spill_bytes;
         if (P.nSmall > 0)
-            launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s);
-        launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.nSmall, nchains - P.nSmall, c->sgdOrder, maxBlock, s);
+            launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s, ahead);
+        launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.nSmall, nchains - P.nSmall, c->sgdOrder, maxBlock, s,
+                            ahead);
     } else if (c->sgdOrder == DSGDPP_SGD_WOR)
         launch_permute_wor(P.dJobs, P.dChains, nchains, P.perm, maxBlock, s);
     else if (c->sgdOrder == DSGDPP_SGD_WR)
@@ -1005,7 +1007,7 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     if (c->lookahead) {
         EpochPlan& N = c->plan[c->cur];
         drawPlan(c, N);
-        buildPlan(c, N, c->prepStream);
+        buildPlan(c, N, c->prepStream, true, true);
     }
     HIPC(hipEventSynchronize(c->ev[2]));
     const auto t1 = std::chrono::steady_clock::now();
@@ -2128,11 +2130,13 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         ahead.push_back(&N);
     }
     if (!ahead.empty()) {
-        // the classes with the fewest workgroups per CU first: their long tails
-        // overlap the launches that follow
+        // the classes with the least LDS per workgroup first: the launches of
+        // one stream start in order, and those can start beside the batch's
+        // own workgroups (a class that needs more LDS than a CU has left would
+        // hold back every launch after it until the batch ends)
         std::vector<std::pair<std::pair<int, int>, PrepClass*>> order;
         for (auto& kv : classes) order.push_back({kv.first, &kv.second});
-        std::sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first.second < y.first.second; });
+        std::sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first.second > y.first.second; });
         std::vector<int> start;
         for (auto& o : order) {
             start.push_back((int)b->cmap.size());

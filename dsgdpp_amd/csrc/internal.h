@@ -171,6 +171,7 @@ struct PrepDesc {
     uint32_t dummy_row, dummy_col;
     int32_t wide;   // H forwarding entries for the streaming kernel
     int32_t pre;    // WOR visit orders precomputed into perm (k_permute_lds_multi) before the prep
+    int32_t rounds; // the round scan places the points (else the sequential scan)
 };
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 // grouped epoch eligibility; *wide (optional) is set when only the variant
@@ -221,9 +222,10 @@ size_t prep_spill_bytes(int64_t max_block, int max_rows, int max_cols, int nchai
 // set (narrow: 16-bit indices, as the prep's)
 void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,
                                hipStream_t s, size_t order_lds);
+// beside: the prep runs beside an epoch kernel (a look-ahead plan)
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block,
-                         hipStream_t s);
+                         hipStream_t s, bool beside = false);
 void launch_epoch_grouped(const DevProblem& P, const UpdParams& U, const GroupedBuffers& B,
                           const EpochLaunch& L, hipStream_t s);
 // batched grouped epoch: one launch over several problems (same precision,

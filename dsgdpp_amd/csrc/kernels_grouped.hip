@@ -76,6 +76,8 @@ __device__ int amdgcn_writelane(int v, int lane, int old) __asm("llvm.amdgcn.wri
 __device__ __forceinline__ uint32_t writelane_u32(uint32_t v, int lane, uint32_t old) {
     return (uint32_t)amdgcn_writelane((int)v, lane, (int)old);
 }
+// keeps the compiler from moving memory accesses across it; emits nothing
+__device__ __forceinline__ void lds_order() { __asm__ __volatile__("" ::: "memory"); }
 // x rotated right by k in [0, 63]
 __device__ __forceinline__ uint64_t rotr64(uint64_t x, int k) { return (x >> k) | (x << ((64 - k) & 63)); }
 // v from lane `src` in [0, 63] (per lane)
@@ -91,12 +93,16 @@ __device__ __forceinline__ uint32_t lane_gather(uint32_t v, int src) {
 // workgroups per CU.
 constexpr int64_t PREP_NARROW = 8000;    // tag ((group + 2) << 2 | slot) < 2^16 with group < 2 n
 // pre: the visit orders come precomputed in perm (k_permute_lds), so the prep
-// keeps no visit-order array
+// keeps no visit-order array.  After the group slot masks: the round scan's
+// bitmap of full groups and its two ownership tables (PREP_HASH words each).
+constexpr int PREP_HASH = 256;
+__host__ __device__ inline int prep_mask_words(int64_t cap) { return (int)((2 * cap + GROUP_PAD + 7) >> 3); }
+__host__ __device__ inline int prep_full_words(int64_t cap) { return (int)((2 * cap + GROUP_PAD + 31) >> 5) + 1; }
 static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow, bool pre = false) {
     const size_t es = narrow ? 2 : 4;
     auto up4 = [](size_t b) { return (b + 3) & ~(size_t)3; };
     return (pre ? 0 : up4(cap * es)) + up4((size_t)rows * es) + up4((size_t)cols * es) +
-           ((2 * (size_t)cap + GROUP_PAD + 7) >> 3) * 4;
+           ((size_t)prep_mask_words(cap) + prep_full_words(cap) + 2 * PREP_HASH) * 4;
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +267,8 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                                            int32_t* __restrict__ ngroups, const int cap, const int rows_cap,
                                            const int cols_cap, const uint32_t dummy_row, const uint32_t dummy_col,
                                            unsigned long long* __restrict__ pprof, uint32_t* sm,
-                                           const bool wide = false, const bool pre = false) {
+                                           const bool wide = false, const bool pre = false,
+                                           const bool rounds = false) {
     auto up4 = [](int k) { return (k * (int)sizeof(IX) + 3) >> 2; };   // words
     // pre (WOR only): the visit orders are already in perm, no LDS copy
     const bool own = !(pre && order == 2);
@@ -270,6 +277,8 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
     IX* lastR = (IX*)(sm + vw);                      // per local row: tag of its last update
     IX* lastC = (IX*)(sm + vw + up4(rows_cap));      // per local column: the same
     uint32_t* gmw = sm + vw + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 bits each
+    uint32_t* fullw = gmw + prep_mask_words(cap);               // round scan: bit g = group g has no free slot
+    uint32_t* ownw = fullw + prep_full_words(cap);              // round scan: row and column ownership tables
     const int lane = threadIdx.x;
     unsigned long long t_order = 0, t_sched = 0, t_fill = 0;
     for (int jx = chain_start[chain]; jx < chain_start[chain + 1]; ++jx) {
@@ -329,10 +338,16 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             }
         }
         const unsigned long long t1 = pprof ? clock64() : 0;
-        const int gwords = (2 * n + GROUP_PAD + 7) >> 3;   // a visit has fewer than 2 n groups
+        const int gwords = prep_mask_words(n);   // a visit has fewer than 2 n groups
+        // the slot mask of a group with no point yet: a two-slot visit (J.pairs)
+        // starts every group with slots 2 and 3 taken, so its points go to
+        // slots 0 and 1 only
+        const uint32_t empty = J.pairs ? 0xCu : 0u;
         for (int k = lane; k < rows_cap; k += 64) lastR[k] = 0;
         for (int k = lane; k < cols_cap; k += 64) lastC[k] = 0;
-        for (int k = lane; k < gwords; k += 64) gmw[k] = 0;
+        for (int k = lane; k < gwords; k += 64) gmw[k] = rounds ? empty * 0x11111111u : 0u;
+        if (rounds)
+            for (int k = lane; k < prep_full_words(n); k += 64) fullw[k] = 0;
         __syncthreads();
         // 2. list scheduling, 64 visit positions at a time.  The scan runs on
         //    uniform values.  Groups are numbered from 2 inside the scan (G = g + 2)
@@ -347,10 +362,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         //    groups of the window, so the search is one rotate and find-first-zero.
         //    A group that left the window is never searched: a point starts at
         //    maxG - 61 or later.
-        //    A two-slot visit (J.pairs) starts every group with slots 2 and 3
-        //    taken, so its points go to slots 0 and 1 only.
         int maxG = 1;
-        const uint32_t empty = J.pairs ? 0xCu : 0u;   // slot mask of a group with no point yet
         uint32_t win = 0;    // lane b: G << 4 | slot mask of the window's group G in lane b
         uint64_t full = 0;   // bit b: that group has no free slot
         // the next chunk's points are fetched while the current chunk is scanned
@@ -371,11 +383,125 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
             }
             return e;
         };
+        // the record of a point placed in group g (from 0), slot sl; prevR and
+        // prevC are its row's and its column's tags before it
+        auto emit = [&](const PT& e, const uint32_t c, const uint32_t g, const uint32_t sl, const bool hreg,
+                        const uint32_t prevR, const uint32_t prevC) {
+            // W_i from the LDS ring when the row was updated in the last GROUP_RING groups
+            const int gR = (int)(prevR >> 2) - 2;
+            const bool wf = prevR != 0u && (int)g - gR < GROUP_RING;
+            uint4 o;
+            o.x = (uint32_t)e.i |
+                  (wf ? (REC_WFWD | (((((uint32_t)gR & (GROUP_RING - 1)) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
+            o.y = c | (hreg ? REC_HREG : 0u);
+            if (wide && !hreg) {   // H_j from the H ring when the column was updated in the last GROUP_RING groups
+                const int gC = (int)(prevC >> 2) - 2;
+                if (prevC != 0u && (int)g - gC < GROUP_RING)
+                    o.y |= REC_HFWD | (((((uint32_t)gC & (GROUP_RING - 1)) << 2) | (prevC & 3u)) << REC_HRING_SHIFT);
+            }
+            rec_value(e, o);
+            grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
+        };
+        int gtop = -1;   // round scan: the last group this lane placed a point in
         PT en = fetch(lane);
         for (int v0 = 0; v0 < n; v0 += 64) {
             const int v = v0 + lane;
             const PT e = en;
             const uint32_t r = (uint32_t)(e.i - J.row0), c = (uint32_t)(e.j - J.col0);
+            if (rounds) {
+                en = fetch(v + 64);
+                // 2'. the same placement rule in rounds, the chunk's points in
+                //     parallel.  A round places every unplaced point that is the
+                //     earliest unplaced one of its row and of its column in the
+                //     chunk (ownership tables indexed by row and column modulo
+                //     PREP_HASH: a shared entry only delays a point to a later
+                //     round).  All earlier points on its row and column are then
+                //     placed, so its earliest group follows from their tags as in
+                //     the scan below, and the points of one round have distinct
+                //     rows and columns.  Slots are claimed with LDS atomics (a
+                //     point that loses its slot to another tries again); a bitmap
+                //     skips full groups.  Any placement order within the rule
+                //     gives a valid grouping (the factors do not depend on it).
+                //     With no window there is no floor on the group either.
+                bool todo = v < n;
+                uint32_t* const oR = ownw + (r & (PREP_HASH - 1));
+                uint32_t* const oC = ownw + PREP_HASH + (c & (PREP_HASH - 1));
+                // LDS instructions of one wave execute in issue order, so the
+                // steps of a round need no wait between them, only a compiler
+                // fence (lds_order) that keeps their order in the code
+                while (__ballot(todo) != 0ull) {
+                    uint32_t a = 0, b = 0, LR = 0, LC = 0;
+                    if (todo) {
+                        *oR = 64u;
+                        *oC = 64u;
+                        lds_order();
+                        atomicMin(oR, (uint32_t)lane);
+                        atomicMin(oC, (uint32_t)lane);
+                        lds_order();
+                        // the tags are read by every waiting point; they are
+                        // final for the points that own their row and column
+                        a = *oR;
+                        b = *oC;
+                        LR = (uint32_t)lastR[r];
+                        LC = (uint32_t)lastC[c];
+                    }
+                    if (todo && a == (uint32_t)lane && b == (uint32_t)lane) {
+                        const int gH = (int)(LC >> 2) + 1;   // groups numbered from 2, as in the scan below
+                        const uint32_t sc = LC & 3u;
+                        const int g0 = max((int)(LR >> 2) + 2, gH);
+                        int g = -1;
+                        uint32_t sl = 0;
+                        bool hit = false;
+                        int k = g0 - 2;
+                        if (g0 == gH) {   // the column's slot in the group after its last update
+                            const uint32_t sh = 4u * (uint32_t)(k & 7), bit = (1u << sc) << sh;
+                            const uint32_t old = atomicOr(gmw + (k >> 3), bit);
+                            if (!(old & bit)) {
+                                hit = true;
+                                g = k;
+                                sl = sc;
+                                if ((((old | bit) >> sh) & 15u) == 15u) atomicOr(fullw + (k >> 5), 1u << (k & 31));
+                            } else {
+                                ++k;   // otherwise a group past it, any slot
+                            }
+                        }
+                        while (g < 0) {   // the first group from k with a free slot
+                            const uint32_t fw = fullw[k >> 5] | ((1u << (k & 31)) - 1u);
+                            uint32_t gw = gmw[k >> 3];
+                            if (fw == 0xFFFFFFFFu) {
+                                k = (k | 31) + 1;
+                                continue;
+                            }
+                            const int kf = (k & ~31) + (int)__builtin_ctz(~fw);
+                            if (kf != k) {
+                                k = kf;
+                                gw = gmw[k >> 3];
+                            }
+                            const uint32_t sh = 4u * (uint32_t)(k & 7);
+                            const uint32_t nib = (gw >> sh) & 15u;
+                            if (nib == 15u) {   // filled in this round, its bit not set yet
+                                ++k;
+                                continue;
+                            }
+                            const uint32_t s = (uint32_t)__builtin_ctz(~nib & 15u), bit = (1u << s) << sh;
+                            const uint32_t old = atomicOr(gmw + (k >> 3), bit);
+                            if (!(old & bit)) {
+                                g = k;
+                                sl = s;
+                                if ((((old | bit) >> sh) & 15u) == 15u) atomicOr(fullw + (k >> 5), 1u << (k & 31));
+                            }
+                        }
+                        const uint32_t tag = ((uint32_t)(g + 2) << 2) | sl;
+                        lastR[r] = (IX)tag;
+                        lastC[c] = (IX)tag;
+                        gtop = max(gtop, g);
+                        emit(e, c, (uint32_t)g, sl, hit, LR, LC);
+                        todo = false;
+                    }
+                    lds_order();
+                }
+                continue;
+            }
             uint32_t myR = v < n ? (uint32_t)lastR[r] : 0u, myC = v < n ? (uint32_t)lastC[c] : 0u;
             en = fetch(v + 64);
             uint32_t res = 0;    // G << 3 | hreg << 2 | slot
@@ -425,24 +551,15 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                 lastR[r] = (IX)myR;   // lanes of one row hold the same final tag
                 lastC[c] = (IX)myC;
                 const uint32_t g = (res >> 3) - 2, sl = res & 3u;
-                // W_i from the LDS ring when the row was updated in the last GROUP_RING groups
-                const int gR = (int)(prevR >> 2) - 2;
-                const bool wf = prevR != 0u && (int)g - gR < GROUP_RING;
                 atomicOr(gmw + (g >> 3), (1u << sl) << (4 * (g & 7u)));
-                uint4 o;
-                o.x = (uint32_t)e.i |
-                      (wf ? (REC_WFWD | (((((uint32_t)gR & (GROUP_RING - 1)) << 2) | (prevR & 3u)) << REC_RING_SHIFT)) : 0u);
-                const bool hreg = ((res >> 2) & 1u) != 0;
-                o.y = c | (hreg ? REC_HREG : 0u);
-                if (wide && !hreg) {   // H_j from the H ring when the column was updated in the last GROUP_RING groups
-                    const int gC = (int)(prevC >> 2) - 2;
-                    if (prevC != 0u && (int)g - gC < GROUP_RING)
-                        o.y |= REC_HFWD | (((((uint32_t)gC & (GROUP_RING - 1)) << 2) | (prevC & 3u)) << REC_HRING_SHIFT);
-                }
-                rec_value(e, o);
-                grec[((uint64_t)J.gbase + g) * GROUP_SLOTS + sl] = o;
+                emit(e, c, g, sl, ((res >> 2) & 1u) != 0, prevR, prevC);
             }
             __syncthreads();
+        }
+        if (rounds) {   // the last group any lane placed a point in
+            int t = gtop;
+            for (int o = 32; o > 0; o >>= 1) t = max(t, __shfl_xor(t, o));
+            maxG = t + 2;
         }
         const int maxg = maxG - 2;
         const unsigned long long t2 = pprof ? clock64() : 0;
@@ -450,7 +567,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         const int ngp = (ng + GROUP_PAD - 1) / GROUP_PAD * GROUP_PAD;
         // 3. empty slots (and the padding groups) get the dummy record
         for (int g = lane; g < ngp; g += 64) {
-            const uint32_t mk = (gmw[g >> 3] >> (4 * (g & 7))) & 15u;
+            const uint32_t mk = (gmw[g >> 3] >> (4 * (g & 7))) & 15u & ~empty;
             uint4* gp = grec + ((uint64_t)J.gbase + g) * GROUP_SLOTS;
             for (int sl = 0; sl < GROUP_SLOTS; ++sl)
                 if (!((mk >> sl) & 1u)) gp[sl] = make_uint4(dummy_row, dummy_col, 0u, 0u);
@@ -478,14 +595,18 @@ __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__
                                                      const PT* __restrict__ pts, uint32_t* __restrict__ perm,
                                                      uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
                                                      int rows_cap, int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
-                                                     unsigned long long* __restrict__ pprof, int wide, int pre) {
+                                                     unsigned long long* __restrict__ pprof, int wide, int pre,
+                                                     int rounds) {
     extern __shared__ uint32_t sm[];
     prep_chain<IX, PT>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
-                       dummy_row, dummy_col, pprof, sm, wide != 0, pre != 0);
+                       dummy_row, dummy_col, pprof, sm, wide != 0, pre != 0, rounds != 0);
 }
 
 // Blocks whose prep state does not fit in LDS: the same prep with its state
-// in a global scratch slot per workgroup; a fixed grid walks the chains.
+// in a global scratch slot per workgroup; a fixed grid walks the chains.  The
+// launches pass rounds = 0: the round scan reads words that other lanes change
+// by atomics, which in global memory could be served from the CU's vector
+// cache.
 template <typename IX, typename PT>
 __global__ __launch_bounds__(64) void k_prep_grouped_spill(const PermJob* __restrict__ jobs,
                                                            const int32_t* __restrict__ chain_start, int nchains,
@@ -494,11 +615,11 @@ __global__ __launch_bounds__(64) void k_prep_grouped_spill(const PermJob* __rest
                                                            int32_t* __restrict__ ngroups, int cap, int rows_cap,
                                                            int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
                                                            unsigned long long* __restrict__ pprof, int wide,
-                                                           uint32_t* scratch, uint32_t slot_words) {
+                                                           uint32_t* scratch, uint32_t slot_words, int rounds) {
     uint32_t* const sm = scratch + (size_t)blockIdx.x * slot_words;
     for (int ch = blockIdx.x; ch < nchains; ch += gridDim.x)
         prep_chain<IX, PT>(jobs, chain_start, ch, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
-                           dummy_row, dummy_col, pprof, sm, wide != 0);
+                           dummy_row, dummy_col, pprof, sm, wide != 0, false, rounds != 0);
 }
 constexpr int PREP_SPILL_GRID = 2048;   // workgroups of a spilled prep launch (at most)
 
@@ -515,7 +636,8 @@ __global__ __launch_bounds__(64) void k_prep_grouped_multi(const PrepDesc* __res
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(PrepDesc) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
     prep_chain<IX, Pt32>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap,
-                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm, D.wide != 0, D.pre != 0);
+                         D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm, D.wide != 0, D.pre != 0,
+                         D.rounds != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -2081,16 +2203,31 @@ bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_bloc
     return no("not all tasks can be resident");
 }
 
+// Which scan places the points in groups: the sequential one (uniform values
+// on the scalar unit, one point at a time) or the round scan (the points of a
+// chunk in parallel, LDS atomics).  Measured on MI355X, 100M-nonzero workload:
+// a prep alone takes 14.5 ms with the round scan against 17.5 ms, but beside
+// the epoch kernel the round scan's vector instructions slow that kernel by
+// about 1% (29.9 against 29.6 ms), more than the hidden prep gains.  So a prep
+// beside an epoch (look-ahead, batches) scans sequentially, one that the
+// epoch waits for scans in rounds; DSGDPP_PREP_SCAN=seq|rounds sets both.
+static bool prep_rounds(bool beside) {
+    const char* e = getenv("DSGDPP_PREP_SCAN");
+    if (e && strcmp(e, "seq") == 0) return false;
+    if (e && strcmp(e, "rounds") == 0) return true;
+    return !beside;
+}
+
 template <typename IX, typename PT>
 static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                           const int32_t* d_chain_start, int nchains, int order, int cap, size_t bytes, bool pre,
-                          hipStream_t s) {
+                          bool rounds, hipStream_t s) {
     const void* fn = (const void*)k_prep_grouped<IX, PT>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
     k_prep_grouped<IX, PT><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const PT*)P.pts, P.perm,
                                                       (uint4*)B.grec, B.ngroups, cap, B.max_rows, B.max_cols, drow,
-                                                      dcol, B.pprof, B.wide ? 1 : 0, pre ? 1 : 0);
+                                                      dcol, B.pprof, B.wide ? 1 : 0, pre ? 1 : 0, rounds ? 1 : 0);
     HIPK(hipGetLastError());
 }
 
@@ -2146,13 +2283,15 @@ static void prep_spill_t(const DevProblem& P, const GroupedBuffers& B, const Per
     k_prep_grouped_spill<uint32_t, PT><<<grid, 64, 0, s>>>(d_jobs, d_chain_start, nchains, order, (const PT*)P.pts,
                                                            P.perm, (uint4*)B.grec, B.ngroups, cap, B.max_rows,
                                                            B.max_cols, drow, dcol, B.pprof, B.wide ? 1 : 0, B.spill,
-                                                           (uint32_t)(slot / 4));
+                                                           (uint32_t)(slot / 4), 0);
     HIPK(hipGetLastError());
 }
 
 void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
-                         const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s) {
+                         const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s,
+                         bool beside) {
     if (nchains <= 0) return;
+    const bool rounds = prep_rounds(beside);
     const int cap = (int)std::max<int64_t>(max_block, 1);
     const bool narrow = cap < PREP_NARROW;
     // WOR visit orders from the parallel kernel when its LDS allows; the prep
@@ -2168,11 +2307,11 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     }
     if (pre) launch_permute_lds(d_jobs, d_chain_start, nchains, P.perm, cap, s);
     if (f64) {
-        if (narrow) prep_launch_t<uint16_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
-        else prep_launch_t<uint32_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
+        if (narrow) prep_launch_t<uint16_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, rounds, s);
+        else prep_launch_t<uint32_t, Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, rounds, s);
     } else {
-        if (narrow) prep_launch_t<uint16_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
-        else prep_launch_t<uint32_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, s);
+        if (narrow) prep_launch_t<uint16_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, rounds, s);
+        else prep_launch_t<uint32_t, Pt32>(P, B, d_jobs, d_chain_start, nchains, order, cap, bytes, pre, rounds, s);
     }
 }
 
@@ -2192,6 +2331,7 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
     D.dummy_row = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2);
     D.dummy_col = (uint32_t)B.max_cols;
     D.wide = B.wide ? 1 : 0;
+    D.rounds = prep_rounds(true) ? 1 : 0;   // batch preps run beside the batch's epoch launch
     const bool narrow = D.cap < PREP_NARROW;
     D.pre = order == 2 && orders_first(true) && fy_bytes(D.cap) > 0 &&
             prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, true) <= PREP_LDS_PER_CU;

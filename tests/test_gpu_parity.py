@@ -766,3 +766,31 @@ def test_grouped_prep_visit_orders_either_way(monkeypatch, mode):
                 np.testing.assert_array_equal(g.permutation(t, s), o.permutation(t, s))
         np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"carry-over epoch {e}")
     g.close()
+
+
+@pytest.mark.parametrize("scan", ["seq", "rounds"])
+@pytest.mark.parametrize("precision,kernel,pairs,lookahead", [("FP32", 3, "0", "1"), ("FP32", 3, "1", "0"),
+                                                               ("FP32", 4, "0", "1"), ("BF16", 3, "0", "0"),
+                                                               ("FP64", 3, "0", "1")])
+def test_grouped_prep_scans_either_way(monkeypatch, scan, precision, kernel, pairs, lookahead):
+    """The grouped prep places the points in groups with the sequential scan
+    (a look-ahead prep's default) or the round scan (the points of a chunk in
+    parallel; a prep that the epoch waits for): any grouping that keeps each
+    row's and column's visit order gives the same factors, so both equal the
+    oracle's emulation bit for bit.  Covered: the LDS tile and the streaming
+    kernel (H forwarding entries), two-slot visits, look-ahead on and off,
+    FP32, BF16 and FP64, on cells with hot rows and columns."""
+    print(f"case: grouped prep scan={scan} {precision} kernel {kernel} pairs {pairs} lookahead {lookahead}",
+          flush=True)
+    monkeypatch.setenv("DSGDPP_PREP_SCAN", scan)
+    monkeypatch.setenv("DSGDPP_PAIRS", pairs)
+    monkeypatch.setenv("DSGDPP_LOOKAHEAD", lookahead)
+    p = skewed_problem(600, 400, 14000, 16, seed=91)
+    g, o = make_pair(p, update="Nzsl_L2(0.1)", tasks=3, seed=17, precision=precision, kernel=kernel)
+    o.set_arith({"FP32": "FP32", "BF16": "BF16", "FP64": "FP64_GROUPED"}[precision])
+    for e in range(3):
+        assert g.epoch(0.02)["kernel"] == kernel
+        o.epoch(0.02)
+        np.testing.assert_array_equal(g.get_factors()[0], o.get_factors()[0], err_msg=f"W epoch {e}")
+        np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"H epoch {e}")
+    g.close()
