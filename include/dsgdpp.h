@@ -236,9 +236,11 @@ int dsgdpp_dims(dsgdpp_ctx* ctx, int64_t* out);
  * column blocks are too wide for the LDS tile runs the H-streaming body,
  * kernel 4, inside the same launch); every context's result is bit-identical
  * to dsgdpp_epoch on it alone.  With use_graph the
- * launch is captured once and replayed each epoch.  The sum of the contexts'
- * tasks must fit on the device at once (E_UNSUPPORTED otherwise).  The
- * contexts stay owned by the caller and usable on their own.
+ * launch is captured once and replayed each epoch.  The launch has at most one
+ * workgroup per CU; when the contexts have more tasks than that, the library
+ * queues several tasks on each workgroup (longest work first, at most one task
+ * of a context per workgroup, every workgroup in one order of the contexts).
+ * The contexts stay owned by the caller and usable on their own.
  * --------------------------------------------------------------------------- */
 typedef struct dsgdpp_batch dsgdpp_batch;
 int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_graph, dsgdpp_batch** out);
