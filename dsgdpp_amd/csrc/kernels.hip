@@ -383,52 +383,110 @@ struct St4<bf16_t> {
     }
 };
 
-template <typename T>
+// The loss kernels give each point (each factor row) LPP adjacent lanes, the
+// smallest power of two with 4 LPP >= r (at most 64): lane k of a point loads
+// elements [4k, 4k + 4), [4k + 4 LPP, ...) of its W and H rows, so one wave
+// load instruction reads 64 / LPP whole rows instead of touching 64 rows.
+// A lane sums its products in element order; the point's lanes then add
+// their sums in a fixed butterfly (the association of nzsl.h:30-53 up to the
+// order of those partial sums: deterministic, within 1e-15 relative of the
+// sequential sum).
+inline int loss_lanes(int r) {
+    int l = 1;
+    while (l < 64 && 4 * l < r) l <<= 1;
+    return l;
+}
+
+template <int LPP>
+__device__ __forceinline__ double point_lanes_sum(double v) {
+#pragma unroll
+    for (int o = LPP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPP);
+    return v;
+}
+
+constexpr int NZSL_UNROLL = 4;   // points per lane group and pass: their loads overlap
+
+// squared errors of points first, first + step, ... below end (each point on
+// its LPP lanes, NZSL_UNROLL points per lane group at a time), added to acc
+// on the point's first lane
+template <typename T, int LPP>
+__device__ __forceinline__ void nzsl_range(const typename PtOf<T>::type* __restrict__ pts, int64_t first,
+                                           int64_t end, int64_t step, const T* __restrict__ W,
+                                           const T* __restrict__ H, int r, int64_t ld, double& acc) {
+    constexpr int PPB = RED_THREADS / LPP;   // points per workgroup and unrolled step
+    constexpr int U = NZSL_UNROLL;
+    const int sub = threadIdx.x % LPP;
+    // the lanes of one point share p0, so they run the same passes
+    for (int64_t p0 = first + threadIdx.x / LPP; p0 < end; p0 += step) {
+        typename PtOf<T>::type pt[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = p0 + (int64_t)u * PPB < end;
+            pt[u] = pts[ok[u] ? p0 + (int64_t)u * PPB : p0];
+        }
+        // four elements per load (rows are padded to a multiple of 4 elements)
+        double ip[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ip[u] = 0;
+        for (int z0 = 4 * sub; z0 < r; z0 += 4 * LPP) {
+            double wv[U][4], hv[U][4];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                St4<T>::wide(W + (int64_t)pt[u].i * ld + z0, wv[u]);
+                St4<T>::wide(H + (int64_t)pt[u].j * ld + z0, hv[u]);
+            }
+            const int k = min(4, r - z0);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                for (int e = 0; e < k; ++e) ip[u] += wv[u][e] * hv[u][e];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double t = point_lanes_sum<LPP>(ip[u]);
+            if (sub == 0 && ok[u]) {
+                const double diff = (double)pt[u].x - t;
+                acc += diff * diff;
+            }
+        }
+    }
+}
+
+// all points, grid-stride
+template <typename T, int LPP>
 __global__ __launch_bounds__(RED_THREADS) void k_nzsl(const typename PtOf<T>::type* __restrict__ pts,
                                                       int64_t nnz, const T* __restrict__ W,
                                                       const T* __restrict__ H, int r, int64_t ld,
                                                       double* __restrict__ partials) {
     __shared__ double sh[RED_THREADS / 64];
+    constexpr int64_t PASS = RED_THREADS / LPP * NZSL_UNROLL;
     double acc = 0;
-    for (int64_t p = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; p < nnz;
-         p += (int64_t)gridDim.x * RED_THREADS) {
-        const auto pt = pts[p];
-        const T* w = W + (int64_t)pt.i * ld;
-        const T* h = H + (int64_t)pt.j * ld;
-        // four elements per load (rows are padded to a multiple of 4 elements),
-        // summed one by one in element order as nzsl.h:30-53 does
-        double ip = 0;
-        for (int z0 = 0; z0 < r; z0 += 4) {
-            double wv[4], hv[4];
-            St4<T>::wide(w + z0, wv);
-            St4<T>::wide(h + z0, hv);
-            const int k = min(4, r - z0);
-            for (int e = 0; e < k; ++e) ip += wv[e] * hv[e];
-        }
-        const double diff = (double)pt.x - ip;
-        acc += diff * diff;
-    }
+    nzsl_range<T, LPP>(pts, (int64_t)blockIdx.x * PASS, nnz, (int64_t)gridDim.x * PASS, W, H, r, ld, acc);
     const double t = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 // sum_i weight_i * sum_z A[i,z]^2 over rows [lo, hi) (l2.h:40-44, nzl2.h:29-76)
-template <typename T>
+template <typename T, int LPP>
 __global__ __launch_bounds__(RED_THREADS) void k_sumsq(const T* __restrict__ A, int64_t lo, int64_t hi,
                                                        int r, int64_t ld,
                                                        const double* __restrict__ weight,
                                                        double* __restrict__ partials) {
     __shared__ double sh[RED_THREADS / 64];
+    constexpr int PPB = RED_THREADS / LPP;   // rows per workgroup pass
+    const int sub = threadIdx.x % LPP;
     double acc = 0;
-    for (int64_t i = lo + (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < hi;
-         i += (int64_t)gridDim.x * RED_THREADS) {
+    for (int64_t i = lo + (int64_t)blockIdx.x * PPB + threadIdx.x / LPP; i < hi; i += (int64_t)gridDim.x * PPB) {
         const T* a = A + i * ld;
         double v = 0;
-        for (int z = 0; z < r; ++z) {
-            const double e = St<T>::wide(a + z);
-            v += e * e;
+        for (int z0 = 4 * sub; z0 < r; z0 += 4 * LPP) {
+            double av[4];
+            St4<T>::wide(a + z0, av);
+            const int k = min(4, r - z0);
+            for (int e = 0; e < k; ++e) v += av[e] * av[e];
         }
-        acc += weight ? weight[i] * v : v;
+        v = point_lanes_sum<LPP>(v);
+        if (sub == 0) acc += weight ? weight[i] * v : v;
     }
     const double t = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -577,15 +635,57 @@ bool dataflow_fits(const DevProblem& P, int device) {
     return (int64_t)per_cu * cus >= P.d;
 }
 
+// workgroups of a loss launch over `items` points or rows, LPP lanes each
+// (k_nzsl takes 4 points per lane group and pass)
+static int red_blocks(int64_t items, int lpp, int unroll = 1) {
+    const int64_t per = RED_THREADS / lpp * unroll;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (items + per - 1) / per));
+}
+
+// the loss kernel over nnz points with the lane count for rank r; returns the
+// partial sums written
+template <typename T>
+static int launch_nzsl(const void* pts, int64_t nnz, const DevProblem& P, double* partials, hipStream_t s) {
+    const int lpp = loss_lanes((int)P.r), blocks = red_blocks(nnz, lpp, NZSL_UNROLL);
+    const auto* q = (const typename PtOf<T>::type*)pts;
+    const T *W = (const T*)P.W, *H = (const T*)P.H;
+    switch (lpp) {
+    case 1: k_nzsl<T, 1><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    case 2: k_nzsl<T, 2><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    case 4: k_nzsl<T, 4><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    case 8: k_nzsl<T, 8><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    case 16: k_nzsl<T, 16><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    case 32: k_nzsl<T, 32><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    default: k_nzsl<T, 64><<<blocks, RED_THREADS, 0, s>>>(q, nnz, W, H, (int)P.r, P.ld, partials); break;
+    }
+    HIPK(hipGetLastError());
+    return blocks;
+}
+
+// k_sumsq over rows [lo, hi) of A; returns the partial sums written
+template <typename T>
+static int launch_sumsq(const T* A, int64_t lo, int64_t hi, int r, int64_t ld, const double* weight,
+                        double* partials, hipStream_t s) {
+    const int lpp = loss_lanes(r), blocks = red_blocks(hi - lo, lpp);
+    switch (lpp) {
+    case 1: k_sumsq<T, 1><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    case 2: k_sumsq<T, 2><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    case 4: k_sumsq<T, 4><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    case 8: k_sumsq<T, 8><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    case 16: k_sumsq<T, 16><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    case 32: k_sumsq<T, 32><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    default: k_sumsq<T, 64><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, r, ld, weight, partials); break;
+    }
+    HIPK(hipGetLastError());
+    return blocks;
+}
+
 template <typename T>
 static double nzsl_t(const DevProblem& P, int which, double* d_partials, double* h_pinned,
                      hipStream_t s) {
     const void* pts = which == 0 ? P.pts : P.test_pts;
     const int64_t nnz = which == 0 ? P.nnz : P.test_nnz;
-    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (nnz + RED_THREADS - 1) / RED_THREADS));
-    k_nzsl<T><<<blocks, RED_THREADS, 0, s>>>((const typename PtOf<T>::type*)pts, nnz,
-                                             (const T*)P.W, (const T*)P.H, (int)P.r, P.ld, d_partials);
-    HIPK(hipGetLastError());
+    const int blocks = launch_nzsl<T>(pts, nnz, P, d_partials, s);
     k_sum_partials<<<1, 64, 0, s>>>(d_partials, blocks, d_partials + RED_BLOCKS);
     HIPK(hipGetLastError());
     HIPK(hipMemcpyAsync(h_pinned, d_partials + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -607,10 +707,7 @@ template <typename T>
 static double sumsq_t(const DevProblem& P, int which_factor, const double* d_weights, int64_t lo,
                       int64_t hi, double* d_partials, double* h_pinned, hipStream_t s) {
     const T* A = (const T*)(which_factor == 0 ? P.W : P.H);
-    const int64_t rows = hi - lo;
-    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (rows + RED_THREADS - 1) / RED_THREADS));
-    k_sumsq<T><<<blocks, RED_THREADS, 0, s>>>(A, lo, hi, (int)P.r, P.ld, d_weights, d_partials);
-    HIPK(hipGetLastError());
+    const int blocks = launch_sumsq<T>(A, lo, hi, (int)P.r, P.ld, d_weights, d_partials, s);
     k_sum_partials<<<1, 64, 0, s>>>(d_partials, blocks, d_partials + RED_BLOCKS);
     HIPK(hipGetLastError());
     HIPK(hipMemcpyAsync(h_pinned, d_partials + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -665,23 +762,19 @@ double* noalloc_scratch_lookup(int device) {
     return g_scratchPtr[device];
 }
 
-static int red_blocks(int64_t items) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (items + RED_THREADS - 1) / RED_THREADS));
-}
-
 template <typename T>
 static int64_t loss_noalloc_t(const DevProblem& P, int which, int kind, double lambda, double* sp, double* out,
                               hipStream_t s) {
     const void* pts = which == 0 ? P.pts : P.test_pts;
     const int64_t nnz = which == 0 ? P.nnz : P.test_nnz;
     const bool reg = which == 0 && kind != UPD_NZSL;
-    const int nb = red_blocks(nnz), wb = red_blocks(P.m), hb = red_blocks(P.n);
+    const int lpp = loss_lanes((int)P.r);
+    const int nb = red_blocks(nnz, lpp, NZSL_UNROLL), wb = red_blocks(P.m, lpp),
+              hb = red_blocks(P.n, lpp);
     const int64_t need = std::max(nb, reg ? std::max(wb, hb) : 0);
     if (need > RED_BLOCKS) return need;   // the array holds RED_BLOCKS partial sums
     double* const totals = sp + RED_BLOCKS;
-    k_nzsl<T><<<nb, RED_THREADS, 0, s>>>((const typename PtOf<T>::type*)pts, nnz, (const T*)P.W, (const T*)P.H,
-                                         (int)P.r, P.ld, sp);
-    HIPK(hipGetLastError());
+    launch_nzsl<T>(pts, nnz, P, sp, s);
     if (!reg) {
         k_sum_partials<<<1, 64, 0, s>>>(sp, nb, out);
         HIPK(hipGetLastError());
@@ -690,9 +783,9 @@ static int64_t loss_noalloc_t(const DevProblem& P, int which, int kind, double l
     k_sum_partials<<<1, 64, 0, s>>>(sp, nb, totals + 0);
     const double* w1 = kind == UPD_NZSL_NZL2 ? P.cnt1 : nullptr;
     const double* w2 = kind == UPD_NZSL_NZL2 ? P.cnt2 : nullptr;
-    k_sumsq<T><<<wb, RED_THREADS, 0, s>>>((const T*)P.W, 0, P.m, (int)P.r, P.ld, w1, sp);
+    launch_sumsq<T>((const T*)P.W, 0, P.m, (int)P.r, P.ld, w1, sp, s);
     k_sum_partials<<<1, 64, 0, s>>>(sp, wb, totals + 1);
-    k_sumsq<T><<<hb, RED_THREADS, 0, s>>>((const T*)P.H, 0, P.n, (int)P.r, P.ld, w2, sp);
+    launch_sumsq<T>((const T*)P.H, 0, P.n, (int)P.r, P.ld, w2, sp, s);
     k_sum_partials<<<1, 64, 0, s>>>(sp, hb, totals + 2);
     k_loss_combine<<<1, 64, 0, s>>>(totals, lambda, out);
     HIPK(hipGetLastError());
