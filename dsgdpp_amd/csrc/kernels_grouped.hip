@@ -93,16 +93,18 @@ __device__ __forceinline__ uint32_t lane_gather(uint32_t v, int src) {
 // workgroups per CU.
 constexpr int64_t PREP_NARROW = 8000;    // tag ((group + 2) << 2 | slot) < 2^16 with group < 2 n
 // pre: the visit orders come precomputed in perm (k_permute_lds), so the prep
-// keeps no visit-order array.  After the group slot masks: the round scan's
-// bitmap of full groups and its two ownership tables (PREP_HASH words each).
+// keeps no visit-order array.  rounds: after the group slot masks, the round
+// scan's bitmap of full groups and its two ownership tables (PREP_HASH words
+// each); only a prep that scans in rounds takes them (the LDS per workgroup
+// sets how many of the prep's waves share a CU).
 constexpr int PREP_HASH = 256;
 __host__ __device__ inline int prep_mask_words(int64_t cap) { return (int)((2 * cap + GROUP_PAD + 7) >> 3); }
 __host__ __device__ inline int prep_full_words(int64_t cap) { return (int)((2 * cap + GROUP_PAD + 31) >> 5) + 1; }
-static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow, bool pre = false) {
+static size_t prep_lds(int64_t cap, int rows, int cols, bool narrow, bool pre = false, bool rounds = false) {
     const size_t es = narrow ? 2 : 4;
     auto up4 = [](size_t b) { return (b + 3) & ~(size_t)3; };
     return (pre ? 0 : up4(cap * es)) + up4((size_t)rows * es) + up4((size_t)cols * es) +
-           ((size_t)prep_mask_words(cap) + prep_full_words(cap) + 2 * PREP_HASH) * 4;
+           ((size_t)prep_mask_words(cap) + (rounds ? prep_full_words(cap) + 2 * PREP_HASH : 0)) * 4;
 }
 
 // ---------------------------------------------------------------------------
@@ -2291,14 +2293,15 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
                          const int32_t* d_chain_start, int nchains, int order, int64_t max_block, hipStream_t s,
                          bool beside) {
     if (nchains <= 0) return;
-    const bool rounds = prep_rounds(beside);
     const int cap = (int)std::max<int64_t>(max_block, 1);
     const bool narrow = cap < PREP_NARROW;
     // WOR visit orders from the parallel kernel when its LDS allows; the prep
     // then keeps no visit-order array (more of its waves fit on a CU)
     const bool pre = order == 2 && orders_first(false) && fy_bytes(cap) > 0 &&
                      prep_lds(cap, B.max_rows, B.max_cols, narrow, true) <= PREP_LDS_PER_CU;
-    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow, pre);
+    // the round scan where its tables fit beside the state
+    const bool rounds = prep_rounds(beside) && prep_lds(cap, B.max_rows, B.max_cols, narrow, pre, true) <= PREP_LDS_PER_CU;
+    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow, pre, rounds);
     const bool f64 = P.precision == PREC_FP64 || P.precision == PREC_FP64_STRICT;
     if (bytes > PREP_LDS_PER_CU) {   // the state goes to global scratch
         if (f64) prep_spill_t<Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, s);
@@ -2331,17 +2334,18 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
     D.dummy_row = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2);
     D.dummy_col = (uint32_t)B.max_cols;
     D.wide = B.wide ? 1 : 0;
-    D.rounds = prep_rounds(true) ? 1 : 0;   // batch preps run beside the batch's epoch launch
     const bool narrow = D.cap < PREP_NARROW;
     D.pre = order == 2 && orders_first(true) && fy_bytes(D.cap) > 0 &&
             prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, true) <= PREP_LDS_PER_CU;
+    // batch preps run beside the batch's epoch launch
+    D.rounds = prep_rounds(true) && prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, D.pre != 0, true) <= PREP_LDS_PER_CU;
     return D;
 }
 
 bool prep_narrow(int64_t max_block) { return std::max<int64_t>(max_block, 1) < PREP_NARROW; }
 
 size_t prep_desc_lds(const PrepDesc& D) {
-    return prep_lds(D.cap, D.rows_cap, D.cols_cap, D.cap < PREP_NARROW, D.pre != 0);
+    return prep_lds(D.cap, D.rows_cap, D.cols_cap, D.cap < PREP_NARROW, D.pre != 0, D.rounds != 0);
 }
 
 size_t prep_desc_order_lds(const PrepDesc& D) { return D.pre ? fy_bytes(D.cap) : 0; }
