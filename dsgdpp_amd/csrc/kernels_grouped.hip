@@ -2208,9 +2208,9 @@ bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_bloc
 // Which scan places the points in groups: the sequential one (uniform values
 // on the scalar unit, one point at a time) or the round scan (the points of a
 // chunk in parallel, LDS atomics).  Measured on MI355X, 100M-nonzero workload:
-// a prep alone takes 14.5 ms with the round scan against 17.5 ms, but beside
-// the epoch kernel the round scan's vector instructions slow that kernel by
-// about 1% (29.9 against 29.6 ms), more than the hidden prep gains.  So a prep
+// a prep alone takes 14.5 ms with the round scan against 16.0 ms, but beside
+// the epoch kernel (both 18.2 ms there) the round scan's vector instructions
+// slow that kernel by about 1.5% (29.9 against 29.5 ms).  So a prep
 // beside an epoch (look-ahead, batches) scans sequentially, one that the
 // epoch waits for scans in rounds; DSGDPP_PREP_SCAN=seq|rounds sets both.
 static bool prep_rounds(bool beside) {
