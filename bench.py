@@ -516,11 +516,11 @@ def bench_batch(args):
     rates = B.measure_rates(log=log)
     meta = [dict(m=p["m"], n=p["n"], nnz=len(p["train"][2]), rank=p["rank"]) for p in probs]
     budget = torch.cuda.get_device_properties(0).multi_processor_count   # one resident task per CU
-    # kernel 4 streams H from HBM instead of tiling it in LDS: the batch launch
-    # then leaves most of each CU's LDS to the look-ahead preps, which run beside
-    # it (measured: 217 against 250 ms per epoch with the LDS tile)
-    kernel = 4 if args.kernel is None else args.kernel
     d = B.plan_tasks(meta, budget, rates)
+    # the planner's kernel: on batch64 the streaming kernel 4, whose launch
+    # leaves most of each CU's LDS to the look-ahead preps beside it
+    kernel = B.batch_kernel(meta, d) if args.kernel is None else args.kernel
+    log(f"[bench] batch kernel {kernel}")
     log(f"[bench] tasks per problem: {d} (sum {sum(d)}), one launch per epoch")
     t0 = time.time()
     ctxs = []

@@ -8,13 +8,17 @@ one launch of the grouped kernel, bit-identical to running each context's
 ``epoch`` alone.  The launch is captured into a HIP graph once and replayed.
 
 Autotuner.  Per problem the free choice is the number of tasks d (d x 2d
-blocking: d workgroups of one wavefront each, and an H tile of n / 2d
-columns x 64 or 128 floats in LDS, by rank).  All tasks of a batch must be
-resident at once, so the tuner splits the device's task budget: it measures,
-once per rank class on a probe problem, the update rate per task for each
-candidate d (cached on disk), predicts every problem's epoch time for each
-candidate, and repeatedly doubles d of the problem with the largest
-predicted time while the budget and the problem's shape allow.
+blocking, and an H tile of n / 2d columns x 64 or 128 floats in LDS, by
+rank).  The launch has at most one workgroup per CU and the library queues
+tasks beyond that on the workgroups, so the tuner aims at the device's task
+budget: it measures, once per rank class on a probe problem, the update rate
+per task for each candidate d (cached on disk), predicts every problem's
+epoch time for each candidate, and repeatedly doubles d of the problem with
+the largest predicted time while the budget and the problem's shape allow.
+Per batch it picks the kernel (``batch_kernel``): the launch's workgroups
+take the LDS of the widest problem's tile, and when that leaves no room for
+a prep workgroup the batch streams H from HBM instead (kernel 4), so the next
+epoch's preps run beside the launch.
 """
 import ctypes
 import json
@@ -205,6 +209,25 @@ def plan_tasks(problems, budget, rates, wide=True):
             break
         if not grown:
             return d
+
+
+def batch_kernel(problems, tasks):
+    """Kernel for the batch's contexts: 4 (H rows streamed from HBM, at most
+    64 KB of LDS per workgroup) when the widest problem's LDS tile leaves a CU
+    too little LDS for the largest prep workgroup beside it, else 0 (the
+    tile where it fits).  The preps of the next epoch start in order of LDS
+    on one stream; with no room beside the launch they all wait for it
+    (batch64 on MI355X: 251 ms per epoch on the tile against 217.7 ms
+    streaming, whose launch takes 150 against 136 ms)."""
+    def own(p, d):   # a problem whose tile does not fit streams anyway
+        t = lds_need(p["n"], d, p["rank"])
+        return t if t <= LDS_BYTES else lds_need(p["n"], d, p["rank"], wide=True)
+
+    tile = max(own(p, d) for p, d in zip(problems, tasks))
+    # a prep whose state exceeds a CU's LDS keeps it in global memory
+    preps = [prep_lds_estimate(p["m"], p["n"], p["nnz"], d) for p, d in zip(problems, tasks)]
+    prep = max([q for q in preps if q <= LDS_BYTES], default=0)
+    return 4 if tile + prep > LDS_BYTES else 0
 
 
 def predict_time(p, d, rates):

@@ -56,10 +56,33 @@ def test_rank_class():
         B.rank_class(129)
 
 
-def _ctx(p, tasks, seed, lam, precision="FP32"):
+def test_batch_kernel_streams_when_the_tile_leaves_no_room_for_a_prep():
+    """The batch's workgroups take the LDS of its widest tile: small tiles
+    keep the tile kernel (0); a tile that leaves a CU less LDS than the
+    largest prep workgroup needs switches the batch to streaming H (4).  A
+    problem whose tile does not fit at all streams anyway and counts with
+    the streaming kernel's LDS."""
+    small = [dict(m=2000, n=1000, nnz=20000, rank=16)] * 4
+    assert B.batch_kernel(small, [4] * 4) == 0
+    # 2 x 300 columns x 128 floats: about 340 KB of tile, never fits -> counts as streaming
+    huge = dict(m=4000, n=4 * 300, nnz=20000, rank=128)
+    assert B.lds_need(huge["n"], 2, 128) > B.LDS_BYTES
+    assert B.batch_kernel(small + [huge], [4] * 4 + [2]) == 0
+    # a tile of about 150 KB fits on its own, but leaves no room for the prep beside it
+    wide = dict(m=8000, n=2 * 4 * 116, nnz=64000, rank=128)
+    t = B.lds_need(wide["n"], 4, 128)
+    assert t <= B.LDS_BYTES and t + B.prep_lds_estimate(wide["m"], wide["n"], wide["nnz"], 4) > B.LDS_BYTES
+    assert B.batch_kernel(small + [wide], [4] * 4 + [4]) == 4
+    # a prep too large for LDS keeps its state in global memory and takes no LDS
+    assert B.prep_lds_estimate(400000, 1000, 40000, 4) > B.LDS_BYTES
+    assert B.batch_kernel(small + [dict(m=400000, n=1000, nnz=40000, rank=16)], [4] * 5) == 0
+
+
+def _ctx(p, tasks, seed, lam, precision="FP32", kernel=0):
     from dsgdpp_amd import TRAIN, api
     upd = f"Nzsl_L2({lam})"
-    g = api.DsgdPp(update=upd, loss=upd, rank=p["rank"], tasks_per_rank=tasks, seed=seed, precision=precision)
+    g = api.DsgdPp(update=upd, loss=upd, rank=p["rank"], tasks_per_rank=tasks, seed=seed, precision=precision,
+                   kernel=kernel)
     g.load_coo(TRAIN, p["m"], p["n"], *p["train"])
     g.set_factors(p["W0"], p["H0"])
     return g
@@ -86,18 +109,23 @@ def _oracle(p, tasks, seed, lam, precision="FP32"):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_graph,pairs,precision", [(True, None, "FP32"), (False, None, "FP32"),
-                                                       (True, "1", "FP32"), (True, None, "BF16")])
-def test_batch_bitexact_vs_oracle(monkeypatch, use_graph, pairs, precision):
+@pytest.mark.parametrize("use_graph,pairs,precision,kernel", [(True, None, "FP32", 0), (False, None, "FP32", 0),
+                                                              (True, "1", "FP32", 0), (True, None, "BF16", 0),
+                                                              (True, None, "FP32", 4), (True, None, "BF16", 4)])
+def test_batch_bitexact_vs_oracle(monkeypatch, use_graph, pairs, precision, kernel):
     """Every problem of a batch ends each epoch bit-identical to the oracle's
     emulation of the same precision (same schedules and visit orders, the
     grouped kernel's association), with and without the captured graph.
-    pairs "1": every eligible block (FP32, rank <= 64) in two-slot groups."""
-    print(f"case: batch of 4 (ranks 16/64/96/128) {precision}, graph={use_graph}, pairs={pairs}", flush=True)
+    pairs "1": every eligible block (FP32, rank <= 64) in two-slot groups.
+    kernel 4: every context streams H (the batch's choice when its widest
+    tile would leave no LDS for the look-ahead preps beside the launch)."""
+    print(f"case: batch of 4 (ranks 16/64/96/128) {precision}, graph={use_graph}, pairs={pairs}, kernel={kernel}",
+          flush=True)
     if pairs is not None:
         monkeypatch.setenv("DSGDPP_PAIRS", pairs)
     probs = _problems()
-    batch_ctx = [_ctx(p, d, seed=10 + k, lam=lam, precision=precision) for k, (p, d, lam) in enumerate(probs)]
+    batch_ctx = [_ctx(p, d, seed=10 + k, lam=lam, precision=precision, kernel=kernel)
+                 for k, (p, d, lam) in enumerate(probs)]
     orcs = [_oracle(p, d, seed=10 + k, lam=lam, precision=precision) for k, (p, d, lam) in enumerate(probs)]
     bt = B.DsgdPpBatch(batch_ctx, use_graph=use_graph)
     eps = np.array([0.01, 0.02, 0.005, 0.015])
