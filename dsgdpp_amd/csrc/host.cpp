@@ -219,6 +219,7 @@ struct dsgdpp_ctx {
     DevProblem P;
     void* dPts[2] = {nullptr, nullptr};
     int64_t* dBlk[2] = {nullptr, nullptr};
+    void* dRid = nullptr;   // training points' row ranks in their blocks (grouped prep)
     double* dPartials = nullptr;
     double* hPinned = nullptr;
     int32_t* dReady = nullptr;
@@ -525,6 +526,13 @@ void finishBlocks(dsgdpp_ctx* c, int which) {
     if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
     choosePairs(c);
     if (c->groupedOk) {
+        // the prep's row tags are kept by row rank: one per row with points
+        dfree(c->dRid);
+        c->G.rid = nullptr;
+        dalloc(c->dRid, (size_t)std::max<int64_t>(hb.nnz, 1) * sizeof(uint16_t));
+        c->G.rid = (const uint16_t*)c->dRid;
+        c->G.prep_rows = launch_row_ranks(c->dPts[0], c->fp64(), c->dBlk[0], (int64_t)hb.start.size() - 1,
+                                          (uint16_t*)c->dRid, c->stream);
         std::vector<int32_t> o2(c->off2.begin(), c->off2.end());
         int32_t* d2p = const_cast<int32_t*>(c->G.off2);
         dalloc(d2p, o2.size() * sizeof(int32_t));
@@ -913,8 +921,8 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
                           c->sgdOrder == DSGDPP_SGD_WOR ? P.perm : nullptr, s);
     } else if (P.grouped) {
         // blocks whose prep state does not fit in LDS keep it in a scratch buffer
-        const size_t spill = std::max(prep_spill_bytes(P.capSmall, G.max_rows, G.max_cols, P.nSmall),
-                                      prep_spill_bytes(maxBlock, G.max_rows, G.max_cols, nchains - P.nSmall));
+        const size_t spill = std::max(prep_spill_bytes(P.capSmall, G.prep_rows, G.max_cols, P.nSmall),
+                                      prep_spill_bytes(maxBlock, G.prep_rows, G.max_cols, nchains - P.nSmall));
         if (spill > c->G.spill_bytes) {
             HIPC(hipStreamSynchronize(s));   // a prep of the other plan may still use the old one
             void* q = c->G.spill;
@@ -1606,6 +1614,7 @@ void dsgdpp_destroy(dsgdpp_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->prepStream) (void)hipStreamSynchronize(c->prepStream);
     for (int k = 0; k < 2; ++k) { dfree(c->dPts[k]); void* p = c->dBlk[k]; dfree(p); }
+    dfree(c->dRid);
     { void* p = c->G.spill; dfree(p); c->G.spill = nullptr; }
     for (EpochPlan& P : c->plan) {
         void* q;
@@ -2213,7 +2222,7 @@ int dsgdpp_batch_create(dsgdpp_ctx* const* ctxs, int32_t count, int32_t use_grap
             if (!c->groupedOk || !(c->kernel == 0 || c->kernel == 3 || c->kernel == 4))
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) + " cannot use the grouped kernel: " +
                                                       (c->groupedOk ? std::string("kernel forced") : c->groupedWhy));
-            if (prep_spill_bytes(c->maxBlock, c->G.max_rows, c->G.max_cols, 1) > 0)
+            if (prep_spill_bytes(c->maxBlock, c->G.prep_rows, c->G.max_cols, 1) > 0)
                 throw Error(DSGDPP_E_UNSUPPORTED, "context " + std::to_string(k) +
                                                       ": its blocks' prep state does not fit in LDS (batched preps "
                                                       "keep it there; use more tasks)");

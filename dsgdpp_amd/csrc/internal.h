@@ -158,6 +158,8 @@ struct GroupedBuffers {
     uint32_t* spill = nullptr;            // prep state of blocks too large for LDS (prep_spill_bytes)
     const uint8_t* pairs = nullptr;       // per block (t * 2d + b2): its visits run two-slot groups
     size_t spill_bytes = 0;
+    const uint16_t* rid = nullptr;        // per training point: its row's rank among its block's rows with points
+    int prep_rows = 0;                    // the most rows with points in one block (the prep's row tags)
 };
 // The prep kernel's inputs for one problem (many problems share one launch).
 struct PrepDesc {
@@ -172,6 +174,7 @@ struct PrepDesc {
     int32_t wide;   // H forwarding entries for the streaming kernel
     int32_t pre;    // WOR visit orders precomputed into perm (k_permute_lds_multi) before the prep
     int32_t rounds; // the round scan places the points (else the sequential scan)
+    const uint16_t* rid;   // per point: its row's rank in its block
 };
 static_assert(sizeof(PrepDesc) % 4 == 0, "descriptor is read as words");
 // grouped epoch eligibility; *wide (optional) is set when only the variant
@@ -218,6 +221,11 @@ size_t prep_desc_order_lds(const PrepDesc& D);
 // scratch the prep of `nchains` chains of blocks up to max_block points needs
 // in global memory (0 when its state fits in LDS)
 size_t prep_spill_bytes(int64_t max_block, int max_rows, int max_cols, int nchains);
+// rid[q] = rank of point q's row among the rows with points of its block, for
+// the nblocks blocks of d_blk (training records: Pt64 when pt64); returns the
+// largest count of such rows in a block (synchronizes s)
+int launch_row_ranks(const void* pts, bool pt64, const int64_t* d_blk, int64_t nblocks, uint16_t* rid,
+                     hipStream_t s);
 // order_lds > 0: first the visit orders of the chains whose descriptor has pre
 // set (narrow: 16-bit indices, as the prep's)
 void launch_prep_grouped_multi(const PrepDesc* d_descs, const int2* d_cmap, int nchains, bool narrow, size_t lds,

@@ -86,6 +86,54 @@ __device__ __forceinline__ uint32_t lane_gather(uint32_t v, int src) {
 }
 
 // ---------------------------------------------------------------------------
+// Row ranks for the grouped prep: rid[q] = the rank of point q's row among the
+// rows of its block that hold points (a block's points are sorted by row), so
+// the prep keeps a row's tag at that rank and needs an entry per row with
+// points, not per row of the row block.  One workgroup per block; *maxr gets
+// the largest count of such rows in a block.
+// ---------------------------------------------------------------------------
+template <typename PT>
+__global__ __launch_bounds__(256) void k_row_ranks(const PT* __restrict__ pts, const int64_t* __restrict__ blk,
+                                                   uint16_t* __restrict__ rid, int* __restrict__ maxr) {
+    __shared__ int wsum[4];
+    const int64_t s0 = blk[blockIdx.x], s1 = blk[blockIdx.x + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int carry = 0;
+    for (int64_t c0 = s0; c0 < s1; c0 += 256) {
+        const int64_t q = c0 + threadIdx.x;
+        int x = (q < s1 && (q == s0 || pts[q].i != pts[q - 1].i)) ? 1 : 0;   // a new row starts at q
+        for (int o = 1; o < 64; o <<= 1) {   // inclusive scan in the wave
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        int off = 0;
+        for (int k = 0; k < w; ++k) off += wsum[k];
+        if (q < s1) rid[q] = (uint16_t)(carry + off + x - 1);
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && carry > 0) atomicMax(maxr, carry);
+}
+
+int launch_row_ranks(const void* pts, bool pt64, const int64_t* d_blk, int64_t nblocks, uint16_t* rid,
+                     hipStream_t s) {
+    if (nblocks <= 0) return 0;
+    int* d_max = nullptr;
+    HIPK(hipMalloc((void**)&d_max, sizeof(int)));
+    k_zero_i32<<<1, 64, 0, s>>>(d_max, 1);
+    if (pt64) k_row_ranks<Pt64><<<(unsigned)nblocks, 256, 0, s>>>((const Pt64*)pts, d_blk, rid, d_max);
+    else k_row_ranks<Pt32><<<(unsigned)nblocks, 256, 0, s>>>((const Pt32*)pts, d_blk, rid, d_max);
+    HIPK(hipGetLastError());
+    int h = 0;
+    HIPK(hipMemcpyAsync(&h, d_max, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPK(hipStreamSynchronize(s));
+    HIPK(hipFree(d_max));
+    return h;
+}
+
+// ---------------------------------------------------------------------------
 // prep: visit order, groups and packed records
 // ---------------------------------------------------------------------------
 // LDS of the prep kernel.  Narrow mode (blocks below PREP_NARROW points) keeps
@@ -265,7 +313,7 @@ __device__ __forceinline__ void rec_value(const Pt64& e, uint4& o) {
 template <typename IX, typename PT>
 __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, const int32_t* __restrict__ chain_start,
                                            const int chain, const int order, const PT* __restrict__ pts,
-                                           uint32_t* __restrict__ perm, uint4* __restrict__ grec,
+                                           const uint16_t* __restrict__ rid, uint32_t* __restrict__ perm, uint4* __restrict__ grec,
                                            int32_t* __restrict__ ngroups, const int cap, const int rows_cap,
                                            const int cols_cap, const uint32_t dummy_row, const uint32_t dummy_col,
                                            unsigned long long* __restrict__ pprof, uint32_t* sm,
@@ -276,7 +324,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
     const bool own = !(pre && order == 2);
     const int vw = own ? up4(cap) : 0;
     IX* vis = (IX*)sm;                               // visit order
-    IX* lastR = (IX*)(sm + vw);                      // per local row: tag of its last update
+    IX* lastR = (IX*)(sm + vw);                      // per row of the block (by row rank): tag of its last update
     IX* lastC = (IX*)(sm + vw + up4(rows_cap));      // per local column: the same
     uint32_t* gmw = sm + vw + up4(rows_cap) + up4(cols_cap);   // group slot masks, 4 bits each
     uint32_t* fullw = gmw + prep_mask_words(cap);               // round scan: bit g = group g has no free slot
@@ -367,12 +415,14 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         int maxG = 1;
         uint32_t win = 0;    // lane b: G << 4 | slot mask of the window's group G in lane b
         uint64_t full = 0;   // bit b: that group has no free slot
-        // the next chunk's points are fetched while the current chunk is scanned
-        // (a training point carries 1/nnz of its row in .aux)
+        // the next chunk's points, and their rows' ranks in the block, are
+        // fetched while the current chunk is scanned
+        uint32_t rn = 0;
         auto fetch = [&](int v) {
             PT e{};
             e.i = J.row0;
             e.j = J.col0;
+            rn = 0;
             if (v < n) {
                 uint32_t q;
                 if (own) {
@@ -382,6 +432,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
                     q = perm[J.base + v];
                 }
                 e = pts[J.s0 + q];
+                rn = rid[J.s0 + q];
             }
             return e;
         };
@@ -409,7 +460,7 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
         for (int v0 = 0; v0 < n; v0 += 64) {
             const int v = v0 + lane;
             const PT e = en;
-            const uint32_t r = (uint32_t)(e.i - J.row0), c = (uint32_t)(e.j - J.col0);
+            const uint32_t r = rn, c = (uint32_t)(e.j - J.col0);
             if (rounds) {
                 en = fetch(v + 64);
                 // 2'. the same placement rule in rounds, the chunk's points in
@@ -594,13 +645,14 @@ __device__ __forceinline__ void prep_chain(const PermJob* __restrict__ jobs, con
 template <typename IX, typename PT>
 __global__ __launch_bounds__(64) void k_prep_grouped(const PermJob* __restrict__ jobs,
                                                      const int32_t* __restrict__ chain_start, int order,
-                                                     const PT* __restrict__ pts, uint32_t* __restrict__ perm,
+                                                     const PT* __restrict__ pts, const uint16_t* __restrict__ rid,
+                                                     uint32_t* __restrict__ perm,
                                                      uint4* __restrict__ grec, int32_t* __restrict__ ngroups, int cap,
                                                      int rows_cap, int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
                                                      unsigned long long* __restrict__ pprof, int wide, int pre,
                                                      int rounds) {
     extern __shared__ uint32_t sm[];
-    prep_chain<IX, PT>(jobs, chain_start, blockIdx.x, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
+    prep_chain<IX, PT>(jobs, chain_start, blockIdx.x, order, pts, rid, perm, grec, ngroups, cap, rows_cap, cols_cap,
                        dummy_row, dummy_col, pprof, sm, wide != 0, pre != 0, rounds != 0);
 }
 
@@ -613,14 +665,14 @@ template <typename IX, typename PT>
 __global__ __launch_bounds__(64) void k_prep_grouped_spill(const PermJob* __restrict__ jobs,
                                                            const int32_t* __restrict__ chain_start, int nchains,
                                                            int order, const PT* __restrict__ pts,
-                                                           uint32_t* __restrict__ perm, uint4* __restrict__ grec,
+                                                           const uint16_t* __restrict__ rid, uint32_t* __restrict__ perm, uint4* __restrict__ grec,
                                                            int32_t* __restrict__ ngroups, int cap, int rows_cap,
                                                            int cols_cap, uint32_t dummy_row, uint32_t dummy_col,
                                                            unsigned long long* __restrict__ pprof, int wide,
                                                            uint32_t* scratch, uint32_t slot_words, int rounds) {
     uint32_t* const sm = scratch + (size_t)blockIdx.x * slot_words;
     for (int ch = blockIdx.x; ch < nchains; ch += gridDim.x)
-        prep_chain<IX, PT>(jobs, chain_start, ch, order, pts, perm, grec, ngroups, cap, rows_cap, cols_cap,
+        prep_chain<IX, PT>(jobs, chain_start, ch, order, pts, rid, perm, grec, ngroups, cap, rows_cap, cols_cap,
                            dummy_row, dummy_col, pprof, sm, wide != 0, false, rounds != 0);
 }
 constexpr int PREP_SPILL_GRID = 2048;   // workgroups of a spilled prep launch (at most)
@@ -637,7 +689,7 @@ __global__ __launch_bounds__(64) void k_prep_grouped_multi(const PrepDesc* __res
     uint32_t* dst = (uint32_t*)&D;
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(PrepDesc) / 4); ++k) dst[k] = (uint32_t)uniform_i32((int)src[k]);
-    prep_chain<IX, Pt32>(D.jobs, D.chain_start, chain, D.order, D.pts, D.perm, (uint4*)D.grec, D.ngroups, D.cap,
+    prep_chain<IX, Pt32>(D.jobs, D.chain_start, chain, D.order, D.pts, D.rid, D.perm, (uint4*)D.grec, D.ngroups, D.cap,
                          D.rows_cap, D.cols_cap, D.dummy_row, D.dummy_col, nullptr, sm, D.wide != 0, D.pre != 0,
                          D.rounds != 0);
 }
@@ -2227,8 +2279,8 @@ static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const Pe
     const void* fn = (const void*)k_prep_grouped<IX, PT>;
     HIPK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
-    k_prep_grouped<IX, PT><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const PT*)P.pts, P.perm,
-                                                      (uint4*)B.grec, B.ngroups, cap, B.max_rows, B.max_cols, drow,
+    k_prep_grouped<IX, PT><<<nchains, 64, bytes, s>>>(d_jobs, d_chain_start, order, (const PT*)P.pts, B.rid, P.perm,
+                                                      (uint4*)B.grec, B.ngroups, cap, B.prep_rows, B.max_cols, drow,
                                                       dcol, B.pprof, B.wide ? 1 : 0, pre ? 1 : 0, rounds ? 1 : 0);
     HIPK(hipGetLastError());
 }
@@ -2277,13 +2329,13 @@ size_t prep_spill_bytes(int64_t max_block, int max_rows, int max_cols, int nchai
 template <typename PT>
 static void prep_spill_t(const DevProblem& P, const GroupedBuffers& B, const PermJob* d_jobs,
                          const int32_t* d_chain_start, int nchains, int order, int cap, hipStream_t s) {
-    const size_t slot = prep_lds(cap, B.max_rows, B.max_cols, false);
+    const size_t slot = prep_lds(cap, B.prep_rows, B.max_cols, false);
     const int grid = std::min(nchains, PREP_SPILL_GRID);
     if (!B.spill || B.spill_bytes < slot * (size_t)grid)
         throw std::runtime_error("prep scratch smaller than prep_spill_bytes");
     const uint32_t drow = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2), dcol = (uint32_t)B.max_cols;
     k_prep_grouped_spill<uint32_t, PT><<<grid, 64, 0, s>>>(d_jobs, d_chain_start, nchains, order, (const PT*)P.pts,
-                                                           P.perm, (uint4*)B.grec, B.ngroups, cap, B.max_rows,
+                                                           B.rid, P.perm, (uint4*)B.grec, B.ngroups, cap, B.prep_rows,
                                                            B.max_cols, drow, dcol, B.pprof, B.wide ? 1 : 0, B.spill,
                                                            (uint32_t)(slot / 4), 0);
     HIPK(hipGetLastError());
@@ -2298,10 +2350,10 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     // WOR visit orders from the parallel kernel when its LDS allows; the prep
     // then keeps no visit-order array (more of its waves fit on a CU)
     const bool pre = order == 2 && orders_first(false) && fy_bytes(cap) > 0 &&
-                     prep_lds(cap, B.max_rows, B.max_cols, narrow, true) <= PREP_LDS_PER_CU;
+                     prep_lds(cap, B.prep_rows, B.max_cols, narrow, true) <= PREP_LDS_PER_CU;
     // the round scan where its tables fit beside the state
-    const bool rounds = prep_rounds(beside) && prep_lds(cap, B.max_rows, B.max_cols, narrow, pre, true) <= PREP_LDS_PER_CU;
-    const size_t bytes = prep_lds(cap, B.max_rows, B.max_cols, narrow, pre, rounds);
+    const bool rounds = prep_rounds(beside) && prep_lds(cap, B.prep_rows, B.max_cols, narrow, pre, true) <= PREP_LDS_PER_CU;
+    const size_t bytes = prep_lds(cap, B.prep_rows, B.max_cols, narrow, pre, rounds);
     const bool f64 = P.precision == PREC_FP64 || P.precision == PREC_FP64_STRICT;
     if (bytes > PREP_LDS_PER_CU) {   // the state goes to global scratch
         if (f64) prep_spill_t<Pt64>(P, B, d_jobs, d_chain_start, nchains, order, cap, s);
@@ -2329,7 +2381,8 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
     D.ngroups = B.ngroups;
     D.order = order;
     D.cap = (int32_t)std::max<int64_t>(max_block, 1);
-    D.rows_cap = B.max_rows;
+    D.rid = B.rid;
+    D.rows_cap = B.prep_rows;
     D.cols_cap = B.max_cols;
     D.dummy_row = (uint32_t)(P.m + FACTOR_TAIL_ROWS / 2);
     D.dummy_col = (uint32_t)B.max_cols;
