@@ -2261,10 +2261,12 @@ bool grouped_supported(const DevProblem& P, const UpdParams& U, int64_t max_bloc
 // on the scalar unit, one point at a time) or the round scan (the points of a
 // chunk in parallel, LDS atomics).  Measured on MI355X, 100M-nonzero workload:
 // a prep alone takes 14.5 ms with the round scan against 16.0 ms, but beside
-// the epoch kernel (both 18.2 ms there) the round scan's vector instructions
-// slow that kernel by about 1.5% (29.9 against 29.5 ms).  So a prep
-// beside an epoch (look-ahead, batches) scans sequentially, one that the
-// epoch waits for scans in rounds; DSGDPP_PREP_SCAN=seq|rounds sets both.
+// the epoch kernel, where it is hidden (both about 18 ms there), the round
+// scan's vector instructions slow that kernel by about 1.5% (29.9 against
+// 29.5 ms).  So a look-ahead prep beside a context's epoch scans sequentially;
+// one that the epoch waits for scans in rounds, and so do a batch's preps,
+// whose last ones end after the batch's launch (batch64: 181.9 against 194.4
+// ms per epoch).  DSGDPP_PREP_SCAN=seq|rounds sets all.
 static bool prep_rounds(bool beside) {
     const char* e = getenv("DSGDPP_PREP_SCAN");
     if (e && strcmp(e, "seq") == 0) return false;
@@ -2390,8 +2392,8 @@ PrepDesc prep_desc(const DevProblem& P, const GroupedBuffers& B, const PermJob* 
     const bool narrow = D.cap < PREP_NARROW;
     D.pre = order == 2 && orders_first(true) && fy_bytes(D.cap) > 0 &&
             prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, true) <= PREP_LDS_PER_CU;
-    // batch preps run beside the batch's epoch launch
-    D.rounds = prep_rounds(true) && prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, D.pre != 0, true) <= PREP_LDS_PER_CU;
+    // batch preps: the round scan (their tail after the launch is on the critical path)
+    D.rounds = prep_rounds(false) && prep_lds(D.cap, D.rows_cap, D.cols_cap, narrow, D.pre != 0, true) <= PREP_LDS_PER_CU;
     return D;
 }
 
