@@ -794,3 +794,24 @@ def test_grouped_prep_scans_either_way(monkeypatch, scan, precision, kernel, pai
         np.testing.assert_array_equal(g.get_factors()[0], o.get_factors()[0], err_msg=f"W epoch {e}")
         np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"H epoch {e}")
     g.close()
+
+
+@pytest.mark.parametrize("scan", ["seq", "rounds"])
+def test_grouped_prep_scans_32bit_tags(monkeypatch, scan):
+    """Blocks of 8,000 points and more: the prep keeps its tags and visit
+    orders in 32 bits.  Both scans give factors bit-identical to the oracle's
+    FP32 emulation, look-ahead on and off."""
+    print(f"case: grouped prep scan={scan}, 20,000-point blocks (32-bit tags)", flush=True)
+    monkeypatch.setenv("DSGDPP_PREP_SCAN", scan)
+    p = skewed_problem(600, 400, 40000, 16, seed=13)
+    for la in ("1", "0"):
+        monkeypatch.setenv("DSGDPP_LOOKAHEAD", la)
+        g, o = make_pair(p, update="Nzsl_L2(0.1)", tasks=1, seed=23, precision="FP32", kernel=3)
+        o.set_arith("FP32")
+        assert g.block_nnz().max() >= 8000
+        for e in range(2):
+            assert g.epoch(0.01)["kernel"] == 3
+            o.epoch(0.01)
+            np.testing.assert_array_equal(g.get_factors()[0], o.get_factors()[0], err_msg=f"W epoch {e} la {la}")
+            np.testing.assert_array_equal(g.get_factors()[1], o.get_factors()[1], err_msg=f"H epoch {e} la {la}")
+        g.close()
