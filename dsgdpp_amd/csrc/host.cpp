@@ -454,8 +454,9 @@ void chooseLevel(dsgdpp_ctx* c) {
     const double mean = nonempty ? (double)c->blocks[0].nnz / (double)nonempty : 0.;
     std::string why;
     if (!level_supported(c->P, c->sgdOrder, &why)) return refuse(why);
-    const int K = level_tasks_width(c->P, c->upd, c->device);
+    int K = level_tasks_width(c->P, c->upd, c->device);
     if (K < 1) return refuse("its d workgroups do not all fit on the device at once");
+    if (const char* e = getenv("DSGDPP_LEVEL_K")) K = std::max(1, std::min(K, atoi(e)));
     if (c->kernel == 0 && mean < LEVEL_MIN_MEAN) return refuse("blocks too small");
     std::vector<int32_t> col0(c->off2.begin(), c->off2.end());
     int32_t* d0 = nullptr;

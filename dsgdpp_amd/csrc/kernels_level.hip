@@ -40,7 +40,12 @@ namespace dsgdpp {
 
 constexpr int LV_THREADS = 1024;   // threads of a prep or epoch workgroup
 constexpr int LV_HIST = 8192;      // levels counted per pass of the prep's LDS histogram
-constexpr int LV_MAX_K = 16;       // workgroups per task of the epoch kernel (half the CUs of an XCD: the look-ahead prep runs beside it)
+// workgroups per task of the epoch kernel, at most: a quarter of an XCD's CUs,
+// so the look-ahead prep runs beside it.  Measured on the quick-start shape
+// (d = 4, FP64_STRICT): 1.66 ms per epoch at 8, 1.73 at 16 (more workgroups
+// meet at every level's barrier), 2.03 at 4 (the widest levels take several
+// passes).
+constexpr int LV_MAX_K = 8;
 static_assert(LV_HIST % LV_THREADS == 0, "every thread scans the same number of counters");
 
 namespace {
