@@ -472,8 +472,9 @@ struct LvArgs {
 
 // Cache policies of the factor accesses: LV_SC1 (agent scope) reaches the
 // device's coherence point, so workgroups on different XCDs see each other's
-// rows; LV_SC0 (workgroup scope) passes the CU's L1 and stops at the XCD's L2,
-// which is coherent for workgroups on one XCD.
+// rows; LV_SC0 (workgroup scope) may be served by the CU's L1 and otherwise by
+// the XCD's L2, which is coherent for workgroups on one XCD (the task's
+// workgroups drop their L1 after every level barrier).
 constexpr int LV_SC1 = 16;
 constexpr int LV_SC0 = 1;
 typedef uint32_t lv_u32x4 __attribute__((ext_vector_type(4)));
@@ -766,6 +767,11 @@ __device__ __forceinline__ void level_task(const LvArgs<T>& A, const int t, cons
             const unsigned long long c2 = prof ? wall_clock64() : 0;
             // the level's stores before the next level's loads, in all K workgroups
             if (!lv_task_barrier(bar, K * ++rounds, A.wait_ticks, A.status, gave_up)) return;
+            // LOCAL: the next level reads rows other workgroups wrote through the
+            // shared L2; workgroup-scope (SC0) loads may be served by this CU's L1,
+            // which may hold an older copy of their lines (a row read or written
+            // here before, or a neighbour on its line), so each wave drops the L1
+            if constexpr (LOCAL) __asm__ __volatile__("buffer_inv sc0" ::: "memory");
             if (prof) {
                 c_lev += c2 - c1;
                 c_bar += wall_clock64() - c2;
