@@ -524,12 +524,13 @@ void finishBlocks(dsgdpp_ctx* c, int which) {
     c->groupedOk = grouped_supported(c->P, c->upd, maxBlock, maxRows, maxCols, c->device, &c->groupedWhy, &wide,
                                      c->kernel == 4);
     c->G.wide = wide;
+    dfree(c->dRid);   // row ranks of the previous training matrix, if any
+    c->G.rid = nullptr;
+    c->G.prep_rows = 0;
     if (c->kernel == 4 && !c->groupedOk) throw Error(DSGDPP_E_UNSUPPORTED, "wide grouped kernel: " + c->groupedWhy);
     choosePairs(c);
     if (c->groupedOk) {
         // the prep's row tags are kept by row rank: one per row with points
-        dfree(c->dRid);
-        c->G.rid = nullptr;
         dalloc(c->dRid, (size_t)std::max<int64_t>(hb.nnz, 1) * sizeof(uint16_t));
         c->G.rid = (const uint16_t*)c->dRid;
         c->G.prep_rows = launch_row_ranks(c->dPts[0], c->fp64(), c->dBlk[0], (int64_t)hb.start.size() - 1,
