@@ -957,6 +957,14 @@ void epoch(dsgdpp_ctx* c, double eps, dsgdpp_epoch_stats* st) {
     if (!P.built || P.dataVersion != c->dataVersion) {
         HIPC(hipStreamSynchronize(c->prepStream));
         buildPlan(c, P, c->prepStream);
+        if (c->kprof) {
+            const auto b1 = std::chrono::steady_clock::now();
+            HIPC(hipEventSynchronize(P.evEnd));
+            const auto b2 = std::chrono::steady_clock::now();
+            fprintf(stderr, "[kprof] plan built in the epoch: draw+host %.2f ms, prep kernels waited %.2f ms\n",
+                    std::chrono::duration<double, std::milli>(b1 - t0).count(),
+                    std::chrono::duration<double, std::milli>(b2 - b1).count());
+        }
     }
     HIPC(hipStreamWaitEvent(c->stream, P.evEnd, 0));
     if (P.dataflow && !c->dReady) dalloc(c->dReady, (size_t)d2 * sizeof(int32_t));
