@@ -2289,14 +2289,16 @@ static void prep_launch_t(const DevProblem& P, const GroupedBuffers& B, const Pe
 
 // Who draws the WOR visit orders: k_permute_lds before the prep, or the prep
 // itself.  Measured on MI355X: the separate kernel shortens a batch's preps
-// (batch64: 224 against 233 ms per epoch), while beside a single problem's
-// epoch kernel the prep's extra resident waves cost that kernel more than the
-// prep gains (100M-nonzero workload: 29.7 against 29.4 ms).  So batches use it
-// and single contexts do not; DSGDPP_PREP_ORDER=1 or 0 sets both.
-static bool orders_first(bool batch) {
+// (batch64: 224 against 233 ms per epoch) and a prep the epoch waits for
+// (100M-nonzero workload: 11.2 against 14.7 ms), while beside a single
+// problem's epoch kernel the prep's extra resident waves cost that kernel more
+// than the prep gains (29.7 against 29.4 ms).  So batches and preps that run
+// alone use it, a context's look-ahead prep does not; DSGDPP_PREP_ORDER=1 or 0
+// sets all.
+static bool orders_first(bool by_default) {
     const char* e = getenv("DSGDPP_PREP_ORDER");
     const int mode = e && (e[0] == '0' || e[0] == '1') ? e[0] - '0' : -1;
-    return mode < 0 ? batch : mode == 1;
+    return mode < 0 ? by_default : mode == 1;
 }
 
 // The LDS of the visit-order kernel for blocks up to cap points, with 16-bit
@@ -2351,7 +2353,7 @@ void launch_prep_grouped(const DevProblem& P, const GroupedBuffers& B, const Per
     const bool narrow = cap < PREP_NARROW;
     // WOR visit orders from the parallel kernel when its LDS allows; the prep
     // then keeps no visit-order array (more of its waves fit on a CU)
-    const bool pre = order == 2 && orders_first(false) && fy_bytes(cap) > 0 &&
+    const bool pre = order == 2 && orders_first(!beside) && fy_bytes(cap) > 0 &&
                      prep_lds(cap, B.prep_rows, B.max_cols, narrow, true) <= PREP_LDS_PER_CU;
     // the round scan where its tables fit beside the state
     const bool rounds = prep_rounds(beside) && prep_lds(cap, B.prep_rows, B.max_cols, narrow, pre, true) <= PREP_LDS_PER_CU;
