@@ -406,6 +406,36 @@ __device__ __forceinline__ double point_lanes_sum(double v) {
 
 constexpr int NZSL_UNROLL = 4;   // points per lane group and pass: their loads overlap
 
+// Four elements of a factor row as stored, widened to double where they are
+// used (holding them in the storage type halves the registers of the loads
+// in flight)
+template <typename T>
+struct Raw4;
+template <>
+struct Raw4<float> {
+    float4 v;
+    __device__ __forceinline__ void load(const float* p) { v = *(const float4*)p; }
+    __device__ __forceinline__ double at(int e) const { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+};
+template <>
+struct Raw4<double> {
+    double2 a, b;
+    __device__ __forceinline__ void load(const double* p) {
+        a = *(const double2*)p;
+        b = *(const double2*)(p + 2);
+    }
+    __device__ __forceinline__ double at(int e) const { return e == 0 ? a.x : e == 1 ? a.y : e == 2 ? b.x : b.y; }
+};
+template <>
+struct Raw4<bf16_t> {
+    uint2 v;
+    __device__ __forceinline__ void load(const bf16_t* p) { v = *(const uint2*)p; }
+    __device__ __forceinline__ double at(int e) const {
+        const uint32_t w = e < 2 ? v.x : v.y;
+        return __uint_as_float((e & 1) ? (w & 0xffff0000u) : (w << 16));
+    }
+};
+
 // squared errors of points first, first + step, ... below end (each point on
 // its LPP lanes, NZSL_UNROLL points per lane group at a time), added to acc
 // on the point's first lane
@@ -430,16 +460,18 @@ __device__ __forceinline__ void nzsl_range(const typename PtOf<T>::type* __restr
 #pragma unroll
         for (int u = 0; u < U; ++u) ip[u] = 0;
         for (int z0 = 4 * sub; z0 < r; z0 += 4 * LPP) {
-            double wv[U][4], hv[U][4];
+            Raw4<T> wv[U], hv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                St4<T>::wide(W + (int64_t)pt[u].i * ld + z0, wv[u]);
-                St4<T>::wide(H + (int64_t)pt[u].j * ld + z0, hv[u]);
+                wv[u].load(W + (int64_t)pt[u].i * ld + z0);
+                hv[u].load(H + (int64_t)pt[u].j * ld + z0);
             }
             const int k = min(4, r - z0);
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                for (int e = 0; e < k; ++e) ip[u] += wv[u][e] * hv[u][e];
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (e < k) ip[u] += wv[u].at(e) * hv[u].at(e);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
