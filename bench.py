@@ -552,6 +552,13 @@ def bench_batch(args):
                 out["sgd_seconds"] += st["sgd_seconds"]
             return out
 
+        def loss(self):   # every problem's training loss, one wait per batch
+            out = [0.0] * len(ctxs)
+            for part, bt in zip(parts, batches):
+                for k, v in zip(part, bt.loss(D.TRAIN)):
+                    out[k] = float(v)
+            return out
+
         def close(self):
             for bt in batches:
                 bt.close()
@@ -559,13 +566,12 @@ def bench_batch(args):
     bt = _All()
     rules = [D.BoldDriver(0.01) for _ in ctxs]
     prev = [None] * len(ctxs)
-    cur = [g.loss(D.TRAIN) for g in ctxs]
+    cur = bt.loss()
 
     def step():
         eps = [r(a, b) for r, a, b in zip(rules, prev, cur)]
         st = bt.epoch(eps)
-        for k, g in enumerate(ctxs):
-            prev[k], cur[k] = cur[k], g.loss(D.TRAIN)
+        prev[:], cur[:] = cur, bt.loss()
         return st
 
     for _ in range(args.warmup):
@@ -578,8 +584,7 @@ def bench_batch(args):
         torch.cuda.synchronize()
         wall += time.perf_counter() - a
         kern += st["sgd_seconds"]
-        for q, g in enumerate(ctxs):
-            prev[q], cur[q] = cur[q], g.loss(D.TRAIN)
+        prev[:], cur[:] = cur, bt.loss()
         log(f"[bench] batch step {k + 1}/{args.steps}: {st['seconds'] * 1e3:.1f} ms (launch {st['sgd_seconds'] * 1e3:.1f} ms)")
     total = sum(nnz)
     print(json.dumps({

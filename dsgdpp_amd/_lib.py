@@ -34,7 +34,7 @@ EXPORTS = (
     "dsgdpp_mlp_destroy", "dsgdpp_mlp_set_params", "dsgdpp_mlp_get_params", "dsgdpp_mlp_forward",
     "dsgdpp_mlp_gradient", "dsgdpp_mlp_train",
     "dsgdpp_batch_create", "dsgdpp_batch_create_error", "dsgdpp_batch_last_error", "dsgdpp_batch_destroy",
-    "dsgdpp_batch_epoch",
+    "dsgdpp_batch_epoch", "dsgdpp_batch_loss",
 )
 
 
@@ -159,6 +159,7 @@ def load():
         "dsgdpp_batch_last_error": (ctypes.c_char_p, [P]),
         "dsgdpp_batch_destroy": (None, [P]),
         "dsgdpp_batch_epoch": (i32, [P, pd, ctypes.POINTER(EpochStats)]),
+        "dsgdpp_batch_loss": (i32, [P, i32, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -52,6 +52,15 @@ class DsgdPpBatch:
             raise RuntimeError(f"dsgdpp_batch_epoch failed ({rc}): {self._lib.dsgdpp_batch_last_error(self._h).decode()}")
         return {"seconds": st.seconds, "sgd_seconds": st.sgd_seconds, "updates": st.updates, "launches": st.launches}
 
+    def loss(self, which=TRAIN):
+        """Every context's loss (what each context's ``loss(which)`` gives),
+        computed on the batch's stream with one wait."""
+        out = np.zeros(len(self.contexts), dtype=np.float64)
+        rc = self._lib.dsgdpp_batch_loss(self._h, which, _lib.ptr(out, ctypes.c_double))
+        if rc != _lib.OK:
+            raise RuntimeError(f"dsgdpp_batch_loss failed ({rc}): {self._lib.dsgdpp_batch_last_error(self._h).decode()}")
+        return out
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.dsgdpp_batch_destroy(self._h)

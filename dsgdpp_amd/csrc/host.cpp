@@ -1967,6 +1967,8 @@ struct dsgdpp_batch {
     int32_t* hStatus = nullptr;   // pinned
     int wallKhz = 0;              // wall-clock ticks per millisecond
     bool counted = false;         // its contexts' batch counts include it
+    double* dLoss = nullptr;      // batch loss: the partial sums, then one total per context
+    double* hLoss = nullptr;      // pinned: the totals
     std::string err;
 };
 
@@ -1984,7 +1986,10 @@ void batchDestroy(dsgdpp_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->hArgs) (void)hipHostFree(b->hArgs);
     if (b->hStatus) (void)hipHostFree(b->hStatus);
+    if (b->hLoss) (void)hipHostFree(b->hLoss);
     void* q = b->dStatus;
+    dfree(q);
+    q = b->dLoss;
     dfree(q);
     q = b->dArgs;
     dfree(q);
@@ -2296,6 +2301,44 @@ const char* dsgdpp_batch_create_error(void) { return g_batchCreateError.c_str();
 const char* dsgdpp_batch_last_error(dsgdpp_batch* b) { return b ? b->err.c_str() : ""; }
 
 void dsgdpp_batch_destroy(dsgdpp_batch* b) { batchDestroy(b); }
+
+// Every context's loss (the value dsgdpp_loss gives), queued on the batch's
+// stream after the work of the contexts' own streams, with one wait.  The
+// partial sums sit in the batch's own buffer, one context after another.
+int dsgdpp_batch_loss(dsgdpp_batch* b, int32_t which, double* out) {
+    if (!b || !out) return DSGDPP_E_ARG;
+    try {
+        if (which != DSGDPP_TRAIN && which != DSGDPP_TEST) throw Error(DSGDPP_E_ARG, "which must be TRAIN or TEST");
+        HIPC(hipSetDevice(b->device));
+        const size_t K = b->ctx.size();
+        const int parts = noalloc_partials();
+        if (!b->dLoss) {
+            dalloc(b->dLoss, ((size_t)parts + 4 + K) * sizeof(double));
+            HIPC(hipHostMalloc((void**)&b->hLoss, K * sizeof(double), hipHostMallocDefault));
+        }
+        double* const totals = b->dLoss + parts + 4;
+        for (size_t k = 0; k < K; ++k) {
+            dsgdpp_ctx* c = b->ctx[k];
+            requireReady(c);
+            if (which == DSGDPP_TEST && !c->blocks[1].loaded) throw Error(DSGDPP_E_STATE, "no test matrix loaded");
+            HIPC(hipEventRecord(c->ev[3], c->stream));   // the context's own earlier work comes first
+            HIPC(hipStreamWaitEvent(b->stream, c->ev[3], 0));
+            const int64_t need = device_loss_noalloc(c->P, which, c->upd.kind, c->lossLambda, b->dLoss, totals + k,
+                                                     b->stream);
+            if (need > 0) throw Error(DSGDPP_E_UNSUPPORTED, "the loss reduction needs more partial sums");
+        }
+        HIPC(hipMemcpyAsync(b->hLoss, totals, K * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIPC(hipStreamSynchronize(b->stream));
+        memcpy(out, b->hLoss, K * sizeof(double));
+        return DSGDPP_OK;
+    } catch (const Error& e) {
+        b->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        b->err = e.what();
+        return DSGDPP_E_DEVICE;
+    }
+}
 
 int dsgdpp_batch_epoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* stats) {
     if (!b || !eps) return DSGDPP_E_ARG;

@@ -249,6 +249,10 @@ const char* dsgdpp_batch_last_error(dsgdpp_batch* b);
 void dsgdpp_batch_destroy(dsgdpp_batch* b);
 /* eps: one step size per context; stats (optional) for the whole batch */
 int dsgdpp_batch_epoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* stats);
+/* out[k] = dsgdpp_loss of context k (TRAIN or TEST), all queued on the
+ * batch's stream with one wait (no counterpart in the reference: it is
+ * SumLoss::operator() (loss.h:99) per problem) */
+int dsgdpp_batch_loss(dsgdpp_batch* b, int32_t which, double* out);
 
 /* ---------------------------------------------------------------------------
  * Step-size controller network (no counterpart in the reference: it extends

@@ -141,6 +141,11 @@ def test_batch_bitexact_vs_oracle(monkeypatch, use_graph, pairs, precision, kern
             np.testing.assert_array_equal(Hb, Ho, err_msg=f"problem {k} epoch {e}")
             lb, lo = batch_ctx[k].loss(0), orcs[k].loss(0)
             assert abs(lb - lo) <= 1e-10 * abs(lo), (k, e, lb, lo)
+    # all contexts' losses in one call: each context's own loss, bit for bit
+    lb = bt.loss(0)
+    assert lb.tolist() == [g.loss(0) for g in batch_ctx]
+    with pytest.raises(RuntimeError, match="no test matrix"):
+        bt.loss(1)
     # the contexts stay usable on their own after batched epochs
     batch_ctx[0].epoch(0.01)
     orcs[0].epoch(0.01)
