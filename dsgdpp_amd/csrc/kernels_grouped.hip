@@ -1301,6 +1301,11 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                 }
                 load_w(qa[(u + GPREF) % GQ], qb[u % GPREF]);
                 qa[u] = load_rec(Gr, rbase + 64 * u);
+                // FP32: each group's instructions stay in its own iteration,
+                // measured faster than letting the scheduler mix neighbouring
+                // groups (303 -> 300 clocks per group on the hottest task,
+                // -0.9 %); BF16 loses 10 % with it
+                if constexpr (std::is_same<T, float>::value) __builtin_amdgcn_sched_barrier(0);
             }
         }
         lds_publish(flags + 1, k);   // the buffer goes back to the mover
@@ -1640,6 +1645,9 @@ __device__ __forceinline__ void grouped_compute_f64(const GArgs64& A, const int 
                 st_d4(hblk_lane + (rec.y << 9), hn);
                 load_w(qa[(u + GPREF64) % GQ64], qb[u % GPREF64], qi[u % GPREF64]);
                 qa[u] = load_rec(Gr, rbase + 64 * u);
+                // each group's instructions stay in its own iteration (measured:
+                // 61.7 -> 58.8 ms per epoch on the 100M-nonzero workload)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         lds_publish(flags + 1, k);
