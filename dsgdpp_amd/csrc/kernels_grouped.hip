@@ -1204,11 +1204,15 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
         const unsigned long long c0 = A.prof ? clock64() : 0;
         lds_await(flags + 0, k);   // the mover has staged this visit's columns
         const unsigned long long c2 = A.prof ? clock64() : 0;
-        float4 hq[VEC], wq[VEC];
+        // ring entries hold W rows as stored (BF16: packed), so the blend with
+        // the prefetched row works on stored words and unpacks once
+        typedef typename Vec4<T>::Raw Raw;
+        float4 hq[VEC];
+        Raw wq[VEC];
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
             hq[v] = *(const float4*)(hblk_lane + (qa[0].y << RSH) + 16 * v);
-            wq[v] = *(const float4*)(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
+            wq[v] = *(const Raw*)(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
         }
         float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
         for (int K0 = 0; K0 < ngp; K0 += GQ) {
@@ -1217,10 +1221,11 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
             for (int u = 0; u < GQ; ++u) {
                 const int K = K0 + u;
                 const uint4 rec = qa[u];
-                float4 w[VEC], h[VEC], fwd[VEC];
+                float4 w[VEC], h[VEC];
+                Raw wsel[VEC], fwd[VEC];
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
-                    w[v] = Vec4<T>::cook(qb[u % GPREF][v]);
+                    wsel[v] = qb[u % GPREF][v];
                     h[v] = hq[v];
                     fwd[v] = wq[v];
                 }
@@ -1234,18 +1239,13 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     hq[v] = *(const float4*)(hblk_lane + (nrec.y << RSH) + 16 * v);
-                    wq[v] = *(const float4*)(nsrc + 4 * v);
+                    wq[v] = *(const Raw*)(nsrc + 4 * v);
                 }
                 if (KIND == UPD_NZSL_L2) ihq = *(const float*)(hinv_b + (nrec.y << 2));
                 {   // W_i from the ring where the prefetch may predate its last update
-                    const uint32_t msk = (uint32_t)((int32_t)rec.x >> 31);   // REC_WFWD
+                    const bool wf = (int32_t)rec.x < 0;   // REC_WFWD
 #pragma unroll
-                    for (int v = 0; v < VEC; ++v) {
-                        w[v].x = __uint_as_float((__float_as_uint(fwd[v].x) & msk) | (__float_as_uint(w[v].x) & ~msk));
-                        w[v].y = __uint_as_float((__float_as_uint(fwd[v].y) & msk) | (__float_as_uint(w[v].y) & ~msk));
-                        w[v].z = __uint_as_float((__float_as_uint(fwd[v].z) & msk) | (__float_as_uint(w[v].z) & ~msk));
-                        w[v].w = __uint_as_float((__float_as_uint(fwd[v].w) & msk) | (__float_as_uint(w[v].w) & ~msk));
-                    }
+                    for (int v = 0; v < VEC; ++v) w[v] = Vec4<T>::cook(wf ? fwd[v] : wsel[v]);
                 }
                 const bool hr = (int32_t)rec.y < 0;   // REC_HREG
 #pragma unroll
@@ -1296,7 +1296,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     Vec4<T>::bstore_raw(Wr, so + 4 * v * ES, wraw[v]);
-                    *(float4*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = wn[v];
+                    *(Raw*)(ring_lane + (K & (GROUP_RING - 1)) * (GROUP_SLOTS * ROW) + 4 * v) = wraw[v];
                     *(float4*)(hblk_lane + (rec.y << RSH) + 16 * v) = hn[v];
                 }
                 load_w(qa[(u + GPREF) % GQ], qb[u % GPREF]);
@@ -1858,6 +1858,10 @@ __device__ __forceinline__ void wide_task(const GArgs<T>& A, const int t, float*
                 load_w(qa[(u + GPREF_W) % GQ_W], qw[u % GPREF_W]);
                 load_h(qa[(u + GPREF_W) % GQ_W], qh[u % GPREF_W], qi[u % GPREF_W]);
                 qa[u] = load_rec(Gr, rbase + 64 * u);
+                // each group's instructions stay in its own iteration (measured:
+                // FP32 47.6 -> 47.2 ms per epoch on the 100M-nonzero workload,
+                // BF16 unchanged)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (A.prof) {
@@ -2107,6 +2111,9 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
                 load_w(qa[(u + GPREF64) % GQ64], qw[u % GPREF64], qwi[u % GPREF64]);
                 load_h(qa[(u + GPREF64) % GQ64], qh[u % GPREF64], qhi[u % GPREF64]);
                 qa[u] = load_rec(Gr, rbase + 64 * u);
+                // each group's instructions stay in its own iteration (measured:
+                // 84.0 -> 82.0 ms per epoch on the 100M-nonzero workload)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (A.prof) {
