@@ -171,8 +171,11 @@ struct EpochPlan {
     std::vector<PermJob> jobs;
     std::vector<int32_t> chains;
     int64_t maxBlock = 0;             // largest block visited
-    int nSmall = 0;                   // grouped prep: leading chains whose blocks are <= PREP_SPLIT points
-    int64_t capSmall = 0;
+    // grouped prep: chains in size classes (largest block of the chain up to
+    // each bound of prep_classes()), one launch per class; class k holds
+    // chains [clsStart[k], clsStart[k + 1]) with blocks of at most clsCap[k]
+    std::vector<int32_t> clsStart;
+    std::vector<int64_t> clsCap;
     uint32_t* dSched = nullptr;
     int64_t* dVoff = nullptr;
     int64_t* dGoff = nullptr;
@@ -263,6 +266,27 @@ struct dsgdpp_ctx {
 namespace {
 
 thread_local std::string g_createError;
+
+// Upper bounds of the grouped prep's size classes (points in a chain's largest
+// block); DSGDPP_PREP_CLASSES="b1,b2,..." (ascending) replaces them.
+const std::vector<int64_t>& prep_classes() {
+    static const std::vector<int64_t> bounds = [] {
+        std::vector<int64_t> b;
+        if (const char* e = getenv("DSGDPP_PREP_CLASSES")) {
+            const char* q = e;
+            while (*q) {
+                char* end = nullptr;
+                const long long v = strtoll(q, &end, 10);
+                if (end == q) break;
+                if (v > 0 && (b.empty() || v > b.back())) b.push_back(v);
+                q = *end == ',' ? end + 1 : end;
+            }
+            return b;
+        }
+        return std::vector<int64_t>(std::begin(PREP_CLASS_BOUNDS), std::end(PREP_CLASS_BOUNDS));
+    }();
+    return bounds;
+}
 
 template <typename F>
 int guarded(dsgdpp_ctx* c, F f) {
@@ -806,27 +830,42 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
                 maxBlock = std::max(maxBlock, j.n);
             }
         chains.push_back((int32_t)jobs.size());
-        P.nSmall = 0;
-        P.capSmall = 0;
-        if (P.grouped && maxBlock > PREP_SPLIT) {
-            // chains of small blocks first: their prep launch needs less LDS per
-            // workgroup, so more of them run at once
+        P.clsStart.assign(1, 0);
+        P.clsCap.clear();
+        if (P.grouped) {
+            // chains of small blocks first, by size class: a class's launches
+            // need LDS for its largest block only, so more of them share a CU
+            const std::vector<int64_t>& bounds = prep_classes();
+            const int ncls = (int)bounds.size() + 1;
+            auto cls_of = [&](int64_t mx) {
+                int k = 0;
+                while (k < ncls - 1 && mx > bounds[k]) ++k;
+                return k;
+            };
+            std::vector<int64_t> mxs(chains.size() - 1);
+            std::vector<int32_t> count(ncls, 0);
+            for (size_t k = 0; k + 1 < chains.size(); ++k) {
+                int64_t mx = 0;
+                for (int32_t q = chains[k]; q < chains[k + 1]; ++q) mx = std::max(mx, jobs[q].n);
+                mxs[k] = mx;
+                ++count[cls_of(mx)];
+            }
             std::vector<PermJob> j2;
             std::vector<int32_t> c2;
             j2.reserve(jobs.size());
             c2.reserve(chains.size());
-            for (int pass = 0; pass < 2; ++pass)
+            for (int cl = 0; cl < ncls; ++cl) {
+                if (count[cl] == 0) continue;
+                int64_t cap = 0;
                 for (size_t k = 0; k + 1 < chains.size(); ++k) {
-                    int64_t mx = 0;
-                    for (int32_t q = chains[k]; q < chains[k + 1]; ++q) mx = std::max(mx, jobs[q].n);
-                    if ((mx <= PREP_SPLIT) != (pass == 0)) continue;
+                    if (cls_of(mxs[k]) != cl) continue;
                     c2.push_back((int32_t)j2.size());
                     for (int32_t q = chains[k]; q < chains[k + 1]; ++q) j2.push_back(jobs[q]);
-                    if (pass == 0) {
-                        ++P.nSmall;
-                        P.capSmall = std::max(P.capSmall, mx);
-                    }
+                    cap = std::max(cap, mxs[k]);
                 }
+                P.clsStart.push_back((int32_t)(c2.size()));
+                P.clsCap.push_back(cap);
+            }
             c2.push_back((int32_t)j2.size());
             jobs.swap(j2);
             chains.swap(c2);
@@ -923,8 +962,10 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
                           c->sgdOrder == DSGDPP_SGD_WOR ? P.perm : nullptr, s);
     } else if (P.grouped) {
         // blocks whose prep state does not fit in LDS keep it in a scratch buffer
-        const size_t spill = std::max(prep_spill_bytes(P.capSmall, G.prep_rows, G.max_cols, P.nSmall),
-                                      prep_spill_bytes(maxBlock, G.prep_rows, G.max_cols, nchains - P.nSmall));
+        size_t spill = 0;
+        for (size_t k = 0; k < P.clsCap.size(); ++k)
+            spill = std::max(spill, prep_spill_bytes(P.clsCap[k], G.prep_rows, G.max_cols,
+                                                     P.clsStart[k + 1] - P.clsStart[k]));
         if (spill > c->G.spill_bytes) {
             HIPC(hipStreamSynchronize(s));   // a prep of the other plan may still use the old one
             void* q = c->G.spill;
@@ -935,10 +976,9 @@ void buildPlan(dsgdpp_ctx* c, EpochPlan& P, hipStream_t s, bool launchPrep = tru
         }
         G.spill = c->G.spill;
         G.spill_bytes = c->G.spill_bytes;
-        if (P.nSmall > 0)
-            launch_prep_grouped(Dp, G, P.dJobs, P.dChains, P.nSmall, c->sgdOrder, P.capSmall, s, ahead);
-        launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.nSmall, nchains - P.nSmall, c->sgdOrder, maxBlock, s,
-                            ahead);
+        for (size_t k = 0; k < P.clsCap.size(); ++k)
+            launch_prep_grouped(Dp, G, P.dJobs, P.dChains + P.clsStart[k], P.clsStart[k + 1] - P.clsStart[k],
+                                c->sgdOrder, P.clsCap[k], s, ahead);
     } else if (c->sgdOrder == DSGDPP_SGD_WOR)
         launch_permute_wor(P.dJobs, P.dChains, nchains, P.perm, maxBlock, s);
     else if (c->sgdOrder == DSGDPP_SGD_WR)
@@ -2143,14 +2183,18 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         GroupedBuffers G = c->G;
         G.grec = N.grec;
         G.ngroups = N.ngroups;
-        const PrepDesc D = prep_desc(Dp, G, N.dJobs, N.dChains, c->sgdOrder, N.maxBlock);
-        const int idx = (int)b->desc.size();
-        b->desc.push_back(D);
-        const size_t l = prep_desc_lds(D);
-        PrepClass& pc = classes[{prep_narrow(N.maxBlock) ? 1 : 0, (int)std::max<size_t>(1, PREP_LDS_PER_CU / l)}];
-        for (int ch = 0; ch + 1 < (int)N.chains.size(); ++ch) pc.chains.push_back(make_int2(idx, ch));
-        pc.lds = std::max(pc.lds, l);
-        pc.order_lds = std::max(pc.order_lds, prep_desc_order_lds(D));
+        // one descriptor per size class of the problem's chains
+        for (size_t q = 0; q < N.clsCap.size(); ++q) {
+            const PrepDesc D = prep_desc(Dp, G, N.dJobs, N.dChains, c->sgdOrder, N.clsCap[q]);
+            const int idx = (int)b->desc.size();
+            b->desc.push_back(D);
+            const size_t l = prep_desc_lds(D);
+            PrepClass& pc =
+                classes[{prep_narrow(N.clsCap[q]) ? 1 : 0, (int)std::max<size_t>(1, PREP_LDS_PER_CU / l)}];
+            for (int ch = N.clsStart[q]; ch < N.clsStart[q + 1]; ++ch) pc.chains.push_back(make_int2(idx, ch));
+            pc.lds = std::max(pc.lds, l);
+            pc.order_lds = std::max(pc.order_lds, prep_desc_order_lds(D));
+        }
         ahead.push_back(&N);
     }
     if (!ahead.empty()) {

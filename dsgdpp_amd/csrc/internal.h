@@ -140,7 +140,12 @@ void launch_epoch_levels(const DevProblem& P, const UpdParams& U, const LevelBuf
 constexpr int GROUP_SLOTS = 4;
 constexpr int GROUP_RING = 16;    // W forwarding window, in groups (power of two)
 constexpr int GROUP_PAD = 32;     // every visit's group list is padded to a multiple of this
-constexpr int64_t PREP_SPLIT = 4096;   // grouped prep: blocks up to this size run in a launch of their own
+// grouped prep: chains whose largest block has up to 1024, 2048, 4096 points,
+// and the rest, run in launches of their own (LDS sized per launch, so the
+// small ones share a CU with more of their kind).  Measured on the 100M-
+// nonzero workload, prep alone: 11.1 ms per epoch with one bound (4096),
+// 9.0 with {1024, 4096}, 8.9 with these, 9.1 with {512, 1024, 2048, 4096}.
+constexpr int64_t PREP_CLASS_BOUNDS[] = {1024, 2048, 4096};
 constexpr size_t PREP_LDS_PER_CU = 160 * 1024;   // LDS of a CU, shared by the prep workgroups on it
 // Factor buffers carry FACTOR_TAIL_ROWS spare rows after the last one: the
 // first half stays zero (read by lanes past the row length), the second half

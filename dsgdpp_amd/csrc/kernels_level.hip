@@ -313,7 +313,6 @@ __global__ __launch_bounds__(LV_THREADS) void k_level_prep(LevelPrepArgs A) {
     const PermJob J = A.jobs[blockIdx.x];
     const int tid = threadIdx.x;
     const int64_t n = J.n, base = J.base, s0 = J.s0;
-    const Pt32* pts = (const Pt32*)A.pts;   // .i, .j only: Pt64 and Pt32 share those offsets
     uint32_t* const pos = A.pos + base;
     int32_t* const lev = A.lev + base;
     int32_t* const prv = A.prv + 2 * base;
