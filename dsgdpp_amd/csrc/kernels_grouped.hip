@@ -1215,10 +1215,17 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
             wq[v] = *(const Raw*)(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
         }
         float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
-        for (int K0 = 0; K0 < ngp; K0 += GQ) {
+        // 32 groups per chunk; the last chunk stops at the visit's last group
+        // (tail), so the prep's padding groups are never run (measured: BF16
+        // 33.7 -> 33.1 ms per epoch on the 100M-nonzero workload, FP32 the
+        // same; the FP64 loop lost 2 % with it and runs its padding)
+        auto chunk = [&](const int K0, auto tail) {
             const int rbase = (int)rec_off + 64 * (K0 + GQ);   // < 2^31: a visit has < 2^22 groups
 #pragma unroll
             for (int u = 0; u < GQ; ++u) {
+                if constexpr (decltype(tail)::value) {
+                    if (K0 + u >= ng) break;
+                }
                 const int K = K0 + u;
                 const uint4 rec = qa[u];
                 float4 w[VEC], h[VEC];
@@ -1307,7 +1314,10 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
                 // -0.9 %); BF16 loses 10 % with it
                 if constexpr (std::is_same<T, float>::value) __builtin_amdgcn_sched_barrier(0);
             }
-        }
+        };
+        int K0 = 0;
+        for (; K0 + GQ < ngp; K0 += GQ) chunk(K0, std::false_type{});
+        chunk(K0, std::true_type{});
         lds_publish(flags + 1, k);   // the buffer goes back to the mover
         ++k;
         if (A.prof) {
@@ -1774,10 +1784,15 @@ __device__ __forceinline__ void wide_task(const GArgs<T>& A, const int t, float*
             wq[v] = *(const float4*)(wring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
             hq[v] = *(const float4*)(hring + ((qa[0].y >> REC_HRING_SHIFT) & REC_RING_MASK) * ROW + e0 + 4 * v);
         }
-        for (int K0 = 0; K0 < ngp; K0 += GQ_W) {
+        // the last chunk stops at the visit's last group (measured: FP32
+        // 47.6 -> 44.7 ms per epoch on the 100M-nonzero workload)
+        auto chunk = [&](const int K0, auto tail) {
             const int rbase = (int)rec_off + 64 * (K0 + GQ_W);
 #pragma unroll
             for (int u = 0; u < GQ_W; ++u) {
+                if constexpr (decltype(tail)::value) {
+                    if (K0 + u >= ng) break;   // the prep's padding groups are not run
+                }
                 const int K = K0 + u;
                 const uint4 rec = qa[u];
                 float4 w[VEC], h[VEC], fw[VEC], fh[VEC];
@@ -1863,7 +1878,10 @@ __device__ __forceinline__ void wide_task(const GArgs<T>& A, const int t, float*
                 // BF16 unchanged)
                 __builtin_amdgcn_sched_barrier(0);
             }
-        }
+        };
+        int K0 = 0;
+        for (; K0 + GQ_W < ngp; K0 += GQ_W) chunk(K0, std::false_type{});
+        chunk(K0, std::true_type{});
         if (A.prof) {
             const unsigned long long c3 = clock64();
             c_wait += c2 - c0;
@@ -1992,10 +2010,15 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
             wq[v] = ld_d4(wring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * RB + e0 * 8 + 32 * v);
             hq[v] = ld_d4(hring + ((qa[0].y >> REC_HRING_SHIFT) & REC_RING_MASK) * RB + e0 * 8 + 32 * v);
         }
-        for (int K0 = 0; K0 < ngp; K0 += GQ64) {
+        // the last chunk stops at the visit's last group (measured: 81.8 ->
+        // 78.5 ms per epoch on the 100M-nonzero workload)
+        auto chunk = [&](const int K0, auto tail) {
             const int rbase = (int)rec_off + 64 * (K0 + GQ64);
 #pragma unroll
             for (int u = 0; u < GQ64; ++u) {
+                if constexpr (decltype(tail)::value) {
+                    if (K0 + u >= ng) break;   // the prep's padding groups are not run
+                }
                 const int K = K0 + u;
                 const uint4 rec = qa[u];
                 D4 w[V], h[V], fw[V], fh[V];
@@ -2115,7 +2138,10 @@ __global__ __launch_bounds__(64) void k_epoch_wide64(GArgs64 A) {
                 // 84.0 -> 82.0 ms per epoch on the 100M-nonzero workload)
                 __builtin_amdgcn_sched_barrier(0);
             }
-        }
+        };
+        int K0 = 0;
+        for (; K0 + GQ64 < ngp; K0 += GQ64) chunk(K0, std::false_type{});
+        chunk(K0, std::true_type{});
         if (A.prof) {
             const unsigned long long c3 = clock64();
             c_wait += c2 - c0;
