@@ -1033,8 +1033,8 @@ __device__ __forceinline__ float pair_sum(float v) {
 // columns (0 without profiling).
 template <int KIND, bool TRUNC>
 __device__ __forceinline__ unsigned long long pair_groups(const GArgs<float>& A, const __amdgpu_buffer_rsrc_t Wr,
-                                                          const __amdgpu_buffer_rsrc_t Gr, const int ngp,
-                                                          float* const ring, const float* const hbp,
+                                                          const __amdgpu_buffer_rsrc_t Gr, const int ng,
+                                                          const int ngp, float* const ring, const float* const hbp,
                                                           int32_t* const flags, const int k) {
     constexpr int ROW = 64;    // floats per staged row
     constexpr int GPR = 16;    // W prefetch distance (groups), <= GROUP_RING
@@ -1070,10 +1070,14 @@ __device__ __forceinline__ unsigned long long pair_groups(const GArgs<float>& A,
     float2 wq = *(const float2*)(ring + ((qa[0].x >> REC_RING_SHIFT) & REC_RING_MASK) * ROW + e0);
     float ihq = KIND == UPD_NZSL_L2 ? *(const float*)(hinv_b + (qa[0].y << 2)) : 0.f;
     float2 hprev = make_float2(0.f, 0.f);
-    for (int K0 = 0; K0 < ngp; K0 += GQ) {
+    // the last chunk stops at the visit's last group
+    auto chunk = [&](const int K0, auto tail) {
         const int rbase = (int)rec_off + 64 * (K0 + GQ);
 #pragma unroll
         for (int u = 0; u < GQ; ++u) {
+            if constexpr (decltype(tail)::value) {
+                if (K0 + u >= ng) break;   // the prep's padding groups are not run
+            }
             const int K = K0 + u;
             const uint4 rec = qa[u];
             float2 w = make_float2(__uint_as_float(qb[u % GPR].x), __uint_as_float(qb[u % GPR].y));
@@ -1124,7 +1128,10 @@ __device__ __forceinline__ unsigned long long pair_groups(const GArgs<float>& A,
             load_w(qa[(u + GPR) % GQ], qb[u % GPR]);
             qa[u] = load_rec(Gr, rbase + 64 * u);
         }
-    }
+    };
+    int K0 = 0;
+    for (; K0 + GQ < ngp; K0 += GQ) chunk(K0, std::false_type{});
+    chunk(K0, std::true_type{});
     return waited;
 }
 
@@ -1176,7 +1183,7 @@ __device__ __forceinline__ void grouped_compute(const GArgs<T>& A, const int t, 
         if constexpr (VEC == 1 && std::is_same<T, float>::value) {
             if (A.pairs && uniform_i32((int)A.pairs[b])) {   // a two-slot visit
                 const unsigned long long cv = A.prof ? clock64() : 0;
-                const unsigned long long waited = pair_groups<KIND, TRUNC>(A, Wr, Gr, ngp, ring, hbp, flags, k);
+                const unsigned long long waited = pair_groups<KIND, TRUNC>(A, Wr, Gr, ng, ngp, ring, hbp, flags, k);
                 lds_publish(flags + 1, k);   // the buffer goes back to the mover
                 ++k;
                 if (A.prof) {
