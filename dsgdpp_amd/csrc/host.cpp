@@ -7,13 +7,16 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
@@ -48,13 +51,45 @@ int hostThreads() {
     return (int)std::max(1u, std::min(h, 16u));
 }
 
+// set in the workers of parallelForEach: work they call runs on that thread
+thread_local bool tl_inWorker = false;
+
 template <typename F>
 void parallelFor(int64_t n, F f) {
     int th = (int)std::min<int64_t>(hostThreads(), std::max<int64_t>(n, 1));
-    if (th <= 1) { f(0, n); return; }
+    if (th <= 1 || tl_inWorker) { f(0, n); return; }
     std::vector<std::thread> ts;
     for (int k = 0; k < th; ++k) ts.emplace_back(f, n * k / th, n * (k + 1) / th);
     for (auto& t : ts) t.join();
+}
+
+// f(i) for i in [0, n) on the host threads, items taken in order from a shared
+// counter; the first exception a worker caught is rethrown after all joined
+template <typename F>
+void parallelForEach(int64_t n, F f) {
+    const int th = (int)std::min<int64_t>(hostThreads(), std::max<int64_t>(n, 1));
+    if (th <= 1 || tl_inWorker) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    std::vector<std::thread> ts;
+    for (int k = 0; k < th; ++k)
+        ts.emplace_back([&] {
+            tl_inWorker = true;
+            for (int64_t i; (i = next.fetch_add(1)) < n;) {
+                try {
+                    f(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (!err) err = std::current_exception();
+                }
+            }
+        });
+    for (auto& t : ts) t.join();
+    if (err) std::rethrow_exception(err);
 }
 
 // ---------------------------------------------------------------------------
@@ -2155,8 +2190,8 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
     HIPC(hipEventRecord(b->ev1, b->stream));
     const auto t_launch = std::chrono::steady_clock::now();
     // every context's next plan is prepared while the batch runs: the host
-    // parts one by one, the grouped preps of all of them in one launch per
-    // index width
+    // parts on the host threads, the grouped preps of all of them in one
+    // launch per class of prep LDS
     HIPC(hipStreamWaitEvent(b->prepStream, b->ev0, 0));
     b->desc.clear();
     b->cmap.clear();
@@ -2174,10 +2209,20 @@ void batchEpoch(dsgdpp_batch* b, const double* eps, dsgdpp_epoch_stats* st) {
         c->lastPlan = c->cur;
         c->plan[c->cur].drawn = false;
         c->cur ^= 1;
+    }
+    // the contexts' plans are independent: drawn and built on the host threads
+    // (one context per thread at a time; their copies go to the prep stream)
+    parallelForEach((int64_t)K, [&](int64_t k) {
+        dsgdpp_ctx* c = b->ctx[k];
+        if (!c->lookahead) return;
+        HIPC(hipSetDevice(b->device));
+        drawPlan(c, c->plan[c->cur]);
+        buildPlan(c, c->plan[c->cur], b->prepStream, false);
+    });
+    for (size_t k = 0; k < K; ++k) {
+        dsgdpp_ctx* c = b->ctx[k];
         if (!c->lookahead) continue;
         EpochPlan& N = c->plan[c->cur];
-        drawPlan(c, N);
-        buildPlan(c, N, b->prepStream, false);
         DevProblem Dp = c->P;
         Dp.perm = N.perm;
         GroupedBuffers G = c->G;
