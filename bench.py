@@ -30,6 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# The FP32 grouped kernel's cost per link of a column's update chain: clocks per
+# four-slot group on the hottest task (DSGDPP_KPROF clock log of the 100M-nonzero
+# workload), at the 2.4 GHz shader clock.
+CHAIN_CLOCKS_PER_LINK = 300
+CLOCK_GHZ = 2.4
 
 
 def log(*a):
@@ -158,6 +163,16 @@ def main():
     avg_launch_s = sgd_s / launches
     achieved = per_launch_bytes / avg_launch_s / 1e9
 
+    # the hottest column's updates run one after another: the chain's length
+    # times the cost per link bounds the FP32 grouped kernel from below
+    chain = None
+    if stats[-1]["kernel"] == 3 and precision == "FP32":
+        import numpy as np
+        hot = int(np.bincount(np.asarray(p["train"][1], dtype=np.int64)).max())
+        chain_ms = hot * CHAIN_CLOCKS_PER_LINK / (CLOCK_GHZ * 1e6)
+        chain = {"hot_column_updates": hot, "clocks_per_link": CHAIN_CLOCKS_PER_LINK, "clock_ghz": CLOCK_GHZ,
+                 "ms": chain_ms, "frac_ceiling": per_launch_bytes / (chain_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
     ttt = None
     if name == "zipf100m" and not args.no_time_to_target:
         ttt = time_to_target(p, update, decay0, tasks, epochs=args.epochs or 30)
@@ -197,6 +212,7 @@ def main():
             "bytes_per_update": b_upd,
             "avg_launch_us": avg_launch_s * 1e6,
         },
+        "chain_bound": chain,
         "epoch_breakdown_ms": {
             "perm": sum(s["perm_seconds"] for s in stats) / args.steps * 1e3,
             "strata": sgd_s / args.steps * 1e3,
